@@ -1,0 +1,272 @@
+// sort.hip -- stable LSD radix sort of (u32 key, u32 value) pairs for gfx950.
+//
+// Replaces the reference's two sorts on the hot path: the recursive ray sort
+// (src/core/RaySort.cpp:7-118, driven from Pathtracer.cpp:321-327) and the hit-point sort
+// (tbb::parallel_sort with CompareHit, Pathtracer.cpp:335-339). Both become one stable
+// radix sort of 32-bit keys; the permutation it returns is bit-exact against a stable sort
+// of the same keys.
+//
+// Design (onesweep, Adinets & Merrill 2022, restated for CDNA4 wave64):
+//   1. one read of the keys builds the 256-bin histograms of every 8-bit digit position;
+//   2. one kernel per digit: each workgroup takes a 4096-key tile (ticketed in launch order),
+//      ranks keys stably inside each wave with 8 ballots per key (a 64-lane match-any),
+//      publishes its per-digit counts, resolves its global offsets by decoupled look-back over
+//      earlier tiles, stages the tile in LDS in digit order and writes runs of equal digits
+//      contiguously.
+// Look-back status words are 64-bit: [63:32] epoch tag of the pass, [31:30] flag
+// (1 aggregate, 2 inclusive prefix), [29:0] count. Words from older passes carry an older
+// epoch and read as "not ready", so the status buffer never needs clearing between passes.
+#include "swr_internal.h"
+
+namespace swr {
+namespace {
+
+constexpr int kRadix = 256;
+constexpr int kWaves = kSortBlock / 64;
+constexpr int kKpt = kSortKeysPerThread;
+constexpr int kTile = kSortTile;
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagPre = 2u << 30;
+constexpr uint32_t kCountMask = (1u << 30) - 1;
+
+__global__ void sort_prepare_kernel(uint32_t* hist, uint32_t* counters) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) hist[p * kRadix + t] = 0;
+  if (t < 4) counters[t] = 0;
+}
+
+// Histograms of all digit positions in one pass over the keys.
+__global__ __launch_bounds__(256) void sort_histogram_kernel(const uint32_t* __restrict__ keys, size_t n, int passes,
+                                                             uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_hist[4 * kRadix];
+  for (int i = threadIdx.x; i < 4 * kRadix; i += blockDim.x) s_hist[i] = 0;
+  __syncthreads();
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t k = keys[i];
+    for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p * kRadix + ((k >> (8 * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x) {
+    const uint32_t v = s_hist[i];
+    if (v) atomicAdd(&hist[i], v);
+  }
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_tmp[w] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i)
+    if (i < w) pre += s_tmp[i];
+  __syncthreads();
+  return pre + x - v;
+}
+
+template <bool kIota>
+__global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __restrict__ keys_in,
+                                                              const uint32_t* __restrict__ vals_in,
+                                                              uint32_t* __restrict__ keys_out,
+                                                              uint32_t* __restrict__ vals_out, uint32_t n, int shift,
+                                                              const uint32_t* __restrict__ hist,
+                                                              uint32_t* __restrict__ counter,
+                                                              uint64_t* __restrict__ status, uint32_t epoch) {
+  __shared__ uint32_t s_buf[kTile];
+  __shared__ uint32_t s_wave[kWaves][kRadix];
+  __shared__ uint32_t s_start[kRadix];
+  __shared__ uint32_t s_global[kRadix];
+  __shared__ uint32_t s_tmp[kWaves];
+  __shared__ uint32_t s_tile;
+
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) s_tile = atomicAdd(counter, 1u);
+  for (int i = t; i < kWaves * kRadix; i += kSortBlock) (&s_wave[0][0])[i] = 0;
+  // Global digit bases of this pass: exclusive scan of the histogram.
+  const uint32_t digit_base = block_exclusive_scan(hist[t], s_tmp);
+  const uint32_t tile = s_tile;  // ordered by the barrier inside the scan
+  const uint32_t tile_base = tile * (uint32_t)kTile;
+  const uint32_t valid = min((uint32_t)kTile, n - tile_base);
+  const uint32_t wbase = (uint32_t)w * 64u * kKpt;
+
+  uint32_t key[kKpt], val[kKpt], rank[kKpt];
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    key[i] = p < valid ? keys_in[tile_base + p] : 0xFFFFFFFFu;
+  }
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    if (kIota)
+      val[i] = tile_base + p;
+    else
+      val[i] = p < valid ? vals_in[tile_base + p] : 0u;
+  }
+
+  // Stable rank of every key among equal digits of its wave, in original order.
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    const bool ok = p < valid;
+    const uint32_t d = (key[i] >> shift) & 255u;
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const int leader = peers ? (__ffsll((unsigned long long)peers) - 1) : 0;
+    uint32_t old = 0;
+    if (ok && lane == leader) {
+      old = s_wave[w][d];
+      s_wave[w][d] = old + (uint32_t)__popcll(peers);
+    }
+    old = __shfl(old, leader, 64);
+    rank[i] = old + (uint32_t)__popcll(peers & lt_mask);
+  }
+  __syncthreads();
+
+  // Per-digit wave prefixes and tile totals (thread t owns digit t).
+  uint32_t total = 0;
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) {
+    const uint32_t c = s_wave[ww][t];
+    s_wave[ww][t] = total;
+    total += c;
+  }
+  const uint64_t tag = (uint64_t)epoch << 32;
+  uint64_t* my_status = status + (size_t)tile * kRadix + t;
+  __hip_atomic_store(my_status, tag | (uint64_t)((tile == 0 ? kFlagPre : kFlagAgg) | total), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t start = block_exclusive_scan(total, s_tmp);
+  s_start[t] = start;
+
+  // Decoupled look-back over earlier tiles for digit t.
+  uint32_t excl = 0;
+  if (tile > 0) {
+    int pred = (int)tile - 1;
+    for (;;) {
+      const uint64_t v = __hip_atomic_load(status + (size_t)pred * kRadix + t, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t lo = (uint32_t)v;
+      const uint32_t flag = lo & ~kCountMask;
+      if ((uint32_t)(v >> 32) != epoch || flag == 0) {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += lo & kCountMask;
+      if (flag == kFlagPre) break;
+      --pred;
+    }
+    __hip_atomic_store(my_status, tag | (uint64_t)(kFlagPre | (excl + total)), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  s_global[t] = digit_base + excl - start;  // wraps; start <= tile position of the digit's keys
+  __syncthreads();
+
+  // Stage keys in digit order.
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    if (p < valid) {
+      const uint32_t d = (key[i] >> shift) & 255u;
+      const uint32_t pos = s_start[d] + s_wave[w][d] + rank[i];
+      s_buf[pos] = key[i];
+      rank[i] = pos;
+    }
+  }
+  __syncthreads();
+  uint32_t dest[kKpt];
+#pragma unroll
+  for (int j = 0; j < kKpt; ++j) {
+    const uint32_t p = j * kSortBlock + t;
+    if (p < valid) {
+      const uint32_t k = s_buf[p];
+      dest[j] = s_global[(k >> shift) & 255u] + p;
+      keys_out[dest[j]] = k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    if (p < valid) s_buf[rank[i]] = val[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kKpt; ++j) {
+    const uint32_t p = j * kSortBlock + t;
+    if (p < valid) vals_out[dest[j]] = s_buf[p];
+  }
+}
+
+}  // namespace
+
+size_t sort_status_bytes(size_t n) {
+  const size_t tiles = (n + kTile - 1) / kTile;
+  return (tiles ? tiles : 1) * kRadix * sizeof(uint64_t);
+}
+
+void sort_reserve(swr_context* ctx, size_t n) {
+  SortWorkspace& ws = ctx->sort;
+  ensure(ctx, ws.hist, 4 * kRadix * sizeof(uint32_t));
+  ensure(ctx, ws.counters, 4 * sizeof(uint32_t));
+  const size_t need = sort_status_bytes(n);
+  if (ws.status.bytes < need) {
+    ensure(ctx, ws.status, need);
+    // Fresh memory may hold any bit pattern, including the current epoch tag.
+    SWR_HIP(hipMemset(ws.status.ptr, 0, ws.status.bytes));
+    ws.epoch = 0;
+  }
+}
+
+void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals, size_t n,
+                int key_bits, bool iota, uint32_t** out_keys, uint32_t** out_vals, hipStream_t stream) {
+  if (key_bits < 1 || key_bits > 32) fail(SWR_E_INVALID, "sort: key_bits must be in 1..32");
+  if (n > (size_t(1) << 30)) fail(SWR_E_INVALID, "sort: at most 2^30 elements");
+  *out_keys = keys;
+  *out_vals = vals;
+  if (n == 0) return;
+  sort_reserve(ctx, n);
+  SortWorkspace& ws = ctx->sort;
+  uint32_t* hist = static_cast<uint32_t*>(ws.hist.ptr);
+  uint32_t* counters = static_cast<uint32_t*>(ws.counters.ptr);
+  uint64_t* status = static_cast<uint64_t*>(ws.status.ptr);
+  const int passes = (key_bits + 7) / 8;
+  sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters);
+  const int hist_blocks = (int)std::min<size_t>((n + 4095) / 4096, (size_t)std::max(ctx->num_cus, 1) * 8);
+  sort_histogram_kernel<<<hist_blocks, 256, 0, stream>>>(keys, n, passes, hist);
+  const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
+  uint32_t *sk = keys, *sv = vals, *dk = alt_keys, *dv = alt_vals;
+  for (int p = 0; p < passes; ++p) {
+    if (++ws.epoch == 0) {  // tag space exhausted: start over from a cleared buffer
+      SWR_HIP(hipMemsetAsync(ws.status.ptr, 0, ws.status.bytes, stream));
+      ws.epoch = 1;
+    }
+    if (iota && p == 0)
+      onesweep_kernel<true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, 8 * p, hist + p * kRadix,
+                                                             counters + p, status, ws.epoch);
+    else
+      onesweep_kernel<false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, 8 * p, hist + p * kRadix,
+                                                              counters + p, status, ws.epoch);
+    std::swap(sk, dk);
+    std::swap(sv, dv);
+  }
+  SWR_HIP(hipGetLastError());
+  *out_keys = sk;
+  *out_vals = sv;
+}
+
+}  // namespace swr

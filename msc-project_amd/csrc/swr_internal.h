@@ -1,0 +1,105 @@
+// swr_internal.h -- host-side internals shared by the libswr translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "swr.h"
+
+namespace swr {
+
+// Raised inside the library and turned into a status code at the C boundary.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+#define SWR_HIP(call)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      ::swr::fail(SWR_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));            \
+  } while (0)
+
+// Device buffer owned by a context (accounted against its budget).
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Workspace of the onesweep radix sort.
+struct SortWorkspace {
+  DeviceBuffer alt_keys, alt_vals;   // ping-pong partners of the caller's buffers
+  DeviceBuffer hist;                 // 4 passes x 256 digits
+  DeviceBuffer counters;             // 4 tile tickets
+  DeviceBuffer status;               // look-back words, tiles x 256 x u64
+  uint32_t epoch = 0;                // look-back tag, bumped per pass
+};
+
+}  // namespace swr
+
+struct swr_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::atomic<bool> cancel{false};
+  uint64_t budget = 0;               // 0 = unlimited
+  uint64_t in_use = 0;
+  std::unordered_map<void*, size_t> allocs;
+  swr::SortWorkspace sort;
+  int num_cus = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // brackets of the last timed entry point
+  double last_ms = 0.0;                     // its device time
+};
+
+namespace swr {
+
+// Device time of one entry point: events on the context stream around its launches.
+struct Timer {
+  swr_context* ctx;
+  explicit Timer(swr_context* c) : ctx(c) { SWR_HIP(hipEventRecord(ctx->ev0, ctx->stream)); }
+  void stop() { SWR_HIP(hipEventRecord(ctx->ev1, ctx->stream)); }
+  void finish() {
+    float ms = 0.f;
+    SWR_HIP(hipEventSynchronize(ctx->ev1));
+    SWR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+  }
+};
+
+// Allocation accounted against the context budget.
+void* dev_alloc(swr_context* ctx, size_t bytes);
+void dev_free(swr_context* ctx, void* p);
+// Grow a context-owned buffer to at least bytes (contents not preserved).
+void ensure(swr_context* ctx, DeviceBuffer& b, size_t bytes);
+void release(swr_context* ctx, DeviceBuffer& b);
+
+// ---- sort.hip ----
+// Stable sort of n (key, value) pairs on key bits [0, key_bits). keys/vals hold the input; the
+// sorted output lands in whichever of (keys, vals) or (alt_keys, alt_vals) the last pass wrote,
+// returned through out_keys / out_vals. iota: values are 0..n-1 (vals is not read).
+void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals, size_t n,
+                int key_bits, bool iota, uint32_t** out_keys, uint32_t** out_vals, hipStream_t stream);
+// Scratch sizes the sort needs for n elements.
+size_t sort_status_bytes(size_t n);
+void sort_reserve(swr_context* ctx, size_t n);
+constexpr int kSortBlock = 256;
+constexpr int kSortKeysPerThread = 16;
+constexpr int kSortTile = kSortBlock * kSortKeysPerThread;
+
+// ---- keys.hip ----
+void launch_ray_keys(const float4* rays, size_t n, float3 lo, float3 scale, uint32_t* keys, hipStream_t s);
+void launch_hit_keys(const uint4* hits, size_t n, uint32_t* keys, hipStream_t s);
+void launch_mesh_face_keys(const uint32_t* mesh, const uint32_t* face, size_t n, uint32_t* keys, hipStream_t s);
+void launch_gather_u32(const uint32_t* in, const uint32_t* perm, size_t n, int words, uint32_t* out, hipStream_t s);
+
+}  // namespace swr
