@@ -1,0 +1,396 @@
+// bvh.hip -- GPU-built LBVH collapsed to a BVH4 (gfx950).
+//
+// Replaces Embree's acceleration build, rtcCommit (src/core/Pathtracer.cpp:274), over the
+// triangle meshes the reference registers per object and light (:178-201, :242-265).
+// Steps (Karras, "Maximizing parallelism in the construction of BVHs, octrees and k-d trees",
+// HPG 2012, plus a greedy top-down BVH2 -> BVH4 collapse):
+//   1. 30-bit Morton code of every triangle centroid over the centroid bounds;
+//   2. stable radix sort of the codes (sort.hip), ties broken by triangle index;
+//   3. one thread per internal node finds its key range and split -> binary radix tree;
+//   4. leaves walk up with per-node arrival counters, the second arrival unions the boxes;
+//   5. level by level, each BVH4 node adopts up to four BVH2 descendants (largest box area
+//      expanded first); subtrees of <= leaf_max triangles become leaves, which are contiguous
+//      runs of the sorted triangle order;
+//   6. the leaf triangles are written in sorted order as (v0|id, e1|mask, e2).
+#include <algorithm>
+#include <cfloat>
+
+#include "scene.h"
+
+namespace swr {
+namespace {
+
+constexpr uint32_t kBvh2Leaf = 0x80000000u;
+
+__device__ inline uint32_t spread10(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ inline void tri_verts(const float4* pos, uint4 t, float3* a, float3* b, float3* c) {
+  const float4 p0 = pos[t.x], p1 = pos[t.y], p2 = pos[t.z];
+  *a = make_float3(p0.x, p0.y, p0.z);
+  *b = make_float3(p1.x, p1.y, p1.z);
+  *c = make_float3(p2.x, p2.y, p2.z);
+}
+
+__global__ void morton_kernel(const float4* __restrict__ pos, const uint4* __restrict__ tri, uint32_t n, float3 lo,
+                              float3 inv_ext, uint32_t* __restrict__ codes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float3 a, b, c;
+  tri_verts(pos, tri[i], &a, &b, &c);
+  const float cx = (a.x + b.x + c.x) * (1.f / 3.f), cy = (a.y + b.y + c.y) * (1.f / 3.f),
+              cz = (a.z + b.z + c.z) * (1.f / 3.f);
+  const uint32_t qx = (uint32_t)fminf(fmaxf((cx - lo.x) * inv_ext.x * 1024.f, 0.f), 1023.f);
+  const uint32_t qy = (uint32_t)fminf(fmaxf((cy - lo.y) * inv_ext.y * 1024.f, 0.f), 1023.f);
+  const uint32_t qz = (uint32_t)fminf(fmaxf((cz - lo.z) * inv_ext.z * 1024.f, 0.f), 1023.f);
+  codes[i] = (spread10(qx) << 2) | (spread10(qy) << 1) | spread10(qz);
+}
+
+// Length of the common prefix of sorted keys i and j, with the index as tie breaker.
+__device__ inline int delta(const uint32_t* __restrict__ k, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = k[i], b = k[j];
+  if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+  return __clz(a ^ b);
+}
+
+// Internal node i: children (bit 31 = BVH2 leaf) and the sorted-primitive range it covers.
+__global__ void karras_kernel(const uint32_t* __restrict__ keys, int n, uint2* __restrict__ children,
+                              uint2* __restrict__ range, uint32_t* __restrict__ parent_internal,
+                              uint32_t* __restrict__ parent_leaf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int s = 0;
+  int t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int gamma = i + s * d + min(d, 0);
+  const int first = min(i, j), last = max(i, j);
+  uint32_t left, right;
+  if (first == gamma) {
+    left = kBvh2Leaf | (uint32_t)gamma;
+    parent_leaf[gamma] = (uint32_t)i;
+  } else {
+    left = (uint32_t)gamma;
+    parent_internal[gamma] = (uint32_t)i;
+  }
+  if (last == gamma + 1) {
+    right = kBvh2Leaf | (uint32_t)(gamma + 1);
+    parent_leaf[gamma + 1] = (uint32_t)i;
+  } else {
+    right = (uint32_t)(gamma + 1);
+    parent_internal[gamma + 1] = (uint32_t)i;
+  }
+  children[i] = make_uint2(left, right);
+  range[i] = make_uint2((uint32_t)first, (uint32_t)last);
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Leaf boxes, then unions up the tree: the second child to arrive at a node computes its box.
+__global__ void refit_kernel(const float4* __restrict__ pos, const uint4* __restrict__ tri,
+                             const uint32_t* __restrict__ perm, int n, const uint2* __restrict__ children,
+                             const uint32_t* __restrict__ parent_internal, const uint32_t* __restrict__ parent_leaf,
+                             uint32_t* __restrict__ visits, float4* __restrict__ leaf_lo, float4* __restrict__ leaf_hi,
+                             float4* __restrict__ node_lo, float4* __restrict__ node_hi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float3 a, b, c;
+  tri_verts(pos, tri[perm[i]], &a, &b, &c);
+  leaf_lo[i] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)), 0.f);
+  leaf_hi[i] = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)), 0.f);
+  if (n == 1) return;
+  uint32_t node = parent_leaf[i];
+  for (;;) {
+    __threadfence();
+    if (atomicAdd(&visits[node], 1u) == 0u) return;  // the sibling finishes this node
+    __threadfence();
+    const uint2 ch = children[node];
+    float4 l0, h0, l1, h1;
+    if (ch.x & kBvh2Leaf) {
+      l0 = leaf_lo[ch.x & ~kBvh2Leaf];
+      h0 = leaf_hi[ch.x & ~kBvh2Leaf];
+    } else {
+      l0 = node_lo[ch.x];
+      h0 = node_hi[ch.x];
+    }
+    if (ch.y & kBvh2Leaf) {
+      l1 = leaf_lo[ch.y & ~kBvh2Leaf];
+      h1 = leaf_hi[ch.y & ~kBvh2Leaf];
+    } else {
+      l1 = node_lo[ch.y];
+      h1 = node_hi[ch.y];
+    }
+    node_lo[node] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+    node_hi[node] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+    if (node == 0) return;
+    node = parent_internal[node];
+  }
+}
+
+__device__ inline float half_area(float4 lo, float4 hi) {
+  const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+  return dx * dy + dy * dz + dz * dx;
+}
+
+struct Bvh2View {
+  const uint2* children;
+  const uint2* range;
+  const float4 *leaf_lo, *leaf_hi, *node_lo, *node_hi;
+};
+
+// Size of the sorted-primitive run under a BVH2 reference (bit 31 = leaf).
+__device__ inline uint32_t ref_size(const Bvh2View& v, uint32_t r) {
+  if (r & kBvh2Leaf) return 1u;
+  const uint2 g = v.range[r];
+  return g.y - g.x + 1u;
+}
+
+// One frontier entry (BVH4 node index, BVH2 internal node) becomes a filled BVH4 node.
+__global__ void collapse_kernel(Bvh2View v, const uint2* __restrict__ frontier, const uint32_t* __restrict__ count_in,
+                                uint2* __restrict__ next, uint32_t* __restrict__ count_next,
+                                uint32_t* __restrict__ node_alloc, Node4* __restrict__ nodes, uint32_t depth,
+                                uint32_t leaf_max) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *count_in) return;
+  const uint2 item = frontier[i];
+  uint32_t ref[4];
+  int cnt = 2;
+  const uint2 ch = v.children[item.y];
+  ref[0] = ch.x;
+  ref[1] = ch.y;
+  // Expand the largest-area internal child that is too big to be a leaf, up to four children.
+  while (cnt < 4) {
+    int best = -1;
+    float best_area = -1.f;
+    for (int c = 0; c < cnt; ++c) {
+      if ((ref[c] & kBvh2Leaf) || ref_size(v, ref[c]) <= leaf_max) continue;
+      const float a = half_area(v.node_lo[ref[c]], v.node_hi[ref[c]]);
+      if (a > best_area) {
+        best_area = a;
+        best = c;
+      }
+    }
+    if (best < 0) break;
+    const uint2 g = v.children[ref[best]];
+    ref[best] = g.x;
+    ref[cnt++] = g.y;
+  }
+  Node4 nd;
+  float lo[3][4], hi[3][4];
+  uint32_t cc[4];
+  for (int c = 0; c < 4; ++c) {
+    if (c >= cnt) {
+      lo[0][c] = lo[1][c] = lo[2][c] = kEmptyBox;
+      hi[0][c] = hi[1][c] = hi[2][c] = kEmptyBox;
+      cc[c] = kNoChild;
+      continue;
+    }
+    const uint32_t r = ref[c];
+    float4 l, h;
+    if (r & kBvh2Leaf) {
+      l = v.leaf_lo[r & ~kBvh2Leaf];
+      h = v.leaf_hi[r & ~kBvh2Leaf];
+    } else {
+      l = v.node_lo[r];
+      h = v.node_hi[r];
+    }
+    lo[0][c] = l.x; lo[1][c] = l.y; lo[2][c] = l.z;
+    hi[0][c] = h.x; hi[1][c] = h.y; hi[2][c] = h.z;
+    if (r & kBvh2Leaf) {
+      cc[c] = leaf_code(r & ~kBvh2Leaf, 1u);
+    } else if (ref_size(v, r) <= leaf_max) {
+      const uint2 g = v.range[r];
+      cc[c] = leaf_code(g.x, g.y - g.x + 1u);
+    } else {
+      const uint32_t idx = atomicAdd(node_alloc, 1u);
+      cc[c] = idx;
+      next[atomicAdd(count_next, 1u)] = make_uint2(idx, r);
+    }
+  }
+  nd.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+  nd.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+  nd.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+  nd.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+  nd.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+  nd.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+  nd.child = make_uint4(cc[0], cc[1], cc[2], cc[3]);
+  nd.meta = make_uint4((uint32_t)cnt, depth, 0u, 0u);
+  nodes[item.x] = nd;
+}
+
+// Leaf triangles in sorted order: v0 (w = global id), e1 = v0 - v1 (w = geometry mask), e2 = v2 - v0.
+__global__ void leaf_tris_kernel(const float4* __restrict__ pos, const uint4* __restrict__ tri,
+                                 const GeomRec* __restrict__ geoms, const uint32_t* __restrict__ perm, uint32_t n,
+                                 float4* __restrict__ out) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t t = perm[j];
+  const uint4 ix = tri[t];
+  float3 a, b, c;
+  tri_verts(pos, ix, &a, &b, &c);
+  const uint32_t mask = geoms[ix.w].mask;
+  out[3 * (size_t)j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(t));
+  out[3 * (size_t)j + 1] = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, __uint_as_float(mask));
+  out[3 * (size_t)j + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.f);
+}
+
+inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const GeomRec* geoms, uint32_t n,
+               const float lo[3], const float hi[3], int leaf_max, BvhBuild* out) {
+  hipStream_t s = ctx->stream;
+  leaf_max = std::max(1, std::min(leaf_max, 16));
+  const size_t nn = std::max<uint32_t>(n, 1);
+  ensure(ctx, out->nodes, nn * sizeof(Node4));
+  ensure(ctx, out->leaf_tris, nn * 3 * sizeof(float4));
+  Node4* nodes = static_cast<Node4*>(out->nodes.ptr);
+
+  if (n == 0) {  // empty scene: a root without children
+    Node4 root;
+    root.lox = root.hix = root.loy = root.hiy = root.loz = root.hiz =
+        make_float4(kEmptyBox, kEmptyBox, kEmptyBox, kEmptyBox);
+    root.child = make_uint4(kNoChild, kNoChild, kNoChild, kNoChild);
+    root.meta = make_uint4(0, 0, 0, 0);
+    SWR_HIP(hipMemcpyAsync(nodes, &root, sizeof root, hipMemcpyHostToDevice, s));
+    SWR_HIP(hipStreamSynchronize(s));
+    out->num_nodes = 1;
+    out->max_depth = 0;
+    return;
+  }
+
+  // Temporary buffers for the build, released at the end.
+  DeviceBuffer codes, perm, alt_k, alt_v, children, range, par_i, par_l, visits, leaf_lo, leaf_hi, node_lo, node_hi,
+      fr_a, fr_b, counters;
+  auto cleanup = [&] {
+    (void)hipStreamSynchronize(s);
+    for (DeviceBuffer* b : {&codes, &perm, &alt_k, &alt_v, &children, &range, &par_i, &par_l, &visits, &leaf_lo,
+                            &leaf_hi, &node_lo, &node_hi, &fr_a, &fr_b, &counters})
+      release(ctx, *b);
+  };
+  try {
+    ensure(ctx, codes, nn * 4);
+    ensure(ctx, perm, nn * 4);
+    ensure(ctx, alt_k, nn * 4);
+    ensure(ctx, alt_v, nn * 4);
+    float3 flo = make_float3(lo[0], lo[1], lo[2]);
+    auto inv = [](float a, float b) { return b > a ? 1.f / (b - a) : 0.f; };
+    float3 inv_ext = make_float3(inv(lo[0], hi[0]), inv(lo[1], hi[1]), inv(lo[2], hi[2]));
+    morton_kernel<<<blocks(n), 256, 0, s>>>(pos, tri, n, flo, inv_ext, static_cast<uint32_t*>(codes.ptr));
+    uint32_t *sk = nullptr, *sv = nullptr;
+    sort_pairs(ctx, static_cast<uint32_t*>(codes.ptr), static_cast<uint32_t*>(perm.ptr),
+               static_cast<uint32_t*>(alt_k.ptr), static_cast<uint32_t*>(alt_v.ptr), n, 30, true, &sk, &sv, s);
+
+    ensure(ctx, leaf_lo, nn * 16);
+    ensure(ctx, leaf_hi, nn * 16);
+    const size_t ni = n > 1 ? n - 1 : 1;
+    ensure(ctx, children, ni * 8);
+    ensure(ctx, range, ni * 8);
+    ensure(ctx, par_i, ni * 4);
+    ensure(ctx, par_l, nn * 4);
+    ensure(ctx, visits, ni * 4);
+    ensure(ctx, node_lo, ni * 16);
+    ensure(ctx, node_hi, ni * 16);
+    if (n > 1)
+      karras_kernel<<<blocks(n - 1), 256, 0, s>>>(sk, (int)n, static_cast<uint2*>(children.ptr),
+                                                  static_cast<uint2*>(range.ptr), static_cast<uint32_t*>(par_i.ptr),
+                                                  static_cast<uint32_t*>(par_l.ptr));
+    fill_u32_kernel<<<blocks(ni), 256, 0, s>>>(static_cast<uint32_t*>(visits.ptr), (uint32_t)ni, 0u);
+    refit_kernel<<<blocks(n), 256, 0, s>>>(pos, tri, sv, (int)n, static_cast<uint2*>(children.ptr),
+                                           static_cast<uint32_t*>(par_i.ptr), static_cast<uint32_t*>(par_l.ptr),
+                                           static_cast<uint32_t*>(visits.ptr), static_cast<float4*>(leaf_lo.ptr),
+                                           static_cast<float4*>(leaf_hi.ptr), static_cast<float4*>(node_lo.ptr),
+                                           static_cast<float4*>(node_hi.ptr));
+    leaf_tris_kernel<<<blocks(n), 256, 0, s>>>(pos, tri, geoms, sv, n, static_cast<float4*>(out->leaf_tris.ptr));
+    SWR_HIP(hipGetLastError());
+
+    if ((int)n <= leaf_max || n == 1) {
+      // Everything fits one leaf: root with a single child spanning all triangles.
+      float4 l, h;
+      SWR_HIP(hipMemcpyAsync(&l, leaf_lo.ptr, 16, hipMemcpyDeviceToHost, s));
+      SWR_HIP(hipMemcpyAsync(&h, leaf_hi.ptr, 16, hipMemcpyDeviceToHost, s));
+      if (n > 1) {
+        SWR_HIP(hipMemcpyAsync(&l, node_lo.ptr, 16, hipMemcpyDeviceToHost, s));
+        SWR_HIP(hipMemcpyAsync(&h, node_hi.ptr, 16, hipMemcpyDeviceToHost, s));
+      }
+      SWR_HIP(hipStreamSynchronize(s));
+      Node4 root;
+      root.lox = make_float4(l.x, kEmptyBox, kEmptyBox, kEmptyBox);
+      root.hix = make_float4(h.x, kEmptyBox, kEmptyBox, kEmptyBox);
+      root.loy = make_float4(l.y, kEmptyBox, kEmptyBox, kEmptyBox);
+      root.hiy = make_float4(h.y, kEmptyBox, kEmptyBox, kEmptyBox);
+      root.loz = make_float4(l.z, kEmptyBox, kEmptyBox, kEmptyBox);
+      root.hiz = make_float4(h.z, kEmptyBox, kEmptyBox, kEmptyBox);
+      root.child = make_uint4(leaf_code(0, n), kNoChild, kNoChild, kNoChild);
+      root.meta = make_uint4(1, 0, 0, 0);
+      SWR_HIP(hipMemcpyAsync(nodes, &root, sizeof root, hipMemcpyHostToDevice, s));
+      SWR_HIP(hipStreamSynchronize(s));
+      out->num_nodes = 1;
+      out->max_depth = 0;
+      cleanup();
+      return;
+    }
+
+    // Level-by-level collapse. counters: [0] node allocator, [1] frontier A count, [2] B count.
+    ensure(ctx, fr_a, nn * 8);
+    ensure(ctx, fr_b, nn * 8);
+    ensure(ctx, counters, 16);
+    uint32_t init[4] = {1u, 1u, 0u, 0u};
+    uint2 root_item = make_uint2(0u, 0u);
+    SWR_HIP(hipMemcpyAsync(counters.ptr, init, 16, hipMemcpyHostToDevice, s));
+    SWR_HIP(hipMemcpyAsync(fr_a.ptr, &root_item, 8, hipMemcpyHostToDevice, s));
+    Bvh2View view{static_cast<uint2*>(children.ptr),  static_cast<uint2*>(range.ptr),
+                  static_cast<float4*>(leaf_lo.ptr),  static_cast<float4*>(leaf_hi.ptr),
+                  static_cast<float4*>(node_lo.ptr),  static_cast<float4*>(node_hi.ptr)};
+    uint32_t* cnt = static_cast<uint32_t*>(counters.ptr);
+    uint2* cur = static_cast<uint2*>(fr_a.ptr);
+    uint2* nxt = static_cast<uint2*>(fr_b.ptr);
+    int cur_slot = 1, nxt_slot = 2;
+    uint32_t frontier = 1, depth = 0;
+    while (frontier) {
+      SWR_HIP(hipMemsetAsync(cnt + nxt_slot, 0, 4, s));
+      collapse_kernel<<<blocks(frontier), 256, 0, s>>>(view, cur, cnt + cur_slot, nxt, cnt + nxt_slot, cnt, nodes, depth,
+                                                       (uint32_t)leaf_max);
+      SWR_HIP(hipGetLastError());
+      SWR_HIP(hipMemcpyAsync(&frontier, cnt + nxt_slot, 4, hipMemcpyDeviceToHost, s));
+      SWR_HIP(hipStreamSynchronize(s));
+      std::swap(cur, nxt);
+      std::swap(cur_slot, nxt_slot);
+      if (frontier) ++depth;
+      if (depth > 256) fail(SWR_E_INVALID, "bvh: collapse did not terminate");
+    }
+    uint32_t num_nodes = 0;
+    SWR_HIP(hipMemcpyAsync(&num_nodes, cnt, 4, hipMemcpyDeviceToHost, s));
+    SWR_HIP(hipStreamSynchronize(s));
+    out->num_nodes = num_nodes;
+    out->max_depth = depth;
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+}
+
+}  // namespace swr

@@ -7,31 +7,11 @@
 // Hit key: CompareHit orders by (geomID, primID) as signed ints, so misses (-1) come first
 // (RayUncompressed.h:54-60). Global triangle ids are geomID-major, so id + 1 (miss -> 0)
 // reproduces that order.
+#include "device_util.h"
 #include "swr_internal.h"
 
 namespace swr {
 namespace {
-
-// Spread the low 10 bits of v to every third bit.
-__device__ inline uint32_t spread3(uint32_t v) {
-  v &= 0x3FFu;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
-// Bits from the top: x9 y9 z8 x8 y8 z7 ... x1 y1 z0 x0 y0 (the 30-bit code of (x, y, 2z) >> 1).
-__device__ inline uint32_t morton29(uint32_t qx, uint32_t qy, uint32_t qz) {
-  return ((spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz << 1)) >> 1;
-}
-
-__device__ inline uint32_t quantise(float o, float lo, float scale, float qmax) {
-  float f = floorf((o - lo) * scale);
-  f = fminf(fmaxf(f, 0.f), qmax);
-  return (uint32_t)f;
-}
 
 __global__ void ray_keys_kernel(const float4* __restrict__ rays, uint32_t n, float3 lo, float3 scale,
                                 uint32_t* __restrict__ keys) {
@@ -39,9 +19,7 @@ __global__ void ray_keys_kernel(const float4* __restrict__ rays, uint32_t n, flo
   if (i >= n) return;
   const float4 o = rays[2 * (size_t)i];
   const float4 d = rays[2 * (size_t)i + 1];
-  const uint32_t oct = (d.x < 0.f ? 1u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 4u : 0u);
-  keys[i] = (oct << 29) | morton29(quantise(o.x, lo.x, scale.x, 1023.f), quantise(o.y, lo.y, scale.y, 1023.f),
-                                   quantise(o.z, lo.z, scale.z, 511.f));
+  keys[i] = ray_key(o.x, o.y, o.z, d.x, d.y, d.z, lo, scale);
 }
 
 __global__ void hit_keys_kernel(const uint4* __restrict__ hits, uint32_t n, uint32_t* __restrict__ keys) {

@@ -1,0 +1,898 @@
+// render.hip -- the sorted wavefront loop of one render on gfx950 (Pathtracer::process,
+// src/core/Pathtracer.cpp:425-493).
+//
+// The reference generates every camera ray up front, bins rays into six direction files,
+// then repeatedly pops a batch, sorts it, traces it with Embree, sorts the hits and shades
+// them, writing continuation rays back to the bins (Pathtracer.cpp:451-475). Here one device
+// queue of 2^bin_exponent rays replaces the bins and batches:
+//   raygen      tops the queue up with camera rays for the next sample ids (Camera.cpp:6-59);
+//   ray sort    octant + Morton keys (written by raygen and shade) -> stable radix sort;
+//   trace       closest hit in ray-key order -> hit record + hit key per queue slot;
+//   hit sort    stable radix sort by (geomID, primID) (CompareHit, RayUncompressed.h:54-60);
+//   shade       MIS shading in hit order (Integrator.cpp:12-255): emission and environment
+//               contributions, one shadow ray per NEE sample, one continuation per survivor,
+//               appended with one atomic per wave;
+//   shadow      any-hit traversal of the shadow queue; unoccluded ones become contributions;
+// and when the contribution buffer fills (and at the end) a stable radix sort by sample id and
+// a segmented fold adds each sample's contributions into its radiance in bounce order, the
+// order of the reference's samples[sampleID] += (Integrator.cpp:66-71, 171-173). No float
+// atomics anywhere: the image is a pure function of the inputs. The tent/box resolve gathers
+// each pixel's 3x3 neighbourhood in the reference's loop order (TentFilter.cpp:17-61).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device_util.h"
+#include "traverse.h"
+
+namespace swr {
+
+void commit_scene(swr_scene* s);
+
+namespace {
+
+const float kPi = 3.14159265358979323846f;  // Constants.h:10-15
+const float kInvPi = 0.31830988618379067154f;
+const float kPi180 = 0.01745329251994329577f;
+const float kEps = 1.19209290e-07f;         // FLT_EPSILON
+
+// ---- small vector helpers (no contraction: -ffp-contract=off) ----------------------------
+struct V3 {
+  float x, y, z;
+};
+__host__ __device__ inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+__device__ inline float norm(V3 a) { return sqrtf(dot(a, a)); }
+__device__ inline V3 normalize(V3 a) {
+  const float n = norm(a);
+  return n > 0.f ? a * (1.f / n) : a;
+}
+__device__ inline float maxc(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ inline V3 apply(const float* m, V3 v) {
+  return v3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+            m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+
+// ---- frame constants -----------------------------------------------------------------------
+struct CameraRec {
+  float rot[9];  // Ry * Rx * Rz (PinHoleCamera.cpp:7-11)
+  float t[3];
+  float n[3];    // R * x-hat, normalised (PinHoleCamera.cpp:13)
+  int32_t type;
+  float focal_length, focal_distance, aperture;
+};
+
+struct FrameParams {
+  uint32_t width, height, spp, base;
+  int32_t sampler, filter, min_depth, max_depth;
+  float threshold;
+  uint32_t iteration;
+  uint32_t seed_lo, seed_hi;
+  uint32_t num_lights;
+  float pick;           // 1 / number of lights
+  float3 key_lo, key_scale;
+  V3 env;
+  int32_t env_nonzero;
+  CameraRec cam;
+};
+
+// ---- BSDFs (LambertShader.cpp, NullShader.cpp; Glossy = GGX reflection) -----------------
+struct Eval {
+  V3 f;
+  float cos;
+  float pdf;
+};
+
+// Local frame of the samplers (LambertShader.cpp:65-67).
+__device__ inline void shading_frame(V3 n, V3* s, V3* t) {
+  const V3 temp = (n.x > 0.99f) ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
+  *s = normalize(cross(temp, n));
+  *t = normalize(cross(n, *s));
+}
+
+__device__ inline float ggx_d(float a2, float ch) {
+  const float k = ch * ch * (a2 - 1.f) + 1.f;
+  return a2 / (kPi * k * k);
+}
+__device__ inline float ggx_g1(float a2, float c) { return 2.f * c / (c + sqrtf(a2 + (1.f - a2) * c * c)); }
+
+__device__ inline Eval bsdf_eval(const ShaderRec& sh, V3 in, V3 out, V3 n) {
+  Eval e;
+  if (sh.type == SWR_SHADER_NULL) {  // NullShader::evaluate (NullShader.cpp:25-38)
+    e.f = v3(0.f, 0.f, 0.f);
+    e.cos = 1.f;
+    e.pdf = 1.f * kInvPi * 0.5f;
+    return e;
+  }
+  const float ci = dot(n, in), co = dot(n, out);
+  const V3 col = v3(sh.colour[0], sh.colour[1], sh.colour[2]);
+  if (sh.type == SWR_SHADER_LAMBERT) {  // LambertShader::evaluate (LambertShader.cpp:27-47)
+    e.f = (ci < 0.f || co < 0.f) ? v3(0.f, 0.f, 0.f) : col * sh.reflectance * kInvPi;
+    e.cos = ci < 0.f ? 0.f : ci;
+    e.pdf = e.cos * kInvPi;
+    return e;
+  }
+  e.cos = ci < 0.f ? 0.f : ci;
+  if (ci <= 0.f || co <= 0.f) {
+    e.f = v3(0.f, 0.f, 0.f);
+    e.pdf = 0.f;
+    return e;
+  }
+  const float a = sh.roughness, a2 = a * a;
+  const V3 h = normalize(in + out);
+  const float ch = dot(n, h), oh = dot(out, h);
+  const float D = ggx_d(a2, ch);
+  const float G = ggx_g1(a2, ci) * ggx_g1(a2, co);
+  e.f = col * (sh.reflectance * D * G / (4.f * ci * co));
+  e.pdf = oh > 0.f ? D * ch / (4.f * oh) : 0.f;
+  return e;
+}
+
+// Samples an input direction; false when the sample carries no energy (the path ends).
+__device__ inline bool bsdf_sample(const ShaderRec& sh, float u1, float u2, V3 out, V3 n, V3* in, Eval* e) {
+  if (sh.type == SWR_SHADER_NULL) return false;  // NullShader::sample: zero weight
+  V3 s, t;
+  shading_frame(n, &s, &t);
+  const V3 col = v3(sh.colour[0], sh.colour[1], sh.colour[2]);
+  if (sh.type == SWR_SHADER_LAMBERT) {  // LambertShader::sample (LambertShader.cpp:49-73)
+    const float azimuth = 2.f * kPi * u1;
+    const float zenith = acosf(sqrtf(u2));
+    // sphericalToCartesianCoord (Common.h:28-33)
+    const float x = cosf(azimuth) * sinf(zenith);
+    const float y = sinf(azimuth) * sinf(zenith);
+    const float z = cosf(zenith);
+    *in = t * x + s * y + n * z;
+    e->f = col * sh.reflectance * kInvPi;
+    e->cos = dot(n, *in);
+    e->pdf = e->cos * kInvPi;
+    return e->pdf > 0.f;
+  }
+  const float a = sh.roughness, a2 = a * a;
+  const float phi = 2.f * kPi * u1;
+  const float tan2 = a2 * u2 / (1.f - u2);
+  const float ch = 1.f / sqrtf(1.f + tan2);
+  const float sh_ = sqrtf(fmaxf(0.f, 1.f - ch * ch));
+  const V3 h = t * (cosf(phi) * sh_) + s * (sinf(phi) * sh_) + n * ch;
+  const float oh = dot(out, h);
+  *in = h * (2.f * oh) - out;
+  *e = bsdf_eval(sh, *in, out, n);
+  return e->pdf > 0.f && maxc(e->f) > 0.f;
+}
+
+// QuadLight::illuminate (QuadLight.cpp:41-67).
+__device__ inline void light_illuminate(const LightRec& L, float ux, float uy, V3 p, V3* dir, float* dist, V3* rad,
+                                        float* pdfw) {
+  const V3 local = v3(0.f, L.sx * (ux - 0.5f), L.sy * (uy - 0.5f));
+  const V3 smp = v3(L.t[0], L.t[1], L.t[2]) + apply(L.rot, local);
+  const V3 ln = v3(L.n[0], L.n[1], L.n[2]);
+  float c = dot(normalize(p - smp), ln);
+  *rad = c < 0.f ? v3(0.f, 0.f, 0.f) : v3(L.intensity, L.intensity, L.intensity);
+  c = c < 0.f ? 0.f : c;
+  const V3 dv = smp - p;
+  *dir = normalize(dv);
+  *dist = norm(dv);
+  *pdfw = (dot(dv, dv) / c) / (L.sx * L.sy);
+}
+
+// ---- camera (Camera.cpp:15-38, PinHoleCamera.cpp:16-32, ThinLensCamera.cpp:16-56) -------
+__device__ inline void camera_ray(const FrameParams& p, uint32_t sid, V3* org, V3* dir, float* sx, float* sy) {
+  const uint32_t W = p.width, H = p.height, spp = p.spp;
+  const uint32_t x = sid / (H * spp);
+  const uint32_t rem = sid - x * H * spp;
+  const uint32_t y = rem / spp;
+  const uint32_t k = rem - y * spp;
+  const Uniform4 r = philox_uniform4(p.seed_lo, p.seed_hi, sid, p.iteration, 0u, 0u);
+  const uint32_t base = p.base > 0 ? p.base : 1u;
+  const float cell = 1.f / (float)base;
+  const uint32_t ix = k / base, iy = k % base;
+  float jx, jy;
+  if (p.sampler == SWR_SAMPLER_INDEPENDENT) {  // IndependentSampler.cpp:11-12
+    jx = r.r[0];
+    jy = r.r[1];
+  } else if (p.sampler == SWR_SAMPLER_GRID) {  // GridSampler.cpp:13-14
+    jx = (float)ix * cell + 0.5f * cell;
+    jy = (float)iy * cell + 0.5f * cell;
+  } else {  // StratifiedSampler.cpp:13-14
+    jx = (float)ix * cell + r.r[0] * cell;
+    jy = (float)iy * cell + r.r[1] * cell;
+  }
+  *sx = (float)x + jx;
+  *sy = (float)y + jy;
+  const float fx = (((float)x + jx) * 2.f - (float)W) / (float)W * 36.f;  // Camera.cpp:34
+  const float fy = (((float)y + jy) * 2.f - (float)H) / (float)W * 36.f;  // Camera.cpp:35
+  const CameraRec& c = p.cam;
+  const V3 ct = v3(c.t[0], c.t[1], c.t[2]), cn = v3(c.n[0], c.n[1], c.n[2]);
+  const V3 nodal = ct + cn * c.focal_length;
+  const V3 film = ct + apply(c.rot, v3(0.f, fy, fx));
+  if (c.type == SWR_CAMERA_PINHOLE) {
+    *org = nodal;
+    *dir = normalize(nodal - film);
+    return;
+  }
+  // Thin lens: rejection-sampled aperture disk, rays through the focal-plane point.
+  const float ratio = c.focal_distance / c.focal_length;
+  const V3 focal = film + (nodal - film) * ratio;
+  float ax = 0.f, ay = 0.f;
+  float cx0 = r.r[2] - 0.5f, cy0 = r.r[3] - 0.5f;
+  for (uint32_t attempt = 0; attempt < 64; ++attempt) {
+    if (attempt) {
+      const Uniform4 q = philox_uniform4(p.seed_lo, p.seed_hi, sid, p.iteration, 0u, attempt);
+      cx0 = q.r[0] - 0.5f;
+      cy0 = q.r[1] - 0.5f;
+    }
+    if (sqrtf(cx0 * cx0 + cy0 * cy0) < 0.5f) {
+      ax = cx0;
+      ay = cy0;
+      break;
+    }
+  }
+  const float scale = c.focal_length / c.aperture;
+  const V3 ap = ct + apply(c.rot, v3(0.f, ax * scale, ay * scale));
+  *org = ap + cn * c.focal_length;
+  *dir = normalize(focal - *org);
+}
+
+// ---- kernels --------------------------------------------------------------------------------
+// Camera rays for sample ids [sid0, sid0 + k) into queue slots [slot0, slot0 + k); resets each
+// sample's radiance and records its film position (Camera.cpp:23-33). In the stand-alone
+// raygen entry point (path == nullptr) ray t goes to index t and xy is indexed by t.
+__global__ void raygen_kernel(FrameParams p, uint32_t sid0, uint32_t k, uint32_t slot0, float4* __restrict__ rays,
+                              float4* __restrict__ path_w, uint2* __restrict__ path_id, uint32_t* __restrict__ keys,
+                              float2* __restrict__ sample_xy, float4* __restrict__ radiance) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k) return;
+  const uint32_t sid = sid0 + t, slot = slot0 + t;
+  V3 o, d;
+  float sx, sy;
+  camera_ray(p, sid, &o, &d, &sx, &sy);
+  rays[2 * (size_t)slot] = make_float4(o.x, o.y, o.z, 0.001f);     // RayDecompress.cpp:15
+  rays[2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, 100000.f);  // RayDecompress.cpp:16
+  if (path_w) {
+    path_w[slot] = make_float4(1.f, 1.f, 1.f, 1.f);  // weight 1, lastPdf 1 (Camera.cpp:23-26)
+    path_id[slot] = make_uint2(sid, 0u);
+    keys[slot] = ray_key(o.x, o.y, o.z, d.x, d.y, d.z, p.key_lo, p.key_scale);
+    sample_xy[sid] = make_float2(sx, sy);
+    radiance[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
+  } else if (sample_xy) {
+    sample_xy[t] = make_float2(sx, sy);
+  }
+}
+
+// Closest hit of every queue slot, visited in ray-key order (perm); hit and hit key are stored
+// per slot.
+template <int kStack, int kBlock>
+__global__ __launch_bounds__(kBlock) void trace_queue_kernel(const Node4* __restrict__ nodes,
+                                                             const float4* __restrict__ leaf_tris,
+                                                             const uint32_t* __restrict__ perm, uint32_t n,
+                                                             const float4* __restrict__ rays, uint4* __restrict__ hits,
+                                                             uint32_t* __restrict__ hit_keys) {
+  __shared__ uint32_t s_stack[kStack * kBlock];
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = perm[j];
+  const RayState r = make_ray(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]);
+  const HitResult h = traverse<kBlock, false>(nodes, leaf_tris, r, SWR_MASK_ALL, s_stack + threadIdx.x);
+  hits[i] = make_uint4(__float_as_uint(h.t), __float_as_uint(h.u), __float_as_uint(h.v), h.id);
+  hit_keys[i] = h.id + 1u;
+}
+
+struct QueueOut {
+  float4* rays;
+  float4* path_w;
+  uint2* path_id;
+  uint32_t* keys;
+};
+
+struct ShadeOut {
+  QueueOut next;
+  float4* shadow_rays;  // 2 float4 per shadow ray
+  float4* shadow_col;   // rgb contribution (already times throughput), sample id bits in w
+  uint32_t* contrib_sid;
+  float4* contrib_rgb;
+  uint32_t* counters;   // [0] continuations, [1] shadow rays, [2] contributions
+};
+
+// Integrator::operator() for every slot, in hit-key order (Integrator.cpp:12-255).
+__global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p, const uint32_t* __restrict__ perm,
+                                                    uint32_t n, const float4* __restrict__ rays,
+                                                    const float4* __restrict__ path_w,
+                                                    const uint2* __restrict__ path_id,
+                                                    const uint4* __restrict__ hits, ShadeOut out) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = m < n;
+  bool has_c = false, has_s = false, has_n = false;
+  V3 contrib = v3(0.f, 0.f, 0.f), srgb = v3(0.f, 0.f, 0.f);
+  V3 P = v3(0.f, 0.f, 0.f), sdir = v3(0.f, 0.f, 0.f), ndir = v3(0.f, 0.f, 0.f), nw = v3(0.f, 0.f, 0.f);
+  float sdist = 0.f, npdf = 0.f;
+  uint32_t sid = 0, depth = 0;
+  if (valid) {
+    const uint32_t i = perm[m];
+    const uint4 h = hits[i];
+    const float4 o4 = rays[2 * (size_t)i], d4 = rays[2 * (size_t)i + 1];
+    const float4 pw = path_w[i];
+    const uint2 pid = path_id[i];
+    sid = pid.x;
+    depth = pid.y;
+    const V3 w = v3(pw.x, pw.y, pw.z);
+    const float last_pdf = pw.w;
+    if (h.w == kMiss) {
+      // Misses add nothing in the reference (Integrator.cpp:26-27); a constant environment
+      // adds throughput * radiance.
+      if (p.env_nonzero) {
+        has_c = true;
+        contrib = v3(w.x * p.env.x, w.y * p.env.y, w.z * p.env.z);
+      }
+    } else {
+      const float t = __uint_as_float(h.x);
+      const uint4 tv = s.tri[h.w];
+      const GeomRec g = s.geoms[tv.w];
+      const V3 dn = normalize(v3(d4.x, d4.y, d4.z));
+      if (g.light >= 0) {  // light hit (Integrator.cpp:37-75)
+        const LightRec& L = s.lights[g.light];
+        const float c = dot(v3(L.n[0], L.n[1], L.n[2]), dn * -1.f);  // QuadLight::radiance
+        const V3 le = c < 0.f ? v3(0.f, 0.f, 0.f) : v3(L.intensity, L.intensity, L.intensity);
+        const float pdfa = 1.f / (L.sx * L.sy);
+        float mis = 1.f;
+        if (depth > 0) {
+          const float pdfw = pdfa * (t * t) / fabsf(c);  // areaToAngleProbability (Common.h:35-38)
+          mis = last_pdf / (last_pdf + pdfw * p.pick);   // misBalance (Common.h:45-48)
+        }
+        if (maxc(le) > kEps) {
+          has_c = true;
+          contrib = v3(le.x * mis * w.x, le.y * mis * w.y, le.z * mis * w.z);
+        }
+      } else {
+        const ShaderRec sh = s.shaders[g.shader];
+        const float4 a = s.pos[tv.x], b = s.pos[tv.y], c = s.pos[tv.z];
+        const V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
+        const V3 nrm = normalize(cross(v0 - v1, v2 - v0)) * -1.f;  // Integrator.cpp:121-125
+        P = v3(o4.x, o4.y, o4.z) + dn * t;                          // Integrator.cpp:126
+        const V3 out_dir = dn * -1.f;
+        const Uniform4 r = philox_uniform4(p.seed_lo, p.seed_hi, sid, p.iteration, shade_word(depth, 1u), 0u);
+        // Next event estimation (Integrator.cpp:102-178)
+        if (p.num_lights) {
+          const uint32_t l = min(p.num_lights - 1u, (uint32_t)((float)p.num_lights * r.r[0]));
+          V3 in, le;
+          float dist, pdfw;
+          light_illuminate(s.lights[l], r.r[1], r.r[2], P, &in, &dist, &le, &pdfw);
+          if (maxc(le) > kEps && isfinite(pdfw)) {
+            const Eval e = bsdf_eval(sh, in, out_dir, nrm);
+            if (maxc(e.f) > kEps) {
+              const float mis = (pdfw * p.pick) / (pdfw * p.pick + e.pdf);
+              const float k = mis * e.cos / (pdfw * p.pick);
+              const V3 cn = v3(k * (le.x * e.f.x), k * (le.y * e.f.y), k * (le.z * e.f.z));
+              has_s = true;
+              sdir = in;
+              sdist = dist;
+              srgb = v3(cn.x * w.x, cn.y * w.y, cn.z * w.z);
+            }
+          }
+        }
+        // Russian roulette and continuation (Integrator.cpp:181-246)
+        if ((int32_t)depth < p.max_depth) {
+          const float cont = sh.type == SWR_SHADER_NULL ? 1.f : sh.reflectance;
+          float q = fminf(1.f, cont / p.threshold);
+          if ((int32_t)depth < p.min_depth) q = 1.f;
+          if (!(r.r[3] > q)) {
+            const Uniform4 r2 = philox_uniform4(p.seed_lo, p.seed_hi, sid, p.iteration, shade_word(depth, 2u), 0u);
+            V3 in;
+            Eval e;
+            if (bsdf_sample(sh, r2.r[0], r2.r[1], out_dir, nrm, &in, &e)) {
+              const float k = e.cos / e.pdf;
+              nw = v3(w.x * e.f.x * k / q, w.y * e.f.y * k / q, w.z * e.f.z * k / q);
+              if (maxc(nw) > 0.f) {
+                has_n = true;
+                ndir = in;
+                npdf = e.pdf;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  const uint32_t cslot = wave_append(&out.counters[0], has_n);
+  const uint32_t sslot = wave_append(&out.counters[1], has_s);
+  const uint32_t pslot = wave_append(&out.counters[2], has_c);
+  if (has_n) {
+    out.next.rays[2 * (size_t)cslot] = make_float4(P.x, P.y, P.z, 0.001f);
+    out.next.rays[2 * (size_t)cslot + 1] = make_float4(ndir.x, ndir.y, ndir.z, 100000.f);
+    out.next.path_w[cslot] = make_float4(nw.x, nw.y, nw.z, npdf);
+    out.next.path_id[cslot] = make_uint2(sid, depth + 1u);
+    out.next.keys[cslot] = ray_key(P.x, P.y, P.z, ndir.x, ndir.y, ndir.z, p.key_lo, p.key_scale);
+  }
+  if (has_s) {
+    out.shadow_rays[2 * (size_t)sslot] = make_float4(P.x, P.y, P.z, 0.001f);
+    out.shadow_rays[2 * (size_t)sslot + 1] = make_float4(sdir.x, sdir.y, sdir.z, sdist);
+    out.shadow_col[sslot] = make_float4(srgb.x, srgb.y, srgb.z, __uint_as_float(sid));
+  }
+  if (has_c) {
+    out.contrib_sid[pslot] = sid;
+    out.contrib_rgb[pslot] = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
+  }
+}
+
+// Shadow rays (mask 0x0FFFFFFF, Integrator.cpp:153-174); unoccluded ones append their
+// contribution. The queue length is read on the device.
+template <int kStack, int kBlock>
+__global__ __launch_bounds__(kBlock) void shadow_kernel(const Node4* __restrict__ nodes,
+                                                        const float4* __restrict__ leaf_tris,
+                                                        const uint32_t* __restrict__ count,
+                                                        const float4* __restrict__ rays,
+                                                        const float4* __restrict__ col,
+                                                        uint32_t* __restrict__ contrib_sid,
+                                                        float4* __restrict__ contrib_rgb,
+                                                        uint32_t* __restrict__ contrib_count) {
+  __shared__ uint32_t s_stack[kStack * kBlock];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool visible = false;
+  float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (i < *count) {
+    const RayState r = make_ray(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]);
+    const HitResult h = traverse<kBlock, true>(nodes, leaf_tris, r, SWR_MASK_SHADOW, s_stack + threadIdx.x);
+    visible = h.id == kMiss;
+    if (visible) c = col[i];
+  }
+  const uint32_t slot = wave_append(contrib_count, visible);
+  if (visible) {
+    contrib_sid[slot] = __float_as_uint(c.w);
+    contrib_rgb[slot] = make_float4(c.x, c.y, c.z, 0.f);
+  }
+}
+
+// Segmented fold over contributions sorted by sample id: each segment head adds its run, in
+// order, onto the sample's radiance (samples[sampleID] += ..., Integrator.cpp:66-71, 171-173).
+__global__ void fold_kernel(uint32_t n, const uint32_t* __restrict__ sorted_sid, const uint32_t* __restrict__ perm,
+                            const float4* __restrict__ rgb, float4* __restrict__ radiance) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const uint32_t sid = sorted_sid[m];
+  if (m > 0 && sorted_sid[m - 1] == sid) return;
+  float4 acc = radiance[sid];
+  for (uint32_t k = m; k < n && sorted_sid[k] == sid; ++k) {
+    const float4 c = rgb[perm[k]];
+    acc.x = acc.x + c.x;
+    acc.y = acc.y + c.y;
+    acc.z = acc.z + c.z;
+  }
+  radiance[sid] = acc;
+}
+
+// Tent / box resolve of one pixel (TentFilter.cpp:17-61, BoxFilter.cpp:17-36), written to
+// pixels[y * W + x] (the iteration's value, not accumulated).
+__global__ void filter_kernel(FrameParams p, const float4* __restrict__ radiance, const float2* __restrict__ sxy,
+                              float* __restrict__ pixels) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t py = blockIdx.y;
+  if (px >= p.width || py >= p.height) return;
+  const int W = (int)p.width, H = (int)p.height, spp = (int)p.spp;
+  float out0, out1, out2;
+  if (p.filter == SWR_FILTER_BOX) {
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int i = 0; i < spp; ++i) {
+      const float4 L = radiance[((size_t)px * H + py) * spp + i];
+      s0 += L.x;
+      s1 += L.y;
+      s2 += L.z;
+    }
+    out0 = s0 / (float)spp;
+    out1 = s1 / (float)spp;
+    out2 = s2 / (float)spp;
+  } else {
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, den = 0.f;
+    const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+    for (int mx = -1; mx < 2; ++mx)
+      for (int my = -1; my < 2; ++my) {
+        const int qx = (int)px + mx, qy = (int)py + my;
+        if (qx < 0 || qx >= W || qy < 0 || qy >= H) continue;
+        const size_t base = ((size_t)qx * H + qy) * spp;
+        for (int i = 0; i < spp; ++i) {
+          const float2 q = sxy[base + i];
+          const float dx = fmaxf(0.f, 1.f - fabsf(q.x - cx));
+          const float dy = fmaxf(0.f, 1.f - fabsf(q.y - cy));
+          const float w = dx * dy;
+          den = den + w;
+          const float4 L = radiance[base + i];
+          s0 += L.x * w;
+          s1 += L.y * w;
+          s2 += L.z * w;
+        }
+      }
+    out0 = s0 / den;
+    out1 = s1 / den;
+    out2 = s2 / den;
+  }
+  float* o = pixels + 3 * ((size_t)py * W + px);
+  o[0] = out0;
+  o[1] = out1;
+  o[2] = out2;
+}
+
+// Sample records (x, y, r, g, b) in sample-id order (Image.h:10-14).
+__global__ void pack_samples_kernel(size_t n, const float2* __restrict__ sxy, const float4* __restrict__ radiance,
+                                    float* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 q = sxy[i];
+  const float4 L = radiance[i];
+  float* o = out + 5 * i;
+  o[0] = q.x;
+  o[1] = q.y;
+  o[2] = L.x;
+  o[3] = L.y;
+  o[4] = L.z;
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// ---- host side ------------------------------------------------------------------------------
+void axis_rotation(int axis, float deg, float m[9]) {
+  const float a = deg * kPi180, c = std::cos(a), s = std::sin(a);
+  std::memset(m, 0, 9 * sizeof(float));
+  const int i = (axis + 1) % 3, j = (axis + 2) % 3;
+  m[axis * 3 + axis] = 1.f;
+  m[i * 3 + i] = c;
+  m[i * 3 + j] = -s;
+  m[j * 3 + i] = s;
+  m[j * 3 + j] = c;
+}
+
+void mat_mul(const float a[9], const float b[9], float out[9]) {
+  float r[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r[i * 3 + j] = a[i * 3 + 0] * b[0 * 3 + j] + a[i * 3 + 1] * b[1 * 3 + j] + a[i * 3 + 2] * b[2 * 3 + j];
+  std::memcpy(out, r, sizeof r);
+}
+
+// PinHoleCamera::construct / ThinLensCamera::construct (PinHoleCamera.cpp:5-14).
+CameraRec make_camera(const swr_camera_desc& d) {
+  CameraRec c{};
+  float ry[9], rx[9], rz[9], t[9];
+  axis_rotation(1, d.rotation[1], ry);
+  axis_rotation(0, d.rotation[0], rx);
+  axis_rotation(2, d.rotation[2], rz);
+  mat_mul(ry, rx, t);
+  mat_mul(t, rz, c.rot);
+  std::memcpy(c.t, d.translation, sizeof c.t);
+  const float n0 = c.rot[0], n1 = c.rot[3], n2 = c.rot[6];
+  const float len = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  const float inv = len > 0.f ? 1.f / len : 1.f;
+  c.n[0] = n0 * inv;
+  c.n[1] = n1 * inv;
+  c.n[2] = n2 * inv;
+  c.type = d.type;
+  c.focal_length = d.focal_length;
+  c.focal_distance = d.focal_distance;
+  c.aperture = d.aperture;
+  return c;
+}
+
+uint32_t spp_of(const swr_render_desc& d) { return d.spp > 0 ? (uint32_t)d.spp : (uint32_t)(d.base * d.base); }
+
+FrameParams make_params(const swr_render_desc& d, const DeviceScene* s) {
+  if (d.width <= 0 || d.height <= 0) fail(SWR_E_INVALID, "render: width and height must be positive");
+  if (d.base <= 0 && d.spp <= 0) fail(SWR_E_INVALID, "render: sample base or spp must be positive");
+  if (d.sampler < 0 || d.sampler > 2 || d.filter < 0 || d.filter > 1 || d.camera.type < 0 || d.camera.type > 1)
+    fail(SWR_E_INVALID, "render: unknown sampler, filter or camera type");
+  if (!(d.threshold > 0.f)) fail(SWR_E_INVALID, "render: threshold must be positive");
+  if (d.max_depth < 0 || d.max_depth > 1000000) fail(SWR_E_INVALID, "render: max depth out of range");
+  if (d.camera.type == SWR_CAMERA_THINLENS && !(d.camera.aperture > 0.f && d.camera.focal_length > 0.f))
+    fail(SWR_E_INVALID, "render: thin lens needs positive focal length and aperture");
+  FrameParams p{};
+  p.width = (uint32_t)d.width;
+  p.height = (uint32_t)d.height;
+  p.spp = spp_of(d);
+  p.base = d.base > 0 ? (uint32_t)d.base : 1u;
+  if (d.sampler != SWR_SAMPLER_INDEPENDENT && p.base * p.base != p.spp)
+    fail(SWR_E_INVALID, "render: stratified and grid samplers need spp = base^2 (Image.h:65)");
+  p.sampler = d.sampler;
+  p.filter = d.filter;
+  p.min_depth = d.min_depth;
+  p.max_depth = d.max_depth;
+  p.threshold = d.threshold;
+  p.iteration = d.iteration;
+  p.seed_lo = (uint32_t)d.seed;
+  p.seed_hi = (uint32_t)(d.seed >> 32);
+  p.cam = make_camera(d.camera);
+  if (s) {
+    p.num_lights = s->num_lights;
+    p.pick = s->num_lights ? 1.f / (float)s->num_lights : 0.f;
+    p.env = v3(s->env[0], s->env[1], s->env[2]);
+    p.env_nonzero = (s->env[0] != 0.f || s->env[1] != 0.f || s->env[2] != 0.f) ? 1 : 0;
+    float lo[3], sc[3];
+    const float bits[3] = {1024.f, 1024.f, 512.f};
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = s->lo[k];
+      const float ext = s->hi[k] - s->lo[k];
+      sc[k] = ext > 0.f ? bits[k] / ext : 0.f;
+    }
+    p.key_lo = make_float3(lo[0], lo[1], lo[2]);
+    p.key_scale = make_float3(sc[0], sc[1], sc[2]);
+  }
+  return p;
+}
+
+int bits_for(uint64_t max_value) {
+  int b = 1;
+  while (b < 32 && (max_value >> b) != 0) ++b;
+  return b;
+}
+
+template <class F>
+void with_traversal(uint32_t max_depth, F&& f) {
+  const uint32_t need = stack_need(max_depth);
+  if (need <= 32)
+    f(std::integral_constant<int, 32>(), std::integral_constant<int, 256>());
+  else if (need <= 64)
+    f(std::integral_constant<int, 64>(), std::integral_constant<int, 128>());
+  else if (need <= 128)
+    f(std::integral_constant<int, 128>(), std::integral_constant<int, 64>());
+  else
+    fail(SWR_E_INVALID, "render: BVH too deep for the traversal stack");
+}
+
+enum Ws {
+  kRaysA, kRaysB, kPathWA, kPathWB, kPathIdA, kPathIdB, kKeysA, kKeysB,
+  kHits, kHitKeys, kPerm, kSortAltK, kSortAltV,
+  kShadowRays, kShadowCol, kContribSid, kContribRgb, kContribPerm, kContribAltK, kContribAltV,
+  kRadiance, kSampleXY, kCounters, kPixels
+};
+
+template <class T>
+T* ws(swr_context* ctx, Ws w) {
+  return static_cast<T*>(ctx->render_ws[w].ptr);
+}
+
+struct StageClock {
+  hipEvent_t ev[8];
+  StageClock() {
+    for (auto& e : ev) SWR_HIP(hipEventCreate(&e));
+  }
+  ~StageClock() {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  float ms(int a, int b) {
+    float v = 0.f;
+    SWR_HIP(hipEventElapsedTime(&v, ev[a], ev[b]));
+    return v;
+  }
+};
+
+void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_desc& desc, float* pixels_host,
+                  float* samples_host, swr_stats* stats) {
+  SWR_HIP(hipSetDevice(ctx->device));
+  const DeviceScene& s = scene->dev;
+  const FrameParams p = make_params(desc, &s);
+  const uint64_t total = (uint64_t)p.width * p.height * p.spp;
+  if (total > 0xFFFFFFFFull) fail(SWR_E_INVALID, "render: more than 2^32 - 1 samples in one iteration");
+  const int exp = std::max(10, std::min(desc.bin_exponent > 0 ? desc.bin_exponent : 20, 24));
+  const uint32_t cap = (uint32_t)std::min<uint64_t>(uint64_t(1) << exp, std::max<uint64_t>(total, 64));
+  const uint32_t contrib_cap = 4u * cap;
+  hipStream_t st = ctx->stream;
+
+  // Queues (two of each for the current and the next wave) and per-sample state.
+  auto need = [&](Ws w, size_t bytes) { ensure(ctx, ctx->render_ws[w], bytes); };
+  need(kRaysA, 32ull * cap);
+  need(kRaysB, 32ull * cap);
+  need(kPathWA, 16ull * cap);
+  need(kPathWB, 16ull * cap);
+  need(kPathIdA, 8ull * cap);
+  need(kPathIdB, 8ull * cap);
+  need(kKeysA, 4ull * cap);
+  need(kKeysB, 4ull * cap);
+  need(kHits, 16ull * cap);
+  need(kHitKeys, 4ull * cap);
+  need(kPerm, 4ull * cap);
+  need(kSortAltK, 4ull * cap);
+  need(kSortAltV, 4ull * cap);
+  need(kShadowRays, 32ull * cap);
+  need(kShadowCol, 16ull * cap);
+  need(kContribSid, 4ull * contrib_cap);
+  need(kContribRgb, 16ull * contrib_cap);
+  need(kContribPerm, 4ull * contrib_cap);
+  need(kContribAltK, 4ull * contrib_cap);
+  need(kContribAltV, 4ull * contrib_cap);
+  need(kRadiance, 16ull * total);
+  need(kSampleXY, 8ull * total);
+  need(kCounters, 64);
+  need(kPixels, 12ull * p.width * p.height);
+  sort_reserve(ctx, contrib_cap);
+
+  uint32_t* counters = ws<uint32_t>(ctx, kCounters);
+  float4* rays[2] = {ws<float4>(ctx, kRaysA), ws<float4>(ctx, kRaysB)};
+  float4* path_w[2] = {ws<float4>(ctx, kPathWA), ws<float4>(ctx, kPathWB)};
+  uint2* path_id[2] = {ws<uint2>(ctx, kPathIdA), ws<uint2>(ctx, kPathIdB)};
+  uint32_t* keys[2] = {ws<uint32_t>(ctx, kKeysA), ws<uint32_t>(ctx, kKeysB)};
+  float4* radiance = ws<float4>(ctx, kRadiance);
+  float2* sxy = ws<float2>(ctx, kSampleXY);
+  const int hit_bits = bits_for((uint64_t)s.num_tris);  // keys are prim + 1 <= num_tris
+  const int sid_bits = bits_for(total - 1);
+
+  StageClock clk;
+  swr_stats acc{};
+  hipEvent_t ev_start, ev_end;
+  SWR_HIP(hipEventCreate(&ev_start));
+  SWR_HIP(hipEventCreate(&ev_end));
+  struct EvGuard {
+    hipEvent_t a, b;
+    ~EvGuard() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } evg{ev_start, ev_end};
+
+  zero_u32_kernel<<<1, 64, 0, st>>>(counters, 16);
+  SWR_HIP(hipEventRecord(ev_start, st));
+
+  // Folds the pending contributions into the per-sample radiance.
+  uint32_t pending = 0;
+  auto flush = [&]() {
+    if (!pending) return;
+    uint32_t *sk = nullptr, *sv = nullptr;
+    sort_pairs(ctx, ws<uint32_t>(ctx, kContribSid), ws<uint32_t>(ctx, kContribPerm), ws<uint32_t>(ctx, kContribAltK),
+               ws<uint32_t>(ctx, kContribAltV), pending, sid_bits, true, &sk, &sv, st);
+    fold_kernel<<<blocks(pending), 256, 0, st>>>(pending, sk, sv, ws<float4>(ctx, kContribRgb), radiance);
+    SWR_HIP(hipGetLastError());
+    acc.contributions += pending;
+    pending = 0;
+    zero_u32_kernel<<<1, 64, 0, st>>>(counters + 2, 1);
+  };
+
+  uint32_t cur = 0, n_active = 0;
+  uint64_t next_sid = 0;
+  while (true) {
+    if (ctx->cancel.load()) fail(SWR_E_CANCELLED, "render cancelled");
+    const uint32_t k = (uint32_t)std::min<uint64_t>(cap - n_active, total - next_sid);
+    const uint32_t n = n_active + k;
+    if (n == 0) break;
+    if (pending + 2ull * n > contrib_cap) {
+      flush();
+    }
+    SWR_HIP(hipEventRecord(clk.ev[0], st));
+    if (k)
+      raygen_kernel<<<blocks(k), 256, 0, st>>>(p, (uint32_t)next_sid, k, n_active, rays[cur], path_w[cur],
+                                               path_id[cur], keys[cur], sxy, radiance);
+    SWR_HIP(hipEventRecord(clk.ev[1], st));
+    uint32_t *rk = nullptr, *perm = nullptr;
+    sort_pairs(ctx, keys[cur], ws<uint32_t>(ctx, kPerm), ws<uint32_t>(ctx, kSortAltK), ws<uint32_t>(ctx, kSortAltV), n,
+               32, true, &rk, &perm, st);
+    SWR_HIP(hipEventRecord(clk.ev[2], st));
+    with_traversal(s.max_depth, [&](auto kStack, auto kBlock) {
+      trace_queue_kernel<kStack.value, kBlock.value><<<blocks(n, kBlock.value), kBlock.value, 0, st>>>(
+          s.nodes, s.leaf_tris, perm, n, rays[cur], ws<uint4>(ctx, kHits), ws<uint32_t>(ctx, kHitKeys));
+    });
+    SWR_HIP(hipEventRecord(clk.ev[3], st));
+    // The ray permutation is consumed; the hit sort reuses the same buffers.
+    uint32_t *hk = nullptr, *hperm = nullptr;
+    sort_pairs(ctx, ws<uint32_t>(ctx, kHitKeys), perm == ws<uint32_t>(ctx, kPerm) ? ws<uint32_t>(ctx, kSortAltV)
+                                                                                   : ws<uint32_t>(ctx, kPerm),
+               keys[cur], perm, n, hit_bits, true, &hk, &hperm, st);
+    SWR_HIP(hipEventRecord(clk.ev[4], st));
+    zero_u32_kernel<<<1, 64, 0, st>>>(counters, 2);
+    ShadeOut so;
+    const int nxt = cur ^ 1;
+    so.next = QueueOut{rays[nxt], path_w[nxt], path_id[nxt], keys[nxt]};
+    so.shadow_rays = ws<float4>(ctx, kShadowRays);
+    so.shadow_col = ws<float4>(ctx, kShadowCol);
+    so.contrib_sid = ws<uint32_t>(ctx, kContribSid);
+    so.contrib_rgb = ws<float4>(ctx, kContribRgb);
+    so.counters = counters;
+    shade_kernel<<<blocks(n), 256, 0, st>>>(s, p, hperm, n, rays[cur], path_w[cur], path_id[cur],
+                                            ws<uint4>(ctx, kHits), so);
+    SWR_HIP(hipEventRecord(clk.ev[5], st));
+    with_traversal(s.max_depth, [&](auto kStack, auto kBlock) {
+      shadow_kernel<kStack.value, kBlock.value><<<blocks(n, kBlock.value), kBlock.value, 0, st>>>(
+          s.nodes, s.leaf_tris, counters + 1, so.shadow_rays, so.shadow_col, so.contrib_sid, so.contrib_rgb,
+          counters + 2);
+    });
+    SWR_HIP(hipEventRecord(clk.ev[6], st));
+    SWR_HIP(hipGetLastError());
+    uint32_t host_counts[4];
+    SWR_HIP(hipMemcpyAsync(host_counts, counters, 16, hipMemcpyDeviceToHost, st));
+    SWR_HIP(hipStreamSynchronize(st));
+    acc.ms_raygen += clk.ms(0, 1);
+    acc.ms_sort_rays += clk.ms(1, 2);
+    acc.ms_trace += clk.ms(2, 3);
+    acc.ms_sort_hits += clk.ms(3, 4);
+    acc.ms_shade += clk.ms(4, 5);
+    acc.ms_shadow += clk.ms(5, 6);
+    acc.camera_rays += k;
+    acc.closest_rays += n;
+    acc.shadow_rays += host_counts[1];
+    acc.waves += 1;
+    acc.max_wave = std::max(acc.max_wave, n);
+    pending = host_counts[2];
+    n_active = host_counts[0];
+    next_sid += k;
+    cur ^= 1;
+  }
+  SWR_HIP(hipEventRecord(clk.ev[0], st));
+  flush();
+  SWR_HIP(hipEventRecord(clk.ev[1], st));
+  dim3 fgrid(blocks(p.width, 64), p.height);
+  filter_kernel<<<fgrid, 64, 0, st>>>(p, radiance, sxy, ws<float>(ctx, kPixels));
+  SWR_HIP(hipGetLastError());
+  SWR_HIP(hipEventRecord(clk.ev[2], st));
+  SWR_HIP(hipEventRecord(ev_end, st));
+  if (pixels_host)
+    SWR_HIP(hipMemcpyAsync(pixels_host, ws<float>(ctx, kPixels), 12ull * p.width * p.height, hipMemcpyDeviceToHost,
+                           st));
+  if (samples_host) {
+    // Reuse the contribution buffers as staging for the 5-float sample records when they fit.
+    DeviceBuffer staging;
+    ensure(ctx, staging, 20ull * total);
+    pack_samples_kernel<<<blocks(total), 256, 0, st>>>(total, sxy, radiance, static_cast<float*>(staging.ptr));
+    SWR_HIP(hipMemcpyAsync(samples_host, staging.ptr, 20ull * total, hipMemcpyDeviceToHost, st));
+    SWR_HIP(hipStreamSynchronize(st));
+    release(ctx, staging);
+  }
+  SWR_HIP(hipStreamSynchronize(st));
+  acc.ms_accumulate += clk.ms(0, 1);
+  acc.ms_filter = clk.ms(1, 2);
+  float total_ms = 0.f;
+  SWR_HIP(hipEventElapsedTime(&total_ms, ev_start, ev_end));
+  acc.ms_total = total_ms;
+  if (stats) *stats = acc;
+}
+
+}  // namespace
+}  // namespace swr
+
+using swr::Error;
+
+extern "C" {
+
+int swr_raygen(swr_context* ctx, const swr_render_desc* desc, uint32_t sid0, size_t n, float* rays_dev,
+               float* xy_dev) {
+  if (!ctx || !desc) return SWR_E_INVALID;
+  try {
+    const swr::FrameParams p = swr::make_params(*desc, nullptr);
+    const uint64_t total = (uint64_t)p.width * p.height * p.spp;
+    if (n && (!rays_dev || (reinterpret_cast<uintptr_t>(rays_dev) & 15u)))
+      swr::fail(SWR_E_INVALID, "raygen: rays must be a 16-byte aligned device buffer");
+    if ((uint64_t)sid0 + n > total) swr::fail(SWR_E_INVALID, "raygen: sample ids beyond width*height*spp");
+    if (n) {
+      swr::Timer timer(ctx);
+      swr::raygen_kernel<<<swr::blocks(n), 256, 0, ctx->stream>>>(p, sid0, (uint32_t)n, 0u,
+                                                                  reinterpret_cast<float4*>(rays_dev), nullptr, nullptr,
+                                                                  nullptr, reinterpret_cast<float2*>(xy_dev), nullptr);
+      SWR_HIP(hipGetLastError());
+      timer.stop();
+      SWR_HIP(hipStreamSynchronize(ctx->stream));
+      timer.finish();
+    }
+    return SWR_OK;
+  } catch (const Error& e) {
+    ctx->err = e.msg;
+    return e.code;
+  }
+}
+
+int swr_render(swr_context* ctx, const swr_scene* scene, const swr_render_desc* desc, float* pixels, float* samples,
+               swr_stats* stats) {
+  if (!ctx || !scene || !desc) return SWR_E_INVALID;
+  try {
+    if (!scene->committed) swr::fail(SWR_E_INVALID, "render: scene not committed");
+    swr::render_frame(ctx, scene, *desc, pixels, samples, stats);
+    return SWR_OK;
+  } catch (const Error& e) {
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->err = e.what();
+    return SWR_E_INVALID;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// scene.h -- device scene layout shared by the BVH build, traversal and shading kernels.
+//
+// The reference hands Embree one triangle mesh per object and light, float4 vertices and
+// u32x3 indices (src/core/Pathtracer.cpp:178-201, :242-265), and keeps per-object shader and
+// light records on the host (Scene.h:17-25). Here the whole scene lives in HBM:
+//   geometry (global triangle order, geomID-major): pos float4 per vertex, tri uint4
+//     (i0, i1, i2, geomID) per triangle, optional uv float2 per vertex;
+//   records: GeomRec per geomID, ShaderRec per shader, LightRec per quad light;
+//   BVH4 nodes (128 B each) and the leaf triangles in BVH order (48 B each: v0 with the global
+//     triangle id in w, e1 = v0 - v1 with the geometry mask in w, e2 = v2 - v0).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "swr_internal.h"
+
+namespace swr {
+
+// BVH4 node. Child c is internal when child[c] < 0x80000000 (node index), a leaf when bit 31 is
+// set: bits 30..27 hold count-1 (1..16 triangles) and bits 26..0 the first leaf triangle.
+// Unused slots hold kNoChild and a box far outside any ray's range.
+struct alignas(16) Node4 {
+  float4 lox, hix, loy, hiy, loz, hiz;
+  uint4 child;
+  uint4 meta;  // x: number of children, y: depth, z/w: unused
+};
+static_assert(sizeof(Node4) == 128, "BVH4 node must be 128 bytes");
+
+constexpr uint32_t kNoChild = 0xFFFFFFFFu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr float kEmptyBox = 1e30f;
+__host__ __device__ inline uint32_t leaf_code(uint32_t first, uint32_t count) {
+  return kLeafBit | ((count - 1u) << 27) | first;
+}
+__host__ __device__ inline uint32_t leaf_first(uint32_t c) { return c & 0x07FFFFFFu; }
+__host__ __device__ inline uint32_t leaf_count(uint32_t c) { return ((c >> 27) & 15u) + 1u; }
+
+struct GeomRec {
+  int32_t shader;   // index into the shader table
+  int32_t light;    // light index or -1 (Scene::shaders_to_lights, Pathtracer.cpp:228-230)
+  uint32_t mask;    // 0xFFFFFFFF objects, 0xF0000000 lights (Pathtracer.cpp:267)
+  uint32_t first_tri;
+};
+
+struct ShaderRec {
+  int32_t type;     // SWR_SHADER_*
+  float colour[3];
+  float reflectance;
+  float roughness;
+  float pad[2];
+};
+
+// QuadLight after construct() (QuadLight.cpp:5-39): transform T*R*S(1,sx,sy) and normal.
+struct LightRec {
+  float rot[9];     // row-major Rx*Ry*Rz
+  float t[3];
+  float sx, sy;
+  float n[3];
+  float intensity;
+  float pad[2];
+};
+
+struct DeviceScene {
+  const float4* pos = nullptr;
+  const uint4* tri = nullptr;
+  const float2* uv = nullptr;
+  const GeomRec* geoms = nullptr;
+  const ShaderRec* shaders = nullptr;
+  const LightRec* lights = nullptr;
+  const Node4* nodes = nullptr;
+  const float4* leaf_tris = nullptr;  // 3 float4 per triangle in BVH order
+  uint32_t num_tris = 0, num_geoms = 0, num_lights = 0, num_nodes = 0;
+  uint32_t max_depth = 0;             // BVH4 depth (root = 0)
+  float env[3] = {0.f, 0.f, 0.f};
+  float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};  // scene bounds
+};
+
+// ---- bvh.hip ----
+// Builds the BVH4 of the scene's triangles on the device: 30-bit centroid Morton codes, radix
+// sort, Karras hierarchy, bottom-up boxes, BVH4 collapse and the BVH-order triangle array.
+struct BvhBuild {
+  DeviceBuffer nodes, leaf_tris;
+  uint32_t num_nodes = 0, max_depth = 0;
+};
+void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const GeomRec* geoms, uint32_t num_tris,
+               const float lo[3], const float hi[3], int leaf_max, BvhBuild* out);
+
+// ---- trace.hip ----
+// Closest hit (hits: t, u, v bits and global triangle id) and any-hit occlusion queries.
+// keys (optional) receives the hit shading key (prim + 1, misses 0) of every ray.
+void launch_intersect(const DeviceScene& s, const float4* rays, uint32_t n, uint32_t mask, uint4* hits,
+                      uint32_t* keys, hipStream_t stream);
+void launch_occluded(const DeviceScene& s, const float4* rays, uint32_t n, uint32_t mask, uint32_t* flags,
+                     hipStream_t stream);
+// Stack entries one traversal needs for a BVH4 of the given depth: every level leaves at most
+// three siblings behind, and the deepest node may push four before popping one.
+inline uint32_t stack_need(uint32_t max_depth) { return 3u * max_depth + 4u; }
+uint32_t max_supported_depth();
+
+}  // namespace swr
+
+struct swr_scene {
+  swr_context* ctx = nullptr;
+  struct Mesh {
+    std::vector<float> pos4;
+    std::vector<uint32_t> idx;
+    std::vector<float> uv;
+    int32_t shader = 0;
+  };
+  std::vector<Mesh> objects;
+  std::vector<swr_shader_desc> shaders;
+  std::vector<swr_quad_light_desc> light_descs;
+  float env[3] = {0.f, 0.f, 0.f};
+  bool committed = false;
+  int leaf_max = 4;
+
+  // committed
+  std::vector<swr::GeomRec> geoms;
+  std::vector<swr::ShaderRec> all_shaders;
+  std::vector<swr::LightRec> lights;
+  swr::DeviceBuffer d_pos, d_tri, d_uv, d_geoms, d_shaders, d_lights;
+  swr::BvhBuild bvh;
+  swr::DeviceScene dev;
+};

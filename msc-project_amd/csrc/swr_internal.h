@@ -56,6 +56,7 @@ struct swr_context {
   uint64_t in_use = 0;
   std::unordered_map<void*, size_t> allocs;
   swr::SortWorkspace sort;
+  swr::DeviceBuffer render_ws[24];           // wavefront queues of the render loop (render.hip)
   int num_cus = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // brackets of the last timed entry point
   double last_ms = 0.0;                     // its device time

@@ -1,0 +1,174 @@
+// traverse.h -- device-side BVH4 traversal of one ray (closest hit and any hit).
+//
+// Replaces Embree's rtcIntersect (src/core/RayIntersect.cpp:7-12) and rtcOccluded
+// (src/core/Integrator.cpp:153-174). One lane per ray, wave64; each lane keeps its traversal
+// stack in LDS (entry k of lane l at stack[k * BLOCK + l], so a wave's pushes and pops touch
+// 64 consecutive words). Closest-hit traversal visits the children of a node nearest first
+// and pushes the rest far to near.
+//
+// Arithmetic follows the CPU restatement exactly (no contraction): slabs are (lo - o) * (1/d)
+// with zero direction components replaced by +-2^-80, and the triangle test is Moller-Trumbore
+// in Embree 2's arrangement (e1 = v0 - v1, e2 = v2 - v0, Ng = e1 x e2; hit accepted when
+// tmin < t < tmax). The closest hit is the smallest t, and at exactly equal t the smaller global
+// triangle id, so the result does not depend on the tree or the visiting order.
+#pragma once
+
+#include "scene.h"
+
+namespace swr {
+
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+// Boxes are accepted when entry <= exit * (1 + 2^-22): a margin for rounding in the slab
+// arithmetic so that a triangle lying on its box face is never culled.
+constexpr float kBoxSlack = 1.00000024f;
+
+struct RayState {
+  float ox, oy, oz, tmin;
+  float dx, dy, dz, tmax;
+  float ix, iy, iz;
+};
+
+__device__ inline float safe_inv(float d) {
+  const float tiny = 8.27180613e-25f;  // 2^-80
+  return 1.f / (fabsf(d) > tiny ? d : copysignf(tiny, d));
+}
+
+__device__ inline RayState make_ray(float4 o, float4 d) {
+  RayState r;
+  r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = o.w;
+  r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = d.w;
+  r.ix = safe_inv(d.x);
+  r.iy = safe_inv(d.y);
+  r.iz = safe_inv(d.z);
+  return r;
+}
+
+// Slab entry distance of one child box, or +inf when the box is missed.
+__device__ inline float slab(const RayState& r, float lx, float hx, float ly, float hy, float lz, float hz, float tfar) {
+  const float tx0 = (lx - r.ox) * r.ix, tx1 = (hx - r.ox) * r.ix;
+  const float ty0 = (ly - r.oy) * r.iy, ty1 = (hy - r.oy) * r.iy;
+  const float tz0 = (lz - r.oz) * r.iz, tz1 = (hz - r.oz) * r.iz;
+  const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tmin));
+  const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tfar));
+  return tn <= tf * kBoxSlack ? tn : __int_as_float(0x7F800000);
+}
+
+// Triangle test; returns true with t, u, v for a hit in (tmin, tmax).
+__device__ inline bool tri_test(const RayState& r, float4 v0, float4 e1, float4 e2, float* t, float* u, float* v) {
+  const float ngx = e1.y * e2.z - e1.z * e2.y;
+  const float ngy = e1.z * e2.x - e1.x * e2.z;
+  const float ngz = e1.x * e2.y - e1.y * e2.x;
+  const float cx = v0.x - r.ox, cy = v0.y - r.oy, cz = v0.z - r.oz;
+  const float rx = r.dy * cz - r.dz * cy;
+  const float ry = r.dz * cx - r.dx * cz;
+  const float rz = r.dx * cy - r.dy * cx;
+  const float den = ngx * r.dx + ngy * r.dy + ngz * r.dz;
+  const float aden = fabsf(den);
+  const float sgn = den < 0.f ? -1.f : 1.f;
+  const float U = (rx * e2.x + ry * e2.y + rz * e2.z) * sgn;
+  const float V = (rx * e1.x + ry * e1.y + rz * e1.z) * sgn;
+  if (!(den != 0.f && U >= 0.f && V >= 0.f && U + V <= aden)) return false;
+  const float T = (ngx * cx + ngy * cy + ngz * cz) * sgn;
+  if (!(T > aden * r.tmin && T < aden * r.tmax)) return false;
+  const float inv = 1.f / aden;
+  *t = T * inv;
+  *u = U * inv;
+  *v = V * inv;
+  return true;
+}
+
+__device__ inline void cswap(uint32_t& a, uint32_t& b) {
+  const uint32_t lo = min(a, b), hi = max(a, b);
+  a = lo;
+  b = hi;
+}
+
+struct HitResult {
+  float t, u, v;
+  uint32_t id;  // global triangle id or kMiss
+};
+
+// Traverses one ray. kAny: returns on the first hit (id of that triangle), else the closest.
+// st points at this lane's stack column in LDS (stride kBlock).
+template <int kBlock, bool kAny>
+__device__ inline HitResult traverse(const Node4* __restrict__ nodes, const float4* __restrict__ leaf_tris,
+                                     const RayState& r, uint32_t mask, uint32_t* st) {
+  HitResult best;
+  best.t = r.tmax;
+  best.u = best.v = __int_as_float(0x7FC00000);
+  best.id = kMiss;
+  const float inf = __int_as_float(0x7F800000);
+  int sp = 0;
+  uint32_t node = 0;  // root
+  for (;;) {
+    if (!(node & kLeafBit)) {
+      const Node4* nd = nodes + node;
+      const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+      const uint4 ch = nd->child;
+      // Unused slots (kNoChild) are never entered, whatever their box.
+      const float t0 = ch.x == kNoChild ? inf : slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, best.t);
+      const float t1 = ch.y == kNoChild ? inf : slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, best.t);
+      const float t2 = ch.z == kNoChild ? inf : slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, best.t);
+      const float t3 = ch.w == kNoChild ? inf : slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, best.t);
+      if (kAny) {
+        if (t0 != inf) st[(sp++) * kBlock] = ch.x;
+        if (t1 != inf) st[(sp++) * kBlock] = ch.y;
+        if (t2 != inf) st[(sp++) * kBlock] = ch.z;
+        if (t3 != inf) st[(sp++) * kBlock] = ch.w;
+      } else {
+        // Entry distances are >= tmin >= 0, so their bit patterns order like the floats; the low
+        // two bits carry the child slot.
+        uint32_t k0 = (__float_as_uint(t0) & ~3u) | 0u;
+        uint32_t k1 = (__float_as_uint(t1) & ~3u) | 1u;
+        uint32_t k2 = (__float_as_uint(t2) & ~3u) | 2u;
+        uint32_t k3 = (__float_as_uint(t3) & ~3u) | 3u;
+        cswap(k0, k1);
+        cswap(k2, k3);
+        cswap(k0, k2);
+        cswap(k1, k3);
+        cswap(k1, k2);
+        const uint32_t miss_key = 0x7F800000u;
+        const uint32_t cs[4] = {ch.x, ch.y, ch.z, ch.w};
+        // Push hit children far to near except the nearest (keys ascend, so a hit k3 implies
+        // hits k0..k2), then descend into the nearest.
+        if ((k3 & ~3u) < miss_key) st[(sp++) * kBlock] = cs[k3 & 3u];
+        if ((k2 & ~3u) < miss_key) st[(sp++) * kBlock] = cs[k2 & 3u];
+        if ((k1 & ~3u) < miss_key) st[(sp++) * kBlock] = cs[k1 & 3u];
+        if ((k0 & ~3u) < miss_key) {
+          node = cs[k0 & 3u];
+          continue;
+        }
+      }
+    } else {
+      const uint32_t first = leaf_first(node), count = leaf_count(node);
+      for (uint32_t k = 0; k < count; ++k) {
+        const float4* tp = leaf_tris + 3 * (size_t)(first + k);
+        const float4 e1 = tp[1];
+        if ((__float_as_uint(e1.w) & mask) == 0u) continue;
+        const float4 v0 = tp[0], e2 = tp[2];
+        float t, u, v;
+        if (tri_test(r, v0, e1, e2, &t, &u, &v)) {
+          const uint32_t id = __float_as_uint(v0.w);
+          if (kAny) {
+            best.t = t;
+            best.u = u;
+            best.v = v;
+            best.id = id;
+            return best;
+          }
+          if (best.id == kMiss || t < best.t || (t == best.t && id < best.id)) {
+            best.t = t;
+            best.u = u;
+            best.v = v;
+            best.id = id;
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    node = st[(--sp) * kBlock];
+  }
+  return best;
+}
+
+}  // namespace swr

@@ -256,3 +256,202 @@ class DeviceArray:
         if self.ptr and self.ctx.h:
             self.ctx.check(self.ctx.lib.swr_device_free(self.ctx.h, self.ptr))
         self.ptr = 0
+
+
+def render_desc(r: dict, cls=RenderDesc):
+    """RenderDesc (or a layout-identical ctypes class) from a render dict of swr.scenes."""
+    cam = r["camera"]
+    cd = cls._fields_[-1][1](cam["type"], (C.c_float * 3)(*cam["translation"]), (C.c_float * 3)(*cam["rotation"]),
+                             cam["focal_length"], cam.get("focal_distance", 1000.0), cam.get("aperture", 5.6))
+    return cls(r["width"], r["height"], r["base"], r.get("spp", 0), r.get("sampler", 0), r.get("filter", 0),
+               r["min_depth"], r["max_depth"], r["threshold"], r.get("shading_size", 4096),
+               r.get("bin_exponent", 20), r.get("iteration", 0), r.get("seed", 0), cd)
+
+
+class Scene:
+    """A device-resident scene: swr_scene_* (Pathtracer::construct's scene half)."""
+
+    def __init__(self, ctx: Context, desc: dict | None = None, handle=None):
+        self.ctx, self.lib = ctx, ctx.lib
+        if handle is not None:
+            self.h = handle
+            return
+        h = _P()
+        ctx.check(self.lib.swr_scene_create(ctx.h, C.byref(h)))
+        self.h = h
+        if desc is not None:
+            for sh in desc["shaders"]:
+                self.add_shader(sh)
+            for m in desc["meshes"]:
+                self.add_mesh(m["pos4"], m["idx"], m.get("uv"), m["shader"])
+            for l in desc["lights"]:
+                self.add_quad_light(l)
+            self.set_environment(desc.get("environment", (0.0, 0.0, 0.0)))
+            self.commit()
+
+    def _rc(self, rc):
+        if rc < 0:
+            raise SwrError(rc, self.lib.swr_last_error(self.ctx.h).decode())
+        return rc
+
+    def add_shader(self, sh: dict) -> int:
+        d = ShaderDesc(sh["type"], (C.c_float * 3)(*sh["colour"]), sh.get("reflectance", 1.0),
+                       sh.get("roughness", 0.0))
+        return self._rc(self.lib.swr_scene_add_shader(self.h, C.byref(d)))
+
+    def add_mesh(self, pos4, idx, uv, shader: int) -> int:
+        pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        uvp = None
+        if uv is not None:
+            uv = np.ascontiguousarray(uv, dtype=np.float32)
+            uvp = uv.ctypes.data
+        return self._rc(self.lib.swr_scene_add_mesh(self.h, pos4.ctypes.data, pos4.shape[0], idx.ctypes.data,
+                                                    idx.shape[0], uvp, shader))
+
+    def add_quad_light(self, l: dict) -> int:
+        d = QuadLightDesc((C.c_float * 3)(*l["translation"]), (C.c_float * 3)(*l["rotation"]),
+                          (C.c_float * 2)(*l["scale"]), l["intensity"])
+        return self._rc(self.lib.swr_scene_add_quad_light(self.h, C.byref(d)))
+
+    def set_environment(self, rgb):
+        self._rc(self.lib.swr_scene_set_environment(self.h, (C.c_float * 3)(*rgb)))
+
+    def commit(self):
+        self._rc(self.lib.swr_scene_commit(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.swr_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_triangles(self) -> int:
+        return self.lib.swr_scene_num_triangles(self.h)
+
+    @property
+    def num_geometries(self) -> int:
+        return self.lib.swr_scene_num_geometries(self.h)
+
+    def bounds(self):
+        b = (C.c_float * 6)()
+        self._rc(self.lib.swr_scene_bounds(self.h, b))
+        return np.array(b[:3]), np.array(b[3:])
+
+    def bvh_info(self):
+        nodes, depth, valid = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._rc(self.lib.swr_scene_bvh_info(self.h, C.byref(nodes), C.byref(depth), C.byref(valid)))
+        return {"nodes": nodes.value, "max_depth": depth.value, "valid": bool(valid.value)}
+
+    def prim_to_geom(self, prim: int):
+        g, p = C.c_uint32(), C.c_uint32()
+        self._rc(self.lib.swr_scene_prim_to_geom(self.h, prim, C.byref(g), C.byref(p)))
+        return g.value, p.value
+
+    def intersect(self, rays: np.ndarray, mask: int = MASK_ALL) -> np.ndarray:
+        """Closest hits of host rays (n x 8) -> (n x 4) uint32 words (t, u, v bits, prim)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        d_r = self.ctx.upload(rays)
+        d_h = self.ctx.alloc(max(16 * n, 16))
+        try:
+            self.ctx.check(self.lib.swr_intersect(self.ctx.h, self.h, d_r.ptr, n, mask, d_h.ptr))
+            return d_h.download(np.uint32, (n, 4))
+        finally:
+            d_r.free()
+            d_h.free()
+
+    def occluded(self, rays: np.ndarray, mask: int = MASK_SHADOW) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        d_r = self.ctx.upload(rays)
+        d_f = self.ctx.alloc(max(4 * n, 4))
+        try:
+            self.ctx.check(self.lib.swr_occluded(self.ctx.h, self.h, d_r.ptr, n, mask, d_f.ptr))
+            return d_f.download(np.uint32, (n,))
+        finally:
+            d_r.free()
+            d_f.free()
+
+    def render(self, desc: RenderDesc, samples: bool = False):
+        """One iteration (Pathtracer::process): filtered pixels (H, W, 3) and optionally the
+        per-sample (x, y, r, g, b) records; returns (pixels, samples or None, stats dict)."""
+        spp = desc.spp if desc.spp > 0 else desc.base * desc.base
+        pix = np.zeros((desc.height, desc.width, 3), np.float32)
+        smp = np.zeros((desc.width * desc.height * spp, 5), np.float32) if samples else None
+        st = Stats()
+        self.ctx.check(self.lib.swr_render(self.ctx.h, self.h, C.byref(desc), pix.ctypes.data,
+                                           smp.ctypes.data if smp is not None else None, C.byref(st)))
+        return pix, smp, st.as_dict()
+
+
+def raygen(ctx: Context, desc: RenderDesc, sid0: int, n: int):
+    d_r = ctx.alloc(max(32 * n, 32))
+    d_xy = ctx.alloc(max(8 * n, 8))
+    try:
+        ctx.check(ctx.lib.swr_raygen(ctx.h, C.byref(desc), sid0, n, d_r.ptr, d_xy.ptr))
+        return d_r.download(np.float32, (n, 8)), d_xy.download(np.float32, (n, 2))
+    finally:
+        d_r.free()
+        d_xy.free()
+
+
+def load_yaml(ctx: Context, path: str):
+    """swr_scene_load_yaml: a committed Scene plus its RenderDesc from a reference scene file."""
+    h = _P()
+    desc = RenderDesc()
+    ctx.check(ctx.lib.swr_scene_load_yaml(ctx.h, path.encode(), C.byref(h), C.byref(desc)))
+    return Scene(ctx, handle=h), desc
+
+
+def write_exr(path: str, rgb: np.ndarray):
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    rc = load_library().swr_write_exr(path.encode(), rgb.ctypes.data, w, h)
+    if rc != SWR_OK:
+        raise SwrError(rc, "write_exr failed")
+
+
+class Pathtracer:
+    """The reference's host facade (Pathtracer.h:26-69) over the C ABI: load a scene file,
+    process() one iteration at a time into a progressive framebuffer, image(), clear(),
+    active(), terminate()."""
+
+    def __init__(self, filename: str, device: int = 0):
+        self.ctx = Context(device)
+        self.scene, self.desc = load_yaml(self.ctx, filename)
+        self.pixels = np.zeros((self.desc.height, self.desc.width, 3), np.float32)
+        self.iteration = 0
+        self._terminate = False
+
+    def image(self):
+        return self.pixels, self.desc.width, self.desc.height
+
+    def clear(self):
+        self.pixels[:] = 0.0
+        self.iteration = 0
+
+    def active(self) -> bool:
+        return not self._terminate
+
+    def terminate(self):
+        self._terminate = True
+        self.ctx.cancel()
+
+    def process(self) -> int:
+        self.desc.iteration = self.iteration
+        try:
+            pix, _, _ = self.scene.render(self.desc)
+        except SwrError as e:
+            if e.code == SWR_E_CANCELLED:
+                return 0
+            raise
+        self.pixels += pix
+        self.iteration += 1
+        return self.iteration

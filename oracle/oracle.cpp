@@ -972,35 +972,40 @@ int oracle_render(const oracle_scene* s, const oracle_render_desc* d, float* pix
   int prev_threads = omp_get_max_threads();
   if (threads > 0) omp_set_num_threads(threads);
 #endif
+  // Sample records are only kept when the caller wants them or the image is filtered.
   std::vector<float> own;
   float* smp = samples;
-  if (!smp) {
+  if (!smp && pixels) {
     own.assign(total * 5, 0.f);
     smp = own.data();
   }
+  if (smp) {
+    for (size_t i = 0; i < (size_t)sid_begin * 5; ++i) smp[i] = 0.f;
+    for (size_t i = (size_t)sid_end * 5; i < total * 5; ++i) smp[i] = 0.f;
+  }
   CameraXf cx = camera_xf(d->camera);
   uint64_t n_cam = 0, n_closest = 0, n_shadow = 0;
+  double checksum = 0.0;
   auto t0 = std::chrono::steady_clock::now();
-#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : n_cam, n_closest, n_shadow)
-  for (long long sid = 0; sid < (long long)total; ++sid) {
-    float* o = smp + 5 * sid;
-    if ((uint32_t)sid < sid_begin || (uint32_t)sid >= sid_end) {
-      o[0] = o[1] = o[2] = o[3] = o[4] = 0.f;
-      continue;
-    }
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : n_cam, n_closest, n_shadow, checksum)
+  for (long long sid = sid_begin; sid < (long long)sid_end; ++sid) {
     V3 org, dir;
     float sx, sy;
     camera_ray(d, cx, (uint32_t)sid, &org, &dir, &sx, &sy);
     ++n_cam;
     V3 L = trace_path(s, d, (uint32_t)sid, org, dir, &n_closest, &n_shadow);
-    o[0] = sx;
-    o[1] = sy;
-    o[2] = L.x;
-    o[3] = L.y;
-    o[4] = L.z;
+    checksum += (double)L.x + L.y + L.z;
+    if (smp) {
+      float* o = smp + 5 * sid;
+      o[0] = sx;
+      o[1] = sy;
+      o[2] = L.x;
+      o[3] = L.y;
+      o[4] = L.z;
+    }
   }
-  if (pixels) oracle_filter(d->filter, d->width, d->height, spp, smp, pixels);
   auto t1 = std::chrono::steady_clock::now();
+  if (pixels) oracle_filter(d->filter, d->width, d->height, spp, smp, pixels);
 #ifdef _OPENMP
   omp_set_num_threads(prev_threads);
 #endif

@@ -180,9 +180,9 @@ class Scene:
     def occluded(self, rays, mask=0x0FFFFFFF):
         return self._query(lib().oracle_occluded, rays, mask, 1)
 
-    def render(self, rdesc: RenderDesc, samples=False, sid_begin=0, sid_end=0, threads=0):
+    def render(self, rdesc: RenderDesc, samples=False, sid_begin=0, sid_end=0, threads=0, pixels=True):
         spp = rdesc.spp if rdesc.spp > 0 else rdesc.base * rdesc.base
-        pix = np.zeros((rdesc.height, rdesc.width, 3), dtype=np.float32)
+        pix = np.zeros((rdesc.height, rdesc.width, 3), dtype=np.float32) if pixels else None
         smp = np.zeros((rdesc.width * rdesc.height * spp, 5), dtype=np.float32) if samples else None
         st = Stats()
         assert lib().oracle_render(self.h, C.byref(rdesc), _ptr(pix), _ptr(smp), sid_begin, sid_end, threads,

@@ -1,0 +1,100 @@
+// Test driver for the product's host I/O (msc-project_amd/host/scene_io.cpp), built with g++ by
+// tests/test_host_io.py: prints parsed YAML / OBJ as JSON and converts floats to halves.
+#include <cstdio>
+#include <cstring>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+
+#include "scene_io.h"
+
+using swr::io::Node;
+
+static void json_str(const std::string& s) {
+  std::putchar('"');
+  for (char c : s) {
+    if (c == '"' || c == '\\') std::putchar('\\');
+    std::putchar(c);
+  }
+  std::putchar('"');
+}
+
+static void dump(const Node& n) {
+  if (n.kind == Node::kNull) {
+    std::printf("null");
+  } else if (n.kind == Node::kScalar) {
+    json_str(n.scalar);
+  } else if (n.kind == Node::kSeq) {
+    std::printf("[");
+    for (size_t i = 0; i < n.seq.size(); ++i) {
+      if (i) std::printf(",");
+      dump(n.seq[i]);
+    }
+    std::printf("]");
+  } else {
+    std::printf("{\"map\":[");
+    for (size_t i = 0; i < n.map.size(); ++i) {
+      if (i) std::printf(",");
+      std::printf("[");
+      json_str(n.map[i].first);
+      std::printf(",");
+      dump(n.map[i].second);
+      std::printf("]");
+    }
+    std::printf("]}");
+  }
+}
+
+template <class T>
+static void arr(const char* name, const std::vector<T>& v, bool last = false) {
+  std::printf("\"%s\":[", name);
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (i) std::printf(",");
+    std::cout << v[i];
+  }
+  std::printf("]%s", last ? "" : ",");
+  std::fflush(stdout);
+}
+
+int main(int argc, char** argv) {
+  if (argc < 3) return 2;
+  const std::string mode = argv[1];
+  try {
+    if (mode == "yaml") {
+      std::ifstream f(argv[2]);
+      std::stringstream ss;
+      ss << f.rdbuf();
+      auto docs = swr::io::parse_yaml(ss.str());
+      std::printf("[");
+      for (size_t i = 0; i < docs.size(); ++i) {
+        if (i) std::printf(",");
+        dump(docs[i]);
+      }
+      std::printf("]\n");
+    } else if (mode == "obj") {
+      std::cout.precision(9);
+      auto m = swr::io::load_obj(argv[2]);
+      std::printf("{");
+      arr("positions", m.positions);
+      arr("normals", m.normals);
+      arr("texcoords", m.texcoords);
+      arr("indices", m.indices, true);
+      std::printf("}\n");
+    } else if (mode == "half") {
+      for (int i = 2; i < argc; ++i) {
+        unsigned u = (unsigned)std::strtoul(argv[i], nullptr, 16);
+        float f;
+        std::memcpy(&f, &u, 4);
+        std::printf("%04x\n", swr::io::float_to_half(f));
+      }
+    } else {
+      return 2;
+    }
+  } catch (const std::exception& e) {
+    std::printf("ERROR %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -1,0 +1,185 @@
+"""Host I/O of the product (CPU only): the YAML subset against PyYAML, OBJ loading with
+tinyobj's semantics (tiny_obj_loader.cpp:276-407, 643-890) on hand-written fixtures, IEEE
+half rounding against numpy, and the EXR file layout read back by a reader written from the
+OpenEXR file-layout description."""
+import json
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+import yaml
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "msc-project_amd", "host")
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("harness") / "io_harness")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", HOST, os.path.join(ROOT, "tests", "harness", "io_harness.cpp"),
+                    os.path.join(HOST, "scene_io.cpp"), "-o", out], check=True)
+    return out
+
+
+def run(harness, *args):
+    r = subprocess.run([harness, *args], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def pyyaml_tree(node):
+    """PyYAML's composed node tree in the harness's JSON shape (duplicate keys kept in order)."""
+    if isinstance(node, yaml.ScalarNode):
+        return None if (node.tag.endswith(":null") and node.value in ("", "~", "null")) else node.value
+    if isinstance(node, yaml.SequenceNode):
+        return [pyyaml_tree(x) for x in node.value]
+    return {"map": [[k.value, pyyaml_tree(v)] for k, v in node.value]}
+
+
+SCENE = """\
+# setup document
+image: {width: 64, height: 48, sample base: 2}
+settings: {min depth: 2, max depth: 5, threshold: 0.01, bucket size: 16, shading size: 4096, bin exponent: 20}
+camera:
+  type: PinHole
+  translation: [0.0, 1.0, -52.5]
+  rotation: [0, 90, 0]   # yaw
+  focal length: 55
+filter: {type: Tent}
+sampler: {type: "Stratified"}
+---
+shader: {type: Lambert, texture: {type: Constant, colour: [0.5, 0.25, 0.125]}, reflectance: 1.0}
+shader: {type: Lambert,
+         texture: {type: Constant, colour: [1, 1, 1]}, reflectance: 0.5}
+object: {type: Polygon, filename: "a b.obj", translation: [0, 0, 0], rotation: [0, 0, 0], scale: [1, 1, 1], shader: 1}
+object: {type: Polygon, filename: c.obj, translation: [1, 2, 3], rotation: [10, 20, 30], scale: [2, 2, 2], shader: 0}
+light: {type: Quad, translation: [0, 1.98, 0], rotation: [0, 0, -90], scale: [0.5, 0.5], intensity: 12}
+list:
+  - 1
+  - [2, 3]
+  - {k: v}
+"""
+
+
+def test_yaml_subset_matches_pyyaml(harness, tmp_path):
+    p = tmp_path / "scene.yaml"
+    p.write_text(SCENE)
+    ours = json.loads(run(harness, "yaml", str(p)))
+    ref = [pyyaml_tree(d) for d in yaml.compose_all(SCENE)]
+    assert ours == ref
+    # repeated keys keep document order (Pathtracer.cpp:166-272 iterates them in order)
+    assert [k for k, _ in ours[1]["map"]][:5] == ["shader", "shader", "object", "object", "light"]
+
+
+def test_yaml_errors_are_reported(harness, tmp_path):
+    p = tmp_path / "bad.yaml"
+    p.write_text("image: {width: 3, height: 4\n---\n")
+    r = subprocess.run([harness, "yaml", str(p)], capture_output=True, text=True)
+    assert r.returncode == 1 and "ERROR" in r.stdout
+
+
+OBJ = """\
+# a quad, a pentagon with texture and normal indices, and negative (relative) indices
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+vt 0 0
+vt 1 0
+vt 1 1
+vt 0 1
+vn 0 0 1
+g group_is_ignored
+usemtl ignored
+f 1 2 3 4
+v 2 0 0
+f 1/1/1 2/2/1 3/3/1 5/4/1 4/4/1
+f -1 -2 -3
+"""
+
+
+def test_obj_fan_triangulation_and_dedup(harness, tmp_path):
+    p = tmp_path / "m.obj"
+    p.write_text(OBJ)
+    m = json.loads(run(harness, "obj", str(p)))
+    pos = np.array(m["positions"]).reshape(-1, 3)
+    idx = np.array(m["indices"]).reshape(-1, 3)
+    # quad -> 2 fan triangles, pentagon -> 3, triangle -> 1
+    assert idx.shape == (6, 3)
+    # corners dedup on (v, vn, vt): "1" and "1/1/1" are different vertices
+    # order of first use: 1 2 3 4 | 1/1/1 2/2/1 3/3/1 5/4/1 4/4/1 | 5 4 3 (relative -1 -2 -3)
+    want_pos = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                [0, 0, 0], [1, 0, 0], [1, 1, 0], [2, 0, 0], [0, 1, 0],
+                [2, 0, 0]]
+    np.testing.assert_array_equal(pos, want_pos)
+    np.testing.assert_array_equal(idx, [[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7], [4, 7, 8], [9, 3, 2]])
+    # texcoords / normals only for corners that carry them (5 such vertices)
+    assert len(m["texcoords"]) == 10 and len(m["normals"]) == 15
+
+
+def test_half_conversion_matches_numpy(harness):
+    rng = np.random.default_rng(0)
+    vals = np.concatenate([
+        rng.normal(0, 1, 400), rng.normal(0, 1e4, 200), rng.uniform(-1e-5, 1e-5, 200),
+        np.array([0.0, -0.0, 1.0, 65504.0, 65520.0, 70000.0, 2.0 ** -24, 2.0 ** -25, 2.0 ** -25 * 1.0001,
+                  np.inf, -np.inf, 5.960464477539063e-08, 6.097555160522461e-05]),
+    ]).astype(np.float32)
+    out = run(harness, "half", *["%08x" % v for v in vals.view(np.uint32)]).split()
+    ours = np.array([int(x, 16) for x in out], np.uint16)
+    np.testing.assert_array_equal(ours, vals.astype(np.float16).view(np.uint16))
+
+
+def read_exr(path):
+    """Minimal reader of an uncompressed single-part scanline EXR (OpenEXR file layout)."""
+    b = open(path, "rb").read()
+    magic, version = struct.unpack_from("<ii", b, 0)
+    assert magic == 20000630 and version == 2
+    pos, attrs = 8, {}
+    while b[pos] != 0:
+        name_end = b.index(b"\0", pos)
+        name = b[pos:name_end].decode()
+        type_end = b.index(b"\0", name_end + 1)
+        typ = b[name_end + 1:type_end].decode()
+        size = struct.unpack_from("<i", b, type_end + 1)[0]
+        attrs[name] = (typ, b[type_end + 5:type_end + 5 + size])
+        pos = type_end + 5 + size
+    pos += 1
+    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    chans, cb = [], attrs["channels"][1]
+    p = 0
+    while cb[p] != 0:
+        e = cb.index(b"\0", p)
+        chans.append((cb[p:e].decode(), struct.unpack_from("<i", cb, e + 1)[0]))
+        p = e + 17
+    offsets = struct.unpack_from("<%dQ" % h, b, pos)
+    img = {c: np.zeros((h, w), np.float16) for c, _ in chans}
+    for y, off in enumerate(offsets):
+        ly, size = struct.unpack_from("<ii", b, off)
+        assert ly == y and size == w * 2 * len(chans)
+        data = np.frombuffer(b, np.float16, count=w * len(chans), offset=off + 8).reshape(len(chans), w)
+        for k, (c, _) in enumerate(chans):
+            img[c][y] = data[k]
+    return attrs, chans, img
+
+
+def test_exr_layout_round_trip(tmp_path):
+    import swr
+    lib = swr.load_library()
+    rng = np.random.default_rng(4)
+    rgb = rng.uniform(0, 4, (5, 7, 3)).astype(np.float32)
+    rgb[0, 0] = [70000.0, 1e-8, 0.0]
+    path = str(tmp_path / "o.exr")
+    assert lib.swr_write_exr(path.encode(), rgb.ctypes.data, 7, 5) == 0
+    attrs, chans, img = read_exr(path)
+    assert [c for c, _ in chans] == ["A", "B", "G", "R"] and all(t == 1 for _, t in chans)  # HALF
+    assert attrs["compression"] == ("compression", b"\x00")
+    assert attrs["lineOrder"] == ("lineOrder", b"\x00")
+    assert list(attrs) == sorted(attrs)
+    np.testing.assert_array_equal(img["R"], rgb[:, :, 0].astype(np.float16))
+    np.testing.assert_array_equal(img["G"], rgb[:, :, 1].astype(np.float16))
+    np.testing.assert_array_equal(img["B"], rgb[:, :, 2].astype(np.float16))
+    assert np.all(img["A"] == 1.0)
+    assert lib.swr_write_exr(str(tmp_path / "none" / "x.exr").encode(), rgb.ctypes.data, 7, 5) == swr.SWR_E_IO

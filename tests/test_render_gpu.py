@@ -1,0 +1,139 @@
+"""GPU parity of the sorted wavefront render (Pathtracer::process replacement).
+
+Both sides draw every random number from Philox keyed by (seed, sample, iteration, depth,
+slot), so the GPU and the oracle trace the same paths: per-sample radiance agrees except for
+the rare path whose direction differs by a last-ulp transcendental and so hits elsewhere.
+The image test of the north star (independent streams, 3 sigma of the per-pixel Monte Carlo
+standard error) is checked as well, and the result must not depend on how rays are grouped
+into waves (queue capacity), which is the batched-equals-solo property."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_ffi as orc
+import swr
+from swr import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(desc, **over):
+    r = dict(desc["render"])
+    r.update(over)
+    return swr.render_desc(r), swr.render_desc(r, orc.RenderDesc)
+
+
+def _compare_samples(g, o, tol_frac):
+    """Per-sample radiance: near-identical for all but a small fraction of diverged paths."""
+    np.testing.assert_array_equal(g[:, 0:2], o[:, 0:2])  # film positions are bit-identical
+    d = np.abs(g[:, 2:5] - o[:, 2:5])
+    close = np.all(d <= 1e-4 * (1.0 + np.abs(o[:, 2:5])), axis=1)
+    assert close.mean() >= 1.0 - tol_frac, f"{(~close).sum()} of {len(close)} samples differ"
+    return close
+
+
+def test_raygen_matches_oracle(ctx):
+    for cam_type in (0, 1):
+        desc = scenes.box_room(width=40, height=30, base=3)
+        desc["render"]["camera"]["type"] = cam_type
+        desc["render"]["camera"]["aperture"] = 8.0
+        desc["render"]["camera"]["focal_distance"] = 200.0
+        for sampler in (0, 1, 2):
+            rd, od = _pair(desc, sampler=sampler)
+            n = 40 * 30 * 9
+            rg, xg = swr.raygen(ctx, rd, 0, n)
+            ro, xo = orc.raygen(od, 0, n)
+            np.testing.assert_array_equal(xg, xo)
+            np.testing.assert_allclose(rg, ro, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("filt", [0, 1])
+def test_box_room_render_matches_oracle_stream_for_stream(ctx, filt):
+    desc = scenes.box_room(width=64, height=48, base=3, sphere_level=3)
+    rd, od = _pair(desc, filter=filt)
+    sc = swr.Scene(ctx, desc)
+    oc = orc.Scene(desc)
+    pg, sg, st = sc.render(rd, samples=True)
+    po, so, ost = oc.render(od, samples=True)
+    close = _compare_samples(sg, so, 0.01)
+    assert st["camera_rays"] == 64 * 48 * 9
+    # ray counts agree up to the diverged paths
+    assert abs(st["closest_rays"] - ost.closest_rays) <= 0.02 * ost.closest_rays
+    # pixels: exact filter order, so pixels without a diverged sample agree tightly
+    rel = np.abs(pg - po) / (np.abs(po) + 1e-3)
+    assert np.median(rel) < 1e-5
+    assert np.mean(rel < 1e-3) > 0.95
+
+
+def test_render_is_deterministic_and_independent_of_wave_size(ctx):
+    desc = scenes.box_room(width=48, height=48, base=2, sphere_level=3)
+    sc = swr.Scene(ctx, desc)
+    outs = []
+    for exp in (10, 12, 20):
+        rd, _ = _pair(desc, bin_exponent=exp)
+        pix, smp, st = sc.render(rd, samples=True)
+        outs.append((pix, smp, st))
+    for pix, smp, st in outs[1:]:
+        np.testing.assert_array_equal(pix, outs[0][0])
+        np.testing.assert_array_equal(smp, outs[0][1])
+    assert outs[0][2]["waves"] > outs[2][2]["waves"]
+
+
+def test_glossy_terrain_matches_oracle(ctx):
+    desc = scenes.terrain(grid=64, groups=16, width=48, height=27, spp=4, max_depth=8)
+    rd, od = _pair(desc)
+    sc, oc = swr.Scene(ctx, desc), orc.Scene(desc)
+    _, sg, _ = sc.render(rd, samples=True)
+    _, so, _ = oc.render(od, samples=True)
+    _compare_samples(sg, so, 0.02)
+
+
+def test_images_agree_within_monte_carlo_error(ctx):
+    """Independent streams (different seeds): per-pixel means over K = 16 iterations agree within
+    3 sigma of the combined standard error for >= 99% of pixel-channels, none beyond 6 sigma."""
+    desc = scenes.box_room(width=24, height=24, base=4, sphere_level=3)
+    sc, oc = swr.Scene(ctx, desc), orc.Scene(desc)
+    K = 16
+    g, o = [], []
+    for it in range(K):
+        rd, _ = _pair(desc, iteration=it, seed=1000)
+        _, od = _pair(desc, iteration=it, seed=2000)
+        g.append(sc.render(rd)[0])
+        o.append(oc.render(od)[0])
+    g, o = np.array(g, np.float64), np.array(o, np.float64)
+    se = np.sqrt(g.var(axis=0, ddof=1) / K + o.var(axis=0, ddof=1) / K) + 1e-12
+    z = np.abs(g.mean(axis=0) - o.mean(axis=0)) / se
+    assert np.mean(z <= 3.0) >= 0.99
+    assert np.max(z) < 6.0
+
+
+def test_render_errors_and_cancel(ctx):
+    desc = scenes.box_room(width=16, height=16, base=2, sphere_level=1)
+    sc = swr.Scene(ctx, desc)
+    rd, _ = _pair(desc, base=3, spp=8, sampler=0)  # stratified needs spp = base^2
+    with pytest.raises(swr.SwrError):
+        sc.render(rd)
+    rd, _ = _pair(desc)
+    ctx.cancel()
+    with pytest.raises(swr.SwrError) as e:
+        sc.render(rd)
+    assert e.value.code == swr.SWR_E_CANCELLED
+    ctx.reset_cancel()
+    pix, _, _ = sc.render(rd)
+    assert np.isfinite(pix).all()
+
+
+def test_budget_refuses_large_render(ctx):
+    c2 = swr.Context(0)
+    desc = scenes.box_room(width=16, height=16, base=2, sphere_level=1)
+    sc = swr.Scene(c2, desc)
+    c2.set_budget(c2.bytes_in_use() + 4096)
+    rd, _ = _pair(desc)
+    with pytest.raises(swr.SwrError) as e:
+        sc.render(rd)
+    assert e.value.code == swr.SWR_E_BUDGET
+    c2.set_budget(0)
+    sc.render(rd)
+    sc.close()
+    c2.close()
