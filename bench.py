@@ -33,7 +33,25 @@ def parse():
     ap.add_argument("--cpu-samples", type=int, default=0, help="sample ids the CPU baseline traces (0 = auto)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sort-bench", action="store_true", help="also time the 16M-key sort of the sort/traversal bench")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain copy of known size (gather_kernel<4>, identity permutation, 2^26 "
+                         "records: 1280 MiB read, 1024 MiB written) to scale rocprof FETCH_SIZE/WRITE_SIZE")
     return ap.parse_args()
+
+
+def calibrate_copy(ctx):
+    import numpy as np
+    n = 1 << 26
+    src = ctx.alloc(16 * n)
+    perm = ctx.upload(np.arange(n, dtype=np.uint32))
+    dst = ctx.alloc(16 * n)
+    for _ in range(3):
+        ctx.gather(src, perm, n, 4, dst)
+    ms = ctx.last_device_ms()
+    for a in (src, perm, dst):
+        a.free()
+    return {"records": n, "bytes_read": 20 * n, "bytes_written": 16 * n, "ms": round(ms, 4),
+            "GB/s": round(36.0 * n / (ms * 1e-3) / 1e9, 1)}
 
 
 def main():
@@ -59,6 +77,7 @@ def main():
     gen_s = time.time() - t0
 
     ctx = swr.Context(0)
+    copy_cal = calibrate_copy(ctx) if args.calibrate_copy else None
     t0 = time.time()
     scene = swr.Scene(ctx, desc)
     build_s = time.time() - t0
@@ -139,6 +158,19 @@ def main():
         "bvh": info,
         "image_mean": float(np.mean(pix)),
     }
+
+    if copy_cal:
+        out["copy_calibration"] = copy_cal
+    traffic_file = os.path.join(ROOT, "profiles", "trace_queue_kernel_terrain_traffic.json")
+    if os.path.exists(traffic_file) and args.workload == "terrain":
+        # HBM bytes per traversal ray from the committed rocprofv3 --pmc summary (FETCH_SIZE +
+        # WRITE_SIZE per dispatch, scaled by the copy calibration), times this run's rays per launch.
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        per_ray = tr.get("hbm_bytes_per_ray")
+        if per_ray:
+            out["roofline"]["traffic"] = round(per_ray * agg["closest_rays"] / agg["waves"], 1)
+            out["roofline"]["traffic_per_ray"] = per_ray
 
     if args.sort_bench:
         n = 1 << 24

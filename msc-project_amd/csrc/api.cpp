@@ -108,6 +108,8 @@ void swr_destroy(swr_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (swr_scene* s : ctx->scenes) swr::detach_scene(s);
+  ctx->scenes.clear();
   for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
   ctx->allocs.clear();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

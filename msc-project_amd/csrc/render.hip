@@ -881,6 +881,7 @@ int swr_render(swr_context* ctx, const swr_scene* scene, const swr_render_desc* 
                swr_stats* stats) {
   if (!ctx || !scene || !desc) return SWR_E_INVALID;
   try {
+    if (scene->ctx != ctx) swr::fail(SWR_E_INVALID, "render: scene belongs to another (or a destroyed) context");
     if (!scene->committed) swr::fail(SWR_E_INVALID, "render: scene not committed");
     swr::render_frame(ctx, scene, *desc, pixels, samples, stats);
     return SWR_OK;

@@ -195,6 +195,15 @@ void commit_scene(swr_scene* s) {
 
 }  // namespace swr
 
+void swr::detach_scene(swr_scene* s) {
+  for (DeviceBuffer* b : {&s->d_pos, &s->d_tri, &s->d_uv, &s->d_geoms, &s->d_shaders, &s->d_lights, &s->bvh.nodes,
+                          &s->bvh.leaf_tris})
+    *b = DeviceBuffer{};
+  s->dev = DeviceScene{};
+  s->committed = false;
+  s->ctx = nullptr;
+}
+
 using swr::Error;
 
 template <class F>
@@ -219,25 +228,28 @@ int swr_scene_create(swr_context* ctx, swr_scene** out) {
   if (!ctx || !out) return SWR_E_INVALID;
   *out = new swr_scene();
   (*out)->ctx = ctx;
+  ctx->scenes.insert(*out);
   return SWR_OK;
 }
 
 void swr_scene_destroy(swr_scene* s) {
   if (!s) return;
-  swr_context* ctx = s->ctx;
-  (void)hipStreamSynchronize(ctx->stream);
-  for (swr::DeviceBuffer* b : {&s->d_pos, &s->d_tri, &s->d_uv, &s->d_geoms, &s->d_shaders, &s->d_lights,
-                               &s->bvh.nodes, &s->bvh.leaf_tris}) {
-    try {
-      swr::release(ctx, *b);
-    } catch (const Error&) {
+  if (swr_context* ctx = s->ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (swr::DeviceBuffer* b : {&s->d_pos, &s->d_tri, &s->d_uv, &s->d_geoms, &s->d_shaders, &s->d_lights,
+                                 &s->bvh.nodes, &s->bvh.leaf_tris}) {
+      try {
+        swr::release(ctx, *b);
+      } catch (const Error&) {
+      }
     }
+    ctx->scenes.erase(s);
   }
   delete s;
 }
 
 int swr_scene_add_shader(swr_scene* s, const swr_shader_desc* d) {
-  if (!s || !d) return SWR_E_INVALID;
+  if (!s || !d || !s->ctx) return SWR_E_INVALID;
   if (s->committed || d->type < 0 || d->type > SWR_SHADER_GLOSSY) {
     s->ctx->err = "scene_add_shader: scene committed or unknown shader type";
     return SWR_E_INVALID;
@@ -248,7 +260,7 @@ int swr_scene_add_shader(swr_scene* s, const swr_shader_desc* d) {
 
 int swr_scene_add_mesh(swr_scene* s, const float* pos4, uint32_t nverts, const uint32_t* idx3, uint32_t ntris,
                        const float* uv2, int32_t shader) {
-  if (!s) return SWR_E_INVALID;
+  if (!s || !s->ctx) return SWR_E_INVALID;
   if (s->committed || (nverts && !pos4) || (ntris && !idx3)) {
     s->ctx->err = "scene_add_mesh: scene committed or null buffer";
     return SWR_E_INVALID;
@@ -268,7 +280,7 @@ int swr_scene_add_mesh(swr_scene* s, const float* pos4, uint32_t nverts, const u
 }
 
 int swr_scene_add_quad_light(swr_scene* s, const swr_quad_light_desc* d) {
-  if (!s || !d) return SWR_E_INVALID;
+  if (!s || !d || !s->ctx) return SWR_E_INVALID;
   if (s->committed) {
     s->ctx->err = "scene_add_quad_light: scene committed";
     return SWR_E_INVALID;
@@ -278,13 +290,13 @@ int swr_scene_add_quad_light(swr_scene* s, const swr_quad_light_desc* d) {
 }
 
 int swr_scene_set_environment(swr_scene* s, const float rgb[3]) {
-  if (!s || !rgb || s->committed) return SWR_E_INVALID;
+  if (!s || !rgb || !s->ctx || s->committed) return SWR_E_INVALID;
   std::memcpy(s->env, rgb, 3 * sizeof(float));
   return SWR_OK;
 }
 
 int swr_scene_commit(swr_scene* s) {
-  if (!s) return SWR_E_INVALID;
+  if (!s || !s->ctx) return SWR_E_INVALID;
   return scene_guard(s->ctx, [&] {
     if (s->committed) swr::fail(SWR_E_INVALID, "scene already committed");
     swr::commit_scene(s);
@@ -400,7 +412,7 @@ int swr_scene_bvh_info(const swr_scene* s, uint32_t* nodes_out, uint32_t* depth_
 
 int swr_intersect(swr_context* ctx, const swr_scene* s, const float* rays_dev, size_t n, uint32_t mask,
                   uint32_t* hits_dev) {
-  if (!ctx || !s) return SWR_E_INVALID;
+  if (!ctx || !s || s->ctx != ctx) return SWR_E_INVALID;
   return scene_guard(ctx, [&] {
     if (!s->committed) swr::fail(SWR_E_INVALID, "intersect: scene not committed");
     if (n && (!rays_dev || !hits_dev || !aligned16(rays_dev) || !aligned16(hits_dev)))
@@ -417,7 +429,7 @@ int swr_intersect(swr_context* ctx, const swr_scene* s, const float* rays_dev, s
 
 int swr_occluded(swr_context* ctx, const swr_scene* s, const float* rays_dev, size_t n, uint32_t mask,
                  uint32_t* flags_dev) {
-  if (!ctx || !s) return SWR_E_INVALID;
+  if (!ctx || !s || s->ctx != ctx) return SWR_E_INVALID;
   return scene_guard(ctx, [&] {
     if (!s->committed) swr::fail(SWR_E_INVALID, "occluded: scene not committed");
     if (n && (!rays_dev || !flags_dev || !aligned16(rays_dev)))

@@ -9,6 +9,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "swr.h"
@@ -47,6 +48,8 @@ struct SortWorkspace {
 
 }  // namespace swr
 
+struct swr_scene;
+
 struct swr_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -57,6 +60,7 @@ struct swr_context {
   std::unordered_map<void*, size_t> allocs;
   swr::SortWorkspace sort;
   swr::DeviceBuffer render_ws[24];           // wavefront queues of the render loop (render.hip)
+  std::unordered_set<swr_scene*> scenes;     // live scenes; destroying the context detaches them
   int num_cus = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // brackets of the last timed entry point
   double last_ms = 0.0;                     // its device time
@@ -76,6 +80,10 @@ struct Timer {
     ctx->last_ms = ms;
   }
 };
+
+// Called by swr_destroy for every live scene: the scene forgets its (now freed) device memory
+// and context; its handle stays valid for swr_scene_destroy.
+void detach_scene(swr_scene* s);
 
 // Allocation accounted against the context budget.
 void* dev_alloc(swr_context* ctx, size_t bytes);

@@ -222,6 +222,10 @@ struct BlockParser {
   Node block(int indent) {
     Node n;
     if (i >= lines.size()) return n;
+    if (lines[i].text[0] == '{' || lines[i].text[0] == '[') {  // a flow collection on its own
+      const Line ln = lines[i++];
+      return inline_value(ln.text, ln.number);
+    }
     const bool seq = lines[i].text.rfind("- ", 0) == 0 || lines[i].text == "-";
     n.kind = seq ? Node::kSeq : Node::kMap;
     while (i < lines.size() && lines[i].indent == indent) {

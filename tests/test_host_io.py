@@ -183,3 +183,10 @@ def test_exr_layout_round_trip(tmp_path):
     np.testing.assert_array_equal(img["B"], rgb[:, :, 2].astype(np.float16))
     assert np.all(img["A"] == 1.0)
     assert lib.swr_write_exr(str(tmp_path / "none" / "x.exr").encode(), rgb.ctypes.data, 7, 5) == swr.SWR_E_IO
+
+
+def test_yaml_flow_documents_and_block_sequences(harness, tmp_path):
+    text = "{}\n---\n{a: 1, b: [x, 'y z']}\n---\nk:\n  - 1\n  - key: v\n    other: w\n"
+    p = tmp_path / "f.yaml"
+    p.write_text(text)
+    assert json.loads(run(harness, "yaml", str(p))) == [pyyaml_tree(d) for d in yaml.compose_all(text)]

@@ -1,0 +1,64 @@
+"""Scene files in the reference schema (two-document YAML + OBJ) load into the same device scene
+as the arrays they were written from, and the command-line renderer writes an EXR."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import swr
+from swr import scenes
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_yaml_scene_renders_like_the_arrays(ctx, tmp_path):
+    desc = scenes.box_room(width=40, height=32, base=2, sphere_level=2)
+    path = scenes.write_scene(desc, str(tmp_path), "room")
+    sc_file, rd_file = swr.load_yaml(ctx, path)
+    sc_arr = swr.Scene(ctx, desc)
+    assert sc_file.num_triangles == sc_arr.num_triangles
+    assert sc_file.num_geometries == sc_arr.num_geometries
+    rd = swr.render_desc(desc["render"])
+    for f in ("width", "height", "base", "sampler", "filter", "min_depth", "max_depth", "bin_exponent"):
+        assert getattr(rd_file, f) == getattr(rd, f), f
+    assert rd_file.threshold == np.float32(rd.threshold)
+    np.testing.assert_allclose(list(rd_file.camera.translation), list(rd.camera.translation), rtol=1e-7)
+    rd_file.seed = rd.seed
+    p_file, _, _ = sc_file.render(rd_file)
+    p_arr, _, _ = sc_arr.render(rd)
+    np.testing.assert_array_equal(p_file, p_arr)
+
+
+def test_yaml_defaults_and_strict_decoders(ctx, tmp_path):
+    (tmp_path / "t.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    (tmp_path / "ok.yaml").write_text(
+        "{}\n---\nshader: {type: Lambert, texture: {type: Constant, colour: [1, 1, 1]}, reflectance: 1}\n"
+        "object: {type: Polygon, filename: t.obj, translation: [0, 0, 0], rotation: [0, 0, 0], scale: [1, 1, 1], "
+        "shader: 0}\n")
+    sc, rd = swr.load_yaml(ctx, str(tmp_path / "ok.yaml"))
+    # defaults of Pathtracer.cpp:45-161
+    assert (rd.width, rd.height, rd.base, rd.min_depth, rd.max_depth) == (700, 500, 8, 2, 100)
+    assert rd.camera.type == swr.CAMERA_THINLENS and rd.camera.focal_length == 55.0
+    assert sc.num_triangles == 1
+    (tmp_path / "bad.yaml").write_text("image: {width: 10, height: 10}\n---\n{}\n")  # 2 of 3 keys
+    with pytest.raises(swr.SwrError) as e:
+        swr.load_yaml(ctx, str(tmp_path / "bad.yaml"))
+    assert e.value.code == swr.SWR_E_PARSE
+    with pytest.raises(swr.SwrError) as e:
+        swr.load_yaml(ctx, str(tmp_path / "missing.yaml"))
+    assert e.value.code == swr.SWR_E_IO
+
+
+def test_cli_writes_exr(tmp_path):
+    exe = os.path.join(ROOT, "msc-project_amd", "bin", "swr")
+    if not os.path.exists(exe):
+        pytest.skip("command-line renderer not built")
+    desc = scenes.box_room(width=32, height=24, base=2, sphere_level=1)
+    path = scenes.write_scene(desc, str(tmp_path), "room")
+    out = tmp_path / "o.exr"
+    r = subprocess.run([exe, "-o", str(out), path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    data = out.read_bytes()
+    assert data[:4] == bytes([0x76, 0x2F, 0x31, 0x01])

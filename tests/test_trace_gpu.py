@@ -31,6 +31,7 @@ def check_against_oracle(gpu, ora_bvh, ora_brute=None, allow_tie_frac=1e-3):
     same = np.all(gpu == ora_bvh, axis=1) | ((gpu[:, 3] == MISS) & (ora_bvh[:, 3] == MISS))
     assert same.mean() >= 1 - allow_tie_frac, f"{(~same).sum()} of {n} hits differ from the oracle"
     if ora_brute is not None:
+        gpu = gpu[:ora_brute.shape[0]]  # the brute force runs on a prefix of the rays
         hit_g = gpu[:, 3] != MISS
         hit_b = ora_brute[:, 3] != MISS
         agree = (gpu[:, 3] == ora_brute[:, 3])
