@@ -33,6 +33,7 @@ def parse():
     ap.add_argument("--cpu-samples", type=int, default=0, help="sample ids the CPU baseline traces (0 = auto)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sort-bench", action="store_true", help="also time the 16M-key sort of the sort/traversal bench")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain copy of known size (gather_kernel<4>, identity permutation, 2^26 "
                          "records: 1280 MiB read, 1024 MiB written) to scale rocprof FETCH_SIZE/WRITE_SIZE")
@@ -83,6 +84,9 @@ def main():
     build_s = time.time() - t0
     info = scene.bvh_info()
     rd = swr.render_desc(desc["render"])
+    t0 = time.time()
+    tuning = ctx.tuning() if args.no_autotune else ctx.autotune(scene, rd)
+    tune_s = time.time() - t0
 
     for w in range(args.warmup):
         rd.iteration = 1000 + w
@@ -155,6 +159,8 @@ def main():
         "waves_per_step": agg["waves"] // args.steps,
         "scene_gen_s": round(gen_s, 2),
         "scene_upload_bvh_s": round(build_s, 3),
+        "tuning": tuning,
+        "autotune_s": round(tune_s, 3),
         "bvh": info,
         "image_mean": float(np.mean(pix)),
     }

@@ -107,6 +107,15 @@ typedef struct {
   double ms_raygen, ms_sort_rays, ms_trace, ms_sort_hits, ms_shade, ms_shadow, ms_accumulate, ms_filter;
 } swr_stats;
 
+/* Launch shapes of the loop. The reference exposes the equivalent knobs as settings
+ * (bucket size, shading size, bin exponent; Settings.h:16-24); these are chosen per context by
+ * swr_autotune or set directly. */
+typedef struct {
+  int32_t sort_keys_per_thread;  /* onesweep tile = 256 * this: 8, 12, 16, 20 or 24 */
+  int32_t ray_key_bits;          /* top bits of the octant + Morton ray key the ray sort orders by (3..32) */
+  int32_t hit_key_bits;          /* top bits of the hit key the hit sort orders by (0 = all bits) */
+} swr_tuning;
+
 /* ---- context (Pathtracer ctor/dtor, Pathtracer.cpp:369-392) ------------------------------ */
 int swr_create(int device, swr_context** out);
 void swr_destroy(swr_context* ctx);
@@ -123,6 +132,11 @@ int swr_synchronize(swr_context* ctx);
 /* Device milliseconds of the last primitive entry point (sort, keys, gather, intersect,
  * occluded, raygen), measured with HIP events on the context stream around its kernels. */
 double swr_last_device_ms(const swr_context* ctx);
+int swr_get_tuning(const swr_context* ctx, swr_tuning* out);
+int swr_set_tuning(swr_context* ctx, const swr_tuning* tuning);
+/* Times candidate tunings on a reduced render of desc (same scene, same camera, about
+ * 1/16 of the samples) and keeps the fastest on the context; *out receives it. */
+int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc* desc, swr_tuning* out);
 
 /* ---- device memory helpers (plumbing for callers without their own allocator) ------------- */
 int swr_device_alloc(swr_context* ctx, size_t bytes, void** out_dev);

@@ -142,6 +142,28 @@ int swr_reset_cancel(swr_context* ctx) {
 
 double swr_last_device_ms(const swr_context* ctx) { return ctx ? ctx->last_ms : -1.0; }
 
+int swr_get_tuning(const swr_context* ctx, swr_tuning* out) {
+  if (!ctx || !out) return SWR_E_INVALID;
+  out->sort_keys_per_thread = ctx->tuning.sort_kpt;
+  out->ray_key_bits = ctx->tuning.ray_key_bits;
+  out->hit_key_bits = ctx->tuning.hit_key_bits;
+  return SWR_OK;
+}
+
+int swr_set_tuning(swr_context* ctx, const swr_tuning* t) {
+  if (!ctx || !t) return SWR_E_INVALID;
+  const int k = t->sort_keys_per_thread;
+  if (!(k == 8 || k == 12 || k == 16 || k == 20 || k == 24) || t->ray_key_bits < 3 || t->ray_key_bits > 32 ||
+      t->hit_key_bits < 0 || t->hit_key_bits > 32) {
+    ctx->err = "set_tuning: keys per thread must be 8/12/16/20/24, ray key bits 3..32, hit key bits 0..32";
+    return SWR_E_INVALID;
+  }
+  ctx->tuning.sort_kpt = k;
+  ctx->tuning.ray_key_bits = t->ray_key_bits;
+  ctx->tuning.hit_key_bits = t->hit_key_bits;
+  return SWR_OK;
+}
+
 int swr_synchronize(swr_context* ctx) {
   if (!ctx) return SWR_E_INVALID;
   return guarded(ctx, [&] { SWR_HIP(hipStreamSynchronize(ctx->stream)); });

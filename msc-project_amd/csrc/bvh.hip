@@ -252,7 +252,7 @@ __global__ void leaf_tris_kernel(const float4* __restrict__ pos, const uint4* __
   const uint32_t mask = geoms[ix.w].mask;
   out[3 * (size_t)j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(t));
   out[3 * (size_t)j + 1] = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, __uint_as_float(mask));
-  out[3 * (size_t)j + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.f);
+  out[3 * (size_t)j + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, __uint_as_float(ix.w));
 }
 
 inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }

@@ -75,6 +75,7 @@ struct FrameParams {
   uint32_t num_lights;
   float pick;           // 1 / number of lights
   float3 key_lo, key_scale;
+  uint32_t ray_key_shift;  // the ray sort orders by the top (32 - shift) key bits
   V3 env;
   int32_t env_nonzero;
   CameraRec cam;
@@ -236,72 +237,98 @@ __device__ inline void camera_ray(const FrameParams& p, uint32_t sid, V3* org, V
   *dir = normalize(focal - *org);
 }
 
+// ---- queue records ----------------------------------------------------------------------------
+// One queued ray is one 64-byte line: origin + tmin, direction + tmax, throughput + lastPdf, and
+// (sample id, depth, hit t, hit leaf). Traversal fills the last two words, so shading gathers
+// each ray with a single line read.
+struct alignas(16) RayRec {
+  float4 o;
+  float4 d;
+  float4 w;
+  uint4 id;
+};
+static_assert(sizeof(RayRec) == 64, "ray record must be one 64-byte line");
+
+// Per-sample state is stored column-major by sample: slot = (x * spp + k) * H + y, so that the
+// 64 lanes of a resolve wave (consecutive y) read consecutive slots.
+__host__ __device__ inline uint32_t sample_slot(uint32_t x, uint32_t y, uint32_t k, uint32_t H, uint32_t spp) {
+  return (x * spp + k) * H + y;
+}
+__device__ inline uint32_t slot_of_sid(const FrameParams& p, uint32_t sid) {
+  const uint32_t x = sid / (p.height * p.spp);
+  const uint32_t rem = sid - x * p.height * p.spp;
+  const uint32_t y = rem / p.spp;
+  return sample_slot(x, y, rem - y * p.spp, p.height, p.spp);
+}
+
 // ---- kernels --------------------------------------------------------------------------------
 // Camera rays for sample ids [sid0, sid0 + k) into queue slots [slot0, slot0 + k); resets each
-// sample's radiance and records its film position (Camera.cpp:23-33). In the stand-alone
-// raygen entry point (path == nullptr) ray t goes to index t and xy is indexed by t.
-__global__ void raygen_kernel(FrameParams p, uint32_t sid0, uint32_t k, uint32_t slot0, float4* __restrict__ rays,
-                              float4* __restrict__ path_w, uint2* __restrict__ path_id, uint32_t* __restrict__ keys,
-                              float2* __restrict__ sample_xy, float4* __restrict__ radiance) {
+// sample's radiance and records its film position (Camera.cpp:23-33).
+__global__ void raygen_kernel(FrameParams p, uint32_t sid0, uint32_t k, uint32_t slot0, RayRec* __restrict__ q,
+                              uint32_t* __restrict__ keys, float2* __restrict__ sample_xy,
+                              float4* __restrict__ radiance) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= k) return;
   const uint32_t sid = sid0 + t, slot = slot0 + t;
   V3 o, d;
   float sx, sy;
   camera_ray(p, sid, &o, &d, &sx, &sy);
-  rays[2 * (size_t)slot] = make_float4(o.x, o.y, o.z, 0.001f);     // RayDecompress.cpp:15
-  rays[2 * (size_t)slot + 1] = make_float4(d.x, d.y, d.z, 100000.f);  // RayDecompress.cpp:16
-  if (path_w) {
-    path_w[slot] = make_float4(1.f, 1.f, 1.f, 1.f);  // weight 1, lastPdf 1 (Camera.cpp:23-26)
-    path_id[slot] = make_uint2(sid, 0u);
-    keys[slot] = ray_key(o.x, o.y, o.z, d.x, d.y, d.z, p.key_lo, p.key_scale);
-    sample_xy[sid] = make_float2(sx, sy);
-    radiance[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
-  } else if (sample_xy) {
-    sample_xy[t] = make_float2(sx, sy);
-  }
+  RayRec r;
+  r.o = make_float4(o.x, o.y, o.z, 0.001f);    // RayDecompress.cpp:15
+  r.d = make_float4(d.x, d.y, d.z, 100000.f);  // RayDecompress.cpp:16
+  r.w = make_float4(1.f, 1.f, 1.f, 1.f);       // weight 1, lastPdf 1 (Camera.cpp:23-26)
+  r.id = make_uint4(sid, 0u, 0u, kMiss);
+  q[slot] = r;
+  keys[slot] = ray_key(o.x, o.y, o.z, d.x, d.y, d.z, p.key_lo, p.key_scale) >> p.ray_key_shift;
+  const uint32_t s = slot_of_sid(p, sid);
+  sample_xy[s] = make_float2(sx, sy);
+  radiance[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// Closest hit of every queue slot, visited in ray-key order (perm); hit and hit key are stored
-// per slot.
+// Stand-alone camera rays (swr_raygen): ray t as 8 floats, film position t.
+__global__ void raygen_rays_kernel(FrameParams p, uint32_t sid0, uint32_t k, float4* __restrict__ rays,
+                                   float2* __restrict__ sample_xy) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k) return;
+  V3 o, d;
+  float sx, sy;
+  camera_ray(p, sid0 + t, &o, &d, &sx, &sy);
+  rays[2 * (size_t)t] = make_float4(o.x, o.y, o.z, 0.001f);
+  rays[2 * (size_t)t + 1] = make_float4(d.x, d.y, d.z, 100000.f);
+  if (sample_xy) sample_xy[t] = make_float2(sx, sy);
+}
+
+// Closest hit of every queue slot, visited in ray-key order (perm). The hit (t, leaf) goes into
+// the slot's record, the hit key (global triangle id + 1, misses 0) into keys.
 template <int kStack, int kBlock>
 __global__ __launch_bounds__(kBlock) void trace_queue_kernel(const Node4* __restrict__ nodes,
                                                              const float4* __restrict__ leaf_tris,
                                                              const uint32_t* __restrict__ perm, uint32_t n,
-                                                             const float4* __restrict__ rays, uint4* __restrict__ hits,
-                                                             uint32_t* __restrict__ hit_keys) {
+                                                             RayRec* __restrict__ q,
+                                                             uint32_t* __restrict__ hit_keys, uint32_t hit_shift) {
   __shared__ uint32_t s_stack[kStack * kBlock];
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = perm[j];
-  const RayState r = make_ray(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]);
+  const RayState r = make_ray(q[i].o, q[i].d);
   const HitResult h = traverse<kBlock, false>(nodes, leaf_tris, r, SWR_MASK_ALL, s_stack + threadIdx.x);
-  hits[i] = make_uint4(__float_as_uint(h.t), __float_as_uint(h.u), __float_as_uint(h.v), h.id);
-  hit_keys[i] = h.id + 1u;
+  reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
+  hit_keys[i] = (h.id + 1u) >> hit_shift;
 }
 
-struct QueueOut {
-  float4* rays;
-  float4* path_w;
-  uint2* path_id;
-  uint32_t* keys;
-};
-
 struct ShadeOut {
-  QueueOut next;
-  float4* shadow_rays;  // 2 float4 per shadow ray
-  float4* shadow_col;   // rgb contribution (already times throughput), sample id bits in w
-  uint32_t* contrib_sid;
+  RayRec* next;          // continuation queue
+  uint32_t* next_keys;
+  float4* shadow_rays;   // 2 float4 per shadow ray
+  float4* shadow_col;    // rgb contribution (already times throughput), sample slot bits in w
+  uint32_t* contrib_slot;
   float4* contrib_rgb;
-  uint32_t* counters;   // [0] continuations, [1] shadow rays, [2] contributions
+  uint32_t* counters;    // [0] continuations, [1] shadow rays, [2] contributions
 };
 
 // Integrator::operator() for every slot, in hit-key order (Integrator.cpp:12-255).
 __global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p, const uint32_t* __restrict__ perm,
-                                                    uint32_t n, const float4* __restrict__ rays,
-                                                    const float4* __restrict__ path_w,
-                                                    const uint2* __restrict__ path_id,
-                                                    const uint4* __restrict__ hits, ShadeOut out) {
+                                                    uint32_t n, const RayRec* __restrict__ q, ShadeOut out) {
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = m < n;
   bool has_c = false, has_s = false, has_n = false;
@@ -310,16 +337,13 @@ __global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p
   float sdist = 0.f, npdf = 0.f;
   uint32_t sid = 0, depth = 0;
   if (valid) {
-    const uint32_t i = perm[m];
-    const uint4 h = hits[i];
-    const float4 o4 = rays[2 * (size_t)i], d4 = rays[2 * (size_t)i + 1];
-    const float4 pw = path_w[i];
-    const uint2 pid = path_id[i];
-    sid = pid.x;
-    depth = pid.y;
-    const V3 w = v3(pw.x, pw.y, pw.z);
-    const float last_pdf = pw.w;
-    if (h.w == kMiss) {
+    const RayRec rec = q[perm[m]];
+    sid = rec.id.x;
+    depth = rec.id.y;
+    const uint32_t leaf = rec.id.w;
+    const V3 w = v3(rec.w.x, rec.w.y, rec.w.z);
+    const float last_pdf = rec.w.w;
+    if (leaf == kMiss) {
       // Misses add nothing in the reference (Integrator.cpp:26-27); a constant environment
       // adds throughput * radiance.
       if (p.env_nonzero) {
@@ -327,10 +351,10 @@ __global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p
         contrib = v3(w.x * p.env.x, w.y * p.env.y, w.z * p.env.z);
       }
     } else {
-      const float t = __uint_as_float(h.x);
-      const uint4 tv = s.tri[h.w];
-      const GeomRec g = s.geoms[tv.w];
-      const V3 dn = normalize(v3(d4.x, d4.y, d4.z));
+      const float t = __uint_as_float(rec.id.z);
+      const float4 e1 = s.leaf_tris[3 * (size_t)leaf + 1], e2 = s.leaf_tris[3 * (size_t)leaf + 2];
+      const GeomRec g = s.geoms[__float_as_uint(e2.w)];
+      const V3 dn = normalize(v3(rec.d.x, rec.d.y, rec.d.z));
       if (g.light >= 0) {  // light hit (Integrator.cpp:37-75)
         const LightRec& L = s.lights[g.light];
         const float c = dot(v3(L.n[0], L.n[1], L.n[2]), dn * -1.f);  // QuadLight::radiance
@@ -347,10 +371,9 @@ __global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p
         }
       } else {
         const ShaderRec sh = s.shaders[g.shader];
-        const float4 a = s.pos[tv.x], b = s.pos[tv.y], c = s.pos[tv.z];
-        const V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
-        const V3 nrm = normalize(cross(v0 - v1, v2 - v0)) * -1.f;  // Integrator.cpp:121-125
-        P = v3(o4.x, o4.y, o4.z) + dn * t;                          // Integrator.cpp:126
+        // Ng = (v0 - v1) x (v2 - v0), the leaf's e1 x e2; N = -normalize(Ng) (Integrator.cpp:121-125)
+        const V3 nrm = normalize(cross(v3(e1.x, e1.y, e1.z), v3(e2.x, e2.y, e2.z))) * -1.f;
+        P = v3(rec.o.x, rec.o.y, rec.o.z) + dn * t;  // Integrator.cpp:126
         const V3 out_dir = dn * -1.f;
         const Uniform4 r = philox_uniform4(p.seed_lo, p.seed_hi, sid, p.iteration, shade_word(depth, 1u), 0u);
         // Next event estimation (Integrator.cpp:102-178)
@@ -375,15 +398,15 @@ __global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p
         // Russian roulette and continuation (Integrator.cpp:181-246)
         if ((int32_t)depth < p.max_depth) {
           const float cont = sh.type == SWR_SHADER_NULL ? 1.f : sh.reflectance;
-          float q = fminf(1.f, cont / p.threshold);
-          if ((int32_t)depth < p.min_depth) q = 1.f;
-          if (!(r.r[3] > q)) {
+          float qk = fminf(1.f, cont / p.threshold);
+          if ((int32_t)depth < p.min_depth) qk = 1.f;
+          if (!(r.r[3] > qk)) {
             const Uniform4 r2 = philox_uniform4(p.seed_lo, p.seed_hi, sid, p.iteration, shade_word(depth, 2u), 0u);
             V3 in;
             Eval e;
             if (bsdf_sample(sh, r2.r[0], r2.r[1], out_dir, nrm, &in, &e)) {
               const float k = e.cos / e.pdf;
-              nw = v3(w.x * e.f.x * k / q, w.y * e.f.y * k / q, w.z * e.f.z * k / q);
+              nw = v3(w.x * e.f.x * k / qk, w.y * e.f.y * k / qk, w.z * e.f.z * k / qk);
               if (maxc(nw) > 0.f) {
                 has_n = true;
                 ndir = in;
@@ -397,22 +420,25 @@ __global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p
   }
   const uint32_t cslot = wave_append(&out.counters[0], has_n);
   const uint32_t sslot = wave_append(&out.counters[1], has_s);
-  const uint32_t pslot = wave_append(&out.counters[2], has_c);
   if (has_n) {
-    out.next.rays[2 * (size_t)cslot] = make_float4(P.x, P.y, P.z, 0.001f);
-    out.next.rays[2 * (size_t)cslot + 1] = make_float4(ndir.x, ndir.y, ndir.z, 100000.f);
-    out.next.path_w[cslot] = make_float4(nw.x, nw.y, nw.z, npdf);
-    out.next.path_id[cslot] = make_uint2(sid, depth + 1u);
-    out.next.keys[cslot] = ray_key(P.x, P.y, P.z, ndir.x, ndir.y, ndir.z, p.key_lo, p.key_scale);
+    RayRec r;
+    r.o = make_float4(P.x, P.y, P.z, 0.001f);
+    r.d = make_float4(ndir.x, ndir.y, ndir.z, 100000.f);
+    r.w = make_float4(nw.x, nw.y, nw.z, npdf);
+    r.id = make_uint4(sid, depth + 1u, 0u, kMiss);
+    out.next[cslot] = r;
+    out.next_keys[cslot] = ray_key(P.x, P.y, P.z, ndir.x, ndir.y, ndir.z, p.key_lo, p.key_scale) >> p.ray_key_shift;
   }
+  const uint32_t sample = valid ? slot_of_sid(p, sid) : 0u;
   if (has_s) {
     out.shadow_rays[2 * (size_t)sslot] = make_float4(P.x, P.y, P.z, 0.001f);
     out.shadow_rays[2 * (size_t)sslot + 1] = make_float4(sdir.x, sdir.y, sdir.z, sdist);
-    out.shadow_col[sslot] = make_float4(srgb.x, srgb.y, srgb.z, __uint_as_float(sid));
+    out.shadow_col[sslot] = make_float4(srgb.x, srgb.y, srgb.z, __uint_as_float(sample));
   }
+  const uint32_t cs = wave_append(&out.counters[2], has_c);
   if (has_c) {
-    out.contrib_sid[pslot] = sid;
-    out.contrib_rgb[pslot] = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
+    out.contrib_slot[cs] = sample;
+    out.contrib_rgb[cs] = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
   }
 }
 
@@ -424,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void shadow_kernel(const Node4* __restrict_
                                                         const uint32_t* __restrict__ count,
                                                         const float4* __restrict__ rays,
                                                         const float4* __restrict__ col,
-                                                        uint32_t* __restrict__ contrib_sid,
+                                                        uint32_t* __restrict__ contrib_slot,
                                                         float4* __restrict__ contrib_rgb,
                                                         uint32_t* __restrict__ contrib_count) {
   __shared__ uint32_t s_stack[kStack * kBlock];
@@ -439,42 +465,43 @@ __global__ __launch_bounds__(kBlock) void shadow_kernel(const Node4* __restrict_
   }
   const uint32_t slot = wave_append(contrib_count, visible);
   if (visible) {
-    contrib_sid[slot] = __float_as_uint(c.w);
+    contrib_slot[slot] = __float_as_uint(c.w);
     contrib_rgb[slot] = make_float4(c.x, c.y, c.z, 0.f);
   }
 }
 
-// Segmented fold over contributions sorted by sample id: each segment head adds its run, in
+// Segmented fold over contributions sorted by sample slot: each segment head adds its run, in
 // order, onto the sample's radiance (samples[sampleID] += ..., Integrator.cpp:66-71, 171-173).
-__global__ void fold_kernel(uint32_t n, const uint32_t* __restrict__ sorted_sid, const uint32_t* __restrict__ perm,
+__global__ void fold_kernel(uint32_t n, const uint32_t* __restrict__ sorted_slot, const uint32_t* __restrict__ perm,
                             const float4* __restrict__ rgb, float4* __restrict__ radiance) {
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
-  const uint32_t sid = sorted_sid[m];
-  if (m > 0 && sorted_sid[m - 1] == sid) return;
-  float4 acc = radiance[sid];
-  for (uint32_t k = m; k < n && sorted_sid[k] == sid; ++k) {
+  const uint32_t s = sorted_slot[m];
+  if (m > 0 && sorted_slot[m - 1] == s) return;
+  float4 acc = radiance[s];
+  for (uint32_t k = m; k < n && sorted_slot[k] == s; ++k) {
     const float4 c = rgb[perm[k]];
     acc.x = acc.x + c.x;
     acc.y = acc.y + c.y;
     acc.z = acc.z + c.z;
   }
-  radiance[sid] = acc;
+  radiance[s] = acc;
 }
 
-// Tent / box resolve of one pixel (TentFilter.cpp:17-61, BoxFilter.cpp:17-36), written to
-// pixels[y * W + x] (the iteration's value, not accumulated).
-__global__ void filter_kernel(FrameParams p, const float4* __restrict__ radiance, const float2* __restrict__ sxy,
-                              float* __restrict__ pixels) {
-  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t py = blockIdx.y;
-  if (px >= p.width || py >= p.height) return;
+// Tent / box resolve (TentFilter.cpp:17-61, BoxFilter.cpp:17-36), one lane per pixel with the
+// lanes of a wave on consecutive y of one column, summing in the reference's order (mask x,
+// mask y, sample). Writes the iteration's value to pixels[y * W + x].
+__global__ __launch_bounds__(64) void filter_kernel(FrameParams p, const float4* __restrict__ radiance,
+                                                    const float2* __restrict__ sxy, float* __restrict__ pixels) {
+  const uint32_t py = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t px = blockIdx.y;
+  if (py >= p.height) return;
   const int W = (int)p.width, H = (int)p.height, spp = (int)p.spp;
   float out0, out1, out2;
   if (p.filter == SWR_FILTER_BOX) {
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
     for (int i = 0; i < spp; ++i) {
-      const float4 L = radiance[((size_t)px * H + py) * spp + i];
+      const float4 L = radiance[sample_slot(px, py, (uint32_t)i, H, spp)];
       s0 += L.x;
       s1 += L.y;
       s2 += L.z;
@@ -489,14 +516,14 @@ __global__ void filter_kernel(FrameParams p, const float4* __restrict__ radiance
       for (int my = -1; my < 2; ++my) {
         const int qx = (int)px + mx, qy = (int)py + my;
         if (qx < 0 || qx >= W || qy < 0 || qy >= H) continue;
-        const size_t base = ((size_t)qx * H + qy) * spp;
         for (int i = 0; i < spp; ++i) {
-          const float2 q = sxy[base + i];
+          const uint32_t s = sample_slot((uint32_t)qx, (uint32_t)qy, (uint32_t)i, H, spp);
+          const float2 q = sxy[s];
           const float dx = fmaxf(0.f, 1.f - fabsf(q.x - cx));
           const float dy = fmaxf(0.f, 1.f - fabsf(q.y - cy));
           const float w = dx * dy;
           den = den + w;
-          const float4 L = radiance[base + i];
+          const float4 L = radiance[s];
           s0 += L.x * w;
           s1 += L.y * w;
           s2 += L.z * w;
@@ -512,14 +539,15 @@ __global__ void filter_kernel(FrameParams p, const float4* __restrict__ radiance
   o[2] = out2;
 }
 
-// Sample records (x, y, r, g, b) in sample-id order (Image.h:10-14).
-__global__ void pack_samples_kernel(size_t n, const float2* __restrict__ sxy, const float4* __restrict__ radiance,
-                                    float* __restrict__ out) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float2 q = sxy[i];
-  const float4 L = radiance[i];
-  float* o = out + 5 * i;
+// Sample records (x, y, r, g, b) in reference sample-id order (Image.h:10-14).
+__global__ void pack_samples_kernel(FrameParams p, size_t n, const float2* __restrict__ sxy,
+                                    const float4* __restrict__ radiance, float* __restrict__ out) {
+  const size_t sid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sid >= n) return;
+  const uint32_t s = slot_of_sid(p, (uint32_t)sid);
+  const float2 q = sxy[s];
+  const float4 L = radiance[s];
+  float* o = out + 5 * sid;
   o[0] = q.x;
   o[1] = q.y;
   o[2] = L.x;
@@ -641,9 +669,8 @@ void with_traversal(uint32_t max_depth, F&& f) {
 }
 
 enum Ws {
-  kRaysA, kRaysB, kPathWA, kPathWB, kPathIdA, kPathIdB, kKeysA, kKeysB,
-  kHits, kHitKeys, kPerm, kSortAltK, kSortAltV,
-  kShadowRays, kShadowCol, kContribSid, kContribRgb, kContribPerm, kContribAltK, kContribAltV,
+  kQueueA, kQueueB, kKeysA, kKeysB, kHitKeys, kPerm, kSortAltK, kSortAltV,
+  kShadowRays, kShadowCol, kContribSlot, kContribRgb, kContribPerm, kContribAltK, kContribAltV,
   kRadiance, kSampleXY, kCounters, kPixels
 };
 
@@ -671,7 +698,9 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
                   float* samples_host, swr_stats* stats) {
   SWR_HIP(hipSetDevice(ctx->device));
   const DeviceScene& s = scene->dev;
-  const FrameParams p = make_params(desc, &s);
+  FrameParams p = make_params(desc, &s);
+  const int ray_bits = std::max(3, std::min(ctx->tuning.ray_key_bits, 32));
+  p.ray_key_shift = (uint32_t)(32 - ray_bits);
   const uint64_t total = (uint64_t)p.width * p.height * p.spp;
   if (total > 0xFFFFFFFFull) fail(SWR_E_INVALID, "render: more than 2^32 - 1 samples in one iteration");
   const int exp = std::max(10, std::min(desc.bin_exponent > 0 ? desc.bin_exponent : 20, 24));
@@ -679,24 +708,19 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   const uint32_t contrib_cap = 4u * cap;
   hipStream_t st = ctx->stream;
 
-  // Queues (two of each for the current and the next wave) and per-sample state.
+  // Queues (two for the current and the next wave) and per-sample state.
   auto need = [&](Ws w, size_t bytes) { ensure(ctx, ctx->render_ws[w], bytes); };
-  need(kRaysA, 32ull * cap);
-  need(kRaysB, 32ull * cap);
-  need(kPathWA, 16ull * cap);
-  need(kPathWB, 16ull * cap);
-  need(kPathIdA, 8ull * cap);
-  need(kPathIdB, 8ull * cap);
+  need(kQueueA, sizeof(RayRec) * (size_t)cap);
+  need(kQueueB, sizeof(RayRec) * (size_t)cap);
   need(kKeysA, 4ull * cap);
   need(kKeysB, 4ull * cap);
-  need(kHits, 16ull * cap);
   need(kHitKeys, 4ull * cap);
   need(kPerm, 4ull * cap);
   need(kSortAltK, 4ull * cap);
   need(kSortAltV, 4ull * cap);
   need(kShadowRays, 32ull * cap);
   need(kShadowCol, 16ull * cap);
-  need(kContribSid, 4ull * contrib_cap);
+  need(kContribSlot, 4ull * contrib_cap);
   need(kContribRgb, 16ull * contrib_cap);
   need(kContribPerm, 4ull * contrib_cap);
   need(kContribAltK, 4ull * contrib_cap);
@@ -708,14 +732,14 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   sort_reserve(ctx, contrib_cap);
 
   uint32_t* counters = ws<uint32_t>(ctx, kCounters);
-  float4* rays[2] = {ws<float4>(ctx, kRaysA), ws<float4>(ctx, kRaysB)};
-  float4* path_w[2] = {ws<float4>(ctx, kPathWA), ws<float4>(ctx, kPathWB)};
-  uint2* path_id[2] = {ws<uint2>(ctx, kPathIdA), ws<uint2>(ctx, kPathIdB)};
+  RayRec* queue[2] = {ws<RayRec>(ctx, kQueueA), ws<RayRec>(ctx, kQueueB)};
   uint32_t* keys[2] = {ws<uint32_t>(ctx, kKeysA), ws<uint32_t>(ctx, kKeysB)};
   float4* radiance = ws<float4>(ctx, kRadiance);
   float2* sxy = ws<float2>(ctx, kSampleXY);
-  const int hit_bits = bits_for((uint64_t)s.num_tris);  // keys are prim + 1 <= num_tris
-  const int sid_bits = bits_for(total - 1);
+  const int hit_bits_full = bits_for((uint64_t)s.num_tris);  // keys are prim + 1 <= num_tris
+  const int hit_bits = ctx->tuning.hit_key_bits > 0 ? std::min(ctx->tuning.hit_key_bits, hit_bits_full) : hit_bits_full;
+  const uint32_t hit_shift = (uint32_t)(hit_bits_full - hit_bits);
+  const int slot_bits = bits_for(total - 1);
 
   StageClock clk;
   swr_stats acc{};
@@ -738,8 +762,8 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   auto flush = [&]() {
     if (!pending) return;
     uint32_t *sk = nullptr, *sv = nullptr;
-    sort_pairs(ctx, ws<uint32_t>(ctx, kContribSid), ws<uint32_t>(ctx, kContribPerm), ws<uint32_t>(ctx, kContribAltK),
-               ws<uint32_t>(ctx, kContribAltV), pending, sid_bits, true, &sk, &sv, st);
+    sort_pairs(ctx, ws<uint32_t>(ctx, kContribSlot), ws<uint32_t>(ctx, kContribPerm), ws<uint32_t>(ctx, kContribAltK),
+               ws<uint32_t>(ctx, kContribAltV), pending, slot_bits, true, &sk, &sv, st);
     fold_kernel<<<blocks(pending), 256, 0, st>>>(pending, sk, sv, ws<float4>(ctx, kContribRgb), radiance);
     SWR_HIP(hipGetLastError());
     acc.contributions += pending;
@@ -754,44 +778,41 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     const uint32_t k = (uint32_t)std::min<uint64_t>(cap - n_active, total - next_sid);
     const uint32_t n = n_active + k;
     if (n == 0) break;
-    if (pending + 2ull * n > contrib_cap) {
-      flush();
-    }
+    if (pending + 2ull * n > contrib_cap) flush();
     SWR_HIP(hipEventRecord(clk.ev[0], st));
     if (k)
-      raygen_kernel<<<blocks(k), 256, 0, st>>>(p, (uint32_t)next_sid, k, n_active, rays[cur], path_w[cur],
-                                               path_id[cur], keys[cur], sxy, radiance);
+      raygen_kernel<<<blocks(k), 256, 0, st>>>(p, (uint32_t)next_sid, k, n_active, queue[cur], keys[cur], sxy,
+                                               radiance);
     SWR_HIP(hipEventRecord(clk.ev[1], st));
     uint32_t *rk = nullptr, *perm = nullptr;
     sort_pairs(ctx, keys[cur], ws<uint32_t>(ctx, kPerm), ws<uint32_t>(ctx, kSortAltK), ws<uint32_t>(ctx, kSortAltV), n,
-               32, true, &rk, &perm, st);
+               ray_bits, true, &rk, &perm, st);
     SWR_HIP(hipEventRecord(clk.ev[2], st));
     with_traversal(s.max_depth, [&](auto kStack, auto kBlock) {
       trace_queue_kernel<kStack.value, kBlock.value><<<blocks(n, kBlock.value), kBlock.value, 0, st>>>(
-          s.nodes, s.leaf_tris, perm, n, rays[cur], ws<uint4>(ctx, kHits), ws<uint32_t>(ctx, kHitKeys));
+          s.nodes, s.leaf_tris, perm, n, queue[cur], ws<uint32_t>(ctx, kHitKeys), hit_shift);
     });
     SWR_HIP(hipEventRecord(clk.ev[3], st));
-    // The ray permutation is consumed; the hit sort reuses the same buffers.
+    // The ray permutation is consumed; the hit sort reuses its buffers.
     uint32_t *hk = nullptr, *hperm = nullptr;
-    sort_pairs(ctx, ws<uint32_t>(ctx, kHitKeys), perm == ws<uint32_t>(ctx, kPerm) ? ws<uint32_t>(ctx, kSortAltV)
-                                                                                   : ws<uint32_t>(ctx, kPerm),
-               keys[cur], perm, n, hit_bits, true, &hk, &hperm, st);
+    uint32_t* spare = perm == ws<uint32_t>(ctx, kPerm) ? ws<uint32_t>(ctx, kSortAltV) : ws<uint32_t>(ctx, kPerm);
+    sort_pairs(ctx, ws<uint32_t>(ctx, kHitKeys), spare, keys[cur], perm, n, hit_bits, true, &hk, &hperm, st);
     SWR_HIP(hipEventRecord(clk.ev[4], st));
     zero_u32_kernel<<<1, 64, 0, st>>>(counters, 2);
-    ShadeOut so;
     const int nxt = cur ^ 1;
-    so.next = QueueOut{rays[nxt], path_w[nxt], path_id[nxt], keys[nxt]};
+    ShadeOut so;
+    so.next = queue[nxt];
+    so.next_keys = keys[nxt];
     so.shadow_rays = ws<float4>(ctx, kShadowRays);
     so.shadow_col = ws<float4>(ctx, kShadowCol);
-    so.contrib_sid = ws<uint32_t>(ctx, kContribSid);
+    so.contrib_slot = ws<uint32_t>(ctx, kContribSlot);
     so.contrib_rgb = ws<float4>(ctx, kContribRgb);
     so.counters = counters;
-    shade_kernel<<<blocks(n), 256, 0, st>>>(s, p, hperm, n, rays[cur], path_w[cur], path_id[cur],
-                                            ws<uint4>(ctx, kHits), so);
+    shade_kernel<<<blocks(n), 256, 0, st>>>(s, p, hperm, n, queue[cur], so);
     SWR_HIP(hipEventRecord(clk.ev[5], st));
     with_traversal(s.max_depth, [&](auto kStack, auto kBlock) {
       shadow_kernel<kStack.value, kBlock.value><<<blocks(n, kBlock.value), kBlock.value, 0, st>>>(
-          s.nodes, s.leaf_tris, counters + 1, so.shadow_rays, so.shadow_col, so.contrib_sid, so.contrib_rgb,
+          s.nodes, s.leaf_tris, counters + 1, so.shadow_rays, so.shadow_col, so.contrib_slot, so.contrib_rgb,
           counters + 2);
     });
     SWR_HIP(hipEventRecord(clk.ev[6], st));
@@ -818,7 +839,7 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   SWR_HIP(hipEventRecord(clk.ev[0], st));
   flush();
   SWR_HIP(hipEventRecord(clk.ev[1], st));
-  dim3 fgrid(blocks(p.width, 64), p.height);
+  dim3 fgrid(blocks(p.height, 64), p.width);
   filter_kernel<<<fgrid, 64, 0, st>>>(p, radiance, sxy, ws<float>(ctx, kPixels));
   SWR_HIP(hipGetLastError());
   SWR_HIP(hipEventRecord(clk.ev[2], st));
@@ -827,10 +848,9 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     SWR_HIP(hipMemcpyAsync(pixels_host, ws<float>(ctx, kPixels), 12ull * p.width * p.height, hipMemcpyDeviceToHost,
                            st));
   if (samples_host) {
-    // Reuse the contribution buffers as staging for the 5-float sample records when they fit.
     DeviceBuffer staging;
     ensure(ctx, staging, 20ull * total);
-    pack_samples_kernel<<<blocks(total), 256, 0, st>>>(total, sxy, radiance, static_cast<float*>(staging.ptr));
+    pack_samples_kernel<<<blocks(total), 256, 0, st>>>(p, total, sxy, radiance, static_cast<float*>(staging.ptr));
     SWR_HIP(hipMemcpyAsync(samples_host, staging.ptr, 20ull * total, hipMemcpyDeviceToHost, st));
     SWR_HIP(hipStreamSynchronize(st));
     release(ctx, staging);
@@ -862,9 +882,9 @@ int swr_raygen(swr_context* ctx, const swr_render_desc* desc, uint32_t sid0, siz
     if ((uint64_t)sid0 + n > total) swr::fail(SWR_E_INVALID, "raygen: sample ids beyond width*height*spp");
     if (n) {
       swr::Timer timer(ctx);
-      swr::raygen_kernel<<<swr::blocks(n), 256, 0, ctx->stream>>>(p, sid0, (uint32_t)n, 0u,
-                                                                  reinterpret_cast<float4*>(rays_dev), nullptr, nullptr,
-                                                                  nullptr, reinterpret_cast<float2*>(xy_dev), nullptr);
+      swr::raygen_rays_kernel<<<swr::blocks(n), 256, 0, ctx->stream>>>(p, sid0, (uint32_t)n,
+                                                                       reinterpret_cast<float4*>(rays_dev),
+                                                                       reinterpret_cast<float2*>(xy_dev));
       SWR_HIP(hipGetLastError());
       timer.stop();
       SWR_HIP(hipStreamSynchronize(ctx->stream));
@@ -872,6 +892,59 @@ int swr_raygen(swr_context* ctx, const swr_render_desc* desc, uint32_t sid0, siz
     }
     return SWR_OK;
   } catch (const Error& e) {
+    ctx->err = e.msg;
+    return e.code;
+  }
+}
+
+// Coordinate descent over the launch shapes on a reduced render (about 1/16 of the samples of
+// desc: base/4 for square samplers, spp/16 otherwise), best of two device timings each.
+int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc* desc, swr_tuning* out) {
+  if (!ctx || !scene || !desc || scene->ctx != ctx || !scene->committed) return SWR_E_INVALID;
+  try {
+    swr_render_desc d = *desc;
+    if (d.spp > 0 && d.sampler == SWR_SAMPLER_INDEPENDENT)
+      d.spp = std::max(1, d.spp / 16);
+    else
+      d.base = std::max(1, d.base / 4), d.spp = 0;
+    auto time_it = [&](const swr::Tuning& t) {
+      ctx->tuning = t;
+      double best = 1e30;
+      for (int rep = 0; rep < 2; ++rep) {
+        swr_stats st{};
+        swr::render_frame(ctx, scene, d, nullptr, nullptr, &st);
+        best = std::min(best, st.ms_total);
+      }
+      return best;
+    };
+    swr::Tuning best = ctx->tuning;
+    swr::render_frame(ctx, scene, d, nullptr, nullptr, nullptr);  // allocations outside the timings
+    double best_ms = time_it(best);
+    for (int kpt : {8, 12, 16, 20, 24}) {
+      swr::Tuning t = best;
+      t.sort_kpt = kpt;
+      const double ms = time_it(t);
+      if (ms < best_ms) best_ms = ms, best = t;
+    }
+    for (int rb : {32, 24, 20, 16}) {
+      swr::Tuning t = best;
+      t.ray_key_bits = rb;
+      const double ms = time_it(t);
+      if (ms < best_ms) best_ms = ms, best = t;
+    }
+    const int full = swr::bits_for((uint64_t)scene->dev.num_tris);
+    for (int hb : {0, full - 6, full - 12}) {
+      if (hb < 0 || (hb > 0 && hb < 4)) continue;
+      swr::Tuning t = best;
+      t.hit_key_bits = hb;
+      const double ms = time_it(t);
+      if (ms < best_ms) best_ms = ms, best = t;
+    }
+    ctx->tuning = best;
+    if (out) swr_get_tuning(ctx, out);
+    return SWR_OK;
+  } catch (const Error& e) {
+    (void)hipStreamSynchronize(ctx->stream);
     ctx->err = e.msg;
     return e.code;
   }

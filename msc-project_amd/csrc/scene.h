@@ -7,7 +7,8 @@
 //     (i0, i1, i2, geomID) per triangle, optional uv float2 per vertex;
 //   records: GeomRec per geomID, ShaderRec per shader, LightRec per quad light;
 //   BVH4 nodes (128 B each) and the leaf triangles in BVH order (48 B each: v0 with the global
-//     triangle id in w, e1 = v0 - v1 with the geometry mask in w, e2 = v2 - v0).
+//     triangle id in w, e1 = v0 - v1 with the geometry mask in w, e2 = v2 - v0 with the geomID
+//     in w), so that shading gets Ng = e1 x e2 and the geometry from one 48-byte read.
 #pragma once
 
 #include <hip/hip_runtime.h>

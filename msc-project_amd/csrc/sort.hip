@@ -23,8 +23,6 @@ namespace {
 
 constexpr int kRadix = 256;
 constexpr int kWaves = kSortBlock / 64;
-constexpr int kKpt = kSortKeysPerThread;
-constexpr int kTile = kSortTile;
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPre = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1;
@@ -36,20 +34,45 @@ __global__ void sort_prepare_kernel(uint32_t* hist, uint32_t* counters) {
   if (t < 4) counters[t] = 0;
 }
 
-// Histograms of all digit positions in one pass over the keys.
+// Lanes holding the same 8-bit digit (a 64-lane match-any from 8 ballots), among active lanes.
+__device__ inline uint64_t match_digit(uint32_t d, uint64_t active) {
+  uint64_t peers = active;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  return peers;
+}
+
+// Histograms of all digit positions in one pass over the keys. Each wave owns a private copy
+// in LDS; per key and digit position only the lowest lane of each group of equal digits adds
+// the group's size, so skewed digits (e.g. a few direction octants) do not serialise on
+// atomics.
 __global__ __launch_bounds__(256) void sort_histogram_kernel(const uint32_t* __restrict__ keys, size_t n, int passes,
                                                              uint32_t* __restrict__ hist) {
-  __shared__ uint32_t s_hist[4 * kRadix];
-  for (int i = threadIdx.x; i < 4 * kRadix; i += blockDim.x) s_hist[i] = 0;
+  __shared__ uint32_t s_hist[kWaves][4 * kRadix];
+  for (int i = threadIdx.x; i < kWaves * 4 * kRadix; i += blockDim.x) (&s_hist[0][0])[i] = 0;
   __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint32_t k = keys[i];
-    for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p * kRadix + ((k >> (8 * p)) & 255u)], 1u);
+  for (size_t base = (size_t)blockIdx.x * blockDim.x + (size_t)w * 64; base < n; base += stride) {
+    const size_t i = base + lane;
+    const bool ok = i < n;
+    const uint32_t k = ok ? keys[i] : 0u;
+    const uint64_t active = __ballot(ok);
+    for (int p = 0; p < passes; ++p) {
+      const uint32_t d = (k >> (8 * p)) & 255u;
+      const uint64_t peers = match_digit(d, active);
+      if (ok && lane == __ffsll((unsigned long long)peers) - 1) s_hist[w][p * kRadix + d] += (uint32_t)__popcll(peers);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x) {
-    const uint32_t v = s_hist[i];
+    uint32_t v = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) v += s_hist[ww][i];
     if (v) atomicAdd(&hist[i], v);
   }
 }
@@ -73,7 +96,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp) {
   return pre + x - v;
 }
 
-template <bool kIota>
+template <bool kIota, int kKpt>
 __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __restrict__ keys_in,
                                                               const uint32_t* __restrict__ vals_in,
                                                               uint32_t* __restrict__ keys_out,
@@ -81,6 +104,7 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
                                                               const uint32_t* __restrict__ hist,
                                                               uint32_t* __restrict__ counter,
                                                               uint64_t* __restrict__ status, uint32_t epoch) {
+  constexpr int kTile = kSortBlock * kKpt;
   __shared__ uint32_t s_buf[kTile];
   __shared__ uint32_t s_wave[kWaves][kRadix];
   __shared__ uint32_t s_start[kRadix];
@@ -215,7 +239,8 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
 }  // namespace
 
 size_t sort_status_bytes(size_t n) {
-  const size_t tiles = (n + kTile - 1) / kTile;
+  const size_t tile = (size_t)kSortBlock * kSortMinKeysPerThread;  // the smallest tile sizes the buffer
+  const size_t tiles = (n + tile - 1) / tile;
   return (tiles ? tiles : 1) * kRadix * sizeof(uint64_t);
 }
 
@@ -248,19 +273,31 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
   sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters);
   const int hist_blocks = (int)std::min<size_t>((n + 4095) / 4096, (size_t)std::max(ctx->num_cus, 1) * 8);
   sort_histogram_kernel<<<hist_blocks, 256, 0, stream>>>(keys, n, passes, hist);
-  const uint32_t tiles = (uint32_t)((n + kTile - 1) / kTile);
+  const int kpt = ctx->tuning.sort_kpt;
+  const uint32_t tiles = (uint32_t)((n + (size_t)kSortBlock * kpt - 1) / ((size_t)kSortBlock * kpt));
   uint32_t *sk = keys, *sv = vals, *dk = alt_keys, *dv = alt_vals;
   for (int p = 0; p < passes; ++p) {
     if (++ws.epoch == 0) {  // tag space exhausted: start over from a cleared buffer
       SWR_HIP(hipMemsetAsync(ws.status.ptr, 0, ws.status.bytes, stream));
       ws.epoch = 1;
     }
-    if (iota && p == 0)
-      onesweep_kernel<true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, 8 * p, hist + p * kRadix,
-                                                             counters + p, status, ws.epoch);
-    else
-      onesweep_kernel<false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, 8 * p, hist + p * kRadix,
-                                                              counters + p, status, ws.epoch);
+    const bool first_iota = iota && p == 0;
+#define SWR_ONESWEEP(KPT)                                                                                     \
+  if (first_iota)                                                                                             \
+    onesweep_kernel<true, KPT><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, 8 * p,          \
+                                                                 hist + p * kRadix, counters + p, status, ws.epoch); \
+  else                                                                                                        \
+    onesweep_kernel<false, KPT><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, 8 * p,         \
+                                                                  hist + p * kRadix, counters + p, status, ws.epoch);
+    switch (kpt) {
+      case 8: SWR_ONESWEEP(8) break;
+      case 12: SWR_ONESWEEP(12) break;
+      case 16: SWR_ONESWEEP(16) break;
+      case 20: SWR_ONESWEEP(20) break;
+      case 24: SWR_ONESWEEP(24) break;
+      default: fail(SWR_E_INVALID, "sort: keys per thread must be 8, 12, 16, 20 or 24");
+    }
+#undef SWR_ONESWEEP
     std::swap(sk, dk);
     std::swap(sv, dv);
   }

@@ -50,6 +50,15 @@ struct SortWorkspace {
 
 struct swr_scene;
 
+namespace swr {
+// Launch shapes chosen per context (by swr_autotune or swr_set_tuning).
+struct Tuning {
+  int sort_kpt = 16;       // keys per thread of a onesweep tile (tile = 256 * kpt)
+  int ray_key_bits = 32;   // top bits of the octant + Morton ray key that the ray sort orders by
+  int hit_key_bits = 0;    // top bits of the hit key sorted (0 = all: full (geomID, primID) order)
+};
+}  // namespace swr
+
 struct swr_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -59,6 +68,7 @@ struct swr_context {
   uint64_t in_use = 0;
   std::unordered_map<void*, size_t> allocs;
   swr::SortWorkspace sort;
+  swr::Tuning tuning;
   swr::DeviceBuffer render_ws[24];           // wavefront queues of the render loop (render.hip)
   std::unordered_set<swr_scene*> scenes;     // live scenes; destroying the context detaches them
   int num_cus = 0;
@@ -102,8 +112,7 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
 size_t sort_status_bytes(size_t n);
 void sort_reserve(swr_context* ctx, size_t n);
 constexpr int kSortBlock = 256;
-constexpr int kSortKeysPerThread = 16;
-constexpr int kSortTile = kSortBlock * kSortKeysPerThread;
+constexpr int kSortMinKeysPerThread = 8;
 
 // ---- keys.hip ----
 void launch_ray_keys(const float4* rays, size_t n, float3 lo, float3 scale, uint32_t* keys, hipStream_t s);

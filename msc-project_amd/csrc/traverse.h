@@ -85,7 +85,8 @@ __device__ inline void cswap(uint32_t& a, uint32_t& b) {
 
 struct HitResult {
   float t, u, v;
-  uint32_t id;  // global triangle id or kMiss
+  uint32_t id;    // global triangle id or kMiss
+  uint32_t leaf;  // position of that triangle in the BVH-order leaf array
 };
 
 // Traverses one ray. kAny: returns on the first hit (id of that triangle), else the closest.
@@ -97,6 +98,7 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
   best.t = r.tmax;
   best.u = best.v = __int_as_float(0x7FC00000);
   best.id = kMiss;
+  best.leaf = kMiss;
   const float inf = __int_as_float(0x7F800000);
   int sp = 0;
   uint32_t node = 0;  // root
@@ -154,6 +156,7 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
             best.u = u;
             best.v = v;
             best.id = id;
+            best.leaf = first + k;
             return best;
           }
           if (best.id == kMiss || t < best.t || (t == best.t && id < best.id)) {
@@ -161,6 +164,7 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
             best.u = u;
             best.v = v;
             best.id = id;
+            best.leaf = first + k;
           }
         }
       }

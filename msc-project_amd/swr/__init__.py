@@ -59,6 +59,13 @@ class QuadLightDesc(C.Structure):
                 ("intensity", C.c_float)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("sort_keys_per_thread", C.c_int32), ("ray_key_bits", C.c_int32), ("hit_key_bits", C.c_int32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [("camera_rays", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("contributions", C.c_uint64), ("waves", C.c_uint32), ("max_wave", C.c_uint32),
@@ -82,6 +89,9 @@ _SIG = {
     "swr_reset_cancel": (C.c_int, [_P]),
     "swr_synchronize": (C.c_int, [_P]),
     "swr_last_device_ms": (C.c_double, [_P]),
+    "swr_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "swr_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "swr_autotune": (C.c_int, [_P, _P, C.POINTER(RenderDesc), C.POINTER(Tuning)]),
     "swr_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "swr_device_free": (C.c_int, [_P, _P]),
     "swr_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
@@ -202,6 +212,20 @@ class Context:
 
     def last_device_ms(self) -> float:
         return self.lib.swr_last_device_ms(self.h)
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self.check(self.lib.swr_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, sort_keys_per_thread=16, ray_key_bits=32, hit_key_bits=0):
+        t = Tuning(sort_keys_per_thread, ray_key_bits, hit_key_bits)
+        self.check(self.lib.swr_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, scene: "Scene", desc: "RenderDesc") -> dict:
+        t = Tuning()
+        self.check(self.lib.swr_autotune(self.h, scene.h, C.byref(desc), C.byref(t)))
+        return t.as_dict()
 
     def synchronize(self):
         self.check(self.lib.swr_synchronize(self.h))

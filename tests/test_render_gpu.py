@@ -137,3 +137,24 @@ def test_budget_refuses_large_render(ctx):
     sc.render(rd)
     sc.close()
     c2.close()
+
+
+def test_tuning_changes_speed_not_pixels(ctx):
+    """Sort tile sizes and key widths only reorder rays and hits: every tuning gives the same
+    image, bit for bit."""
+    desc = scenes.box_room(width=40, height=40, base=2, sphere_level=3)
+    sc = swr.Scene(ctx, desc)
+    rd, _ = _pair(desc)
+    before = ctx.tuning()
+    ref = sc.render(rd)[0]
+    try:
+        for kpt, rb, hb in [(8, 32, 0), (12, 24, 0), (20, 16, 8), (24, 3, 4)]:
+            ctx.set_tuning(kpt, rb, hb)
+            np.testing.assert_array_equal(sc.render(rd)[0], ref)
+        with pytest.raises(swr.SwrError):
+            ctx.set_tuning(10, 32, 0)
+        t = ctx.autotune(sc, rd)
+        assert t["sort_keys_per_thread"] in (8, 12, 16, 20, 24)
+        np.testing.assert_array_equal(sc.render(rd)[0], ref)
+    finally:
+        ctx.set_tuning(**before)

@@ -118,3 +118,17 @@ def test_gather(ctx, words):
     ds, dp, do = ctx.upload(src), ctx.upload(perm), ctx.alloc(src.nbytes)
     ctx.gather(ds, dp, n, words, do)
     assert np.array_equal(do.download(np.uint32, (n, words)), src[perm])
+
+
+@pytest.mark.parametrize("kpt", [8, 12, 20, 24])
+def test_sort_every_tile_size(ctx, kpt):
+    before = ctx.tuning()
+    try:
+        ctx.set_tuning(kpt, 32, 0)
+        rng = np.random.default_rng(kpt)
+        for n in (1, 5000, 300001):
+            keys = rng.integers(0, 1 << 12, n).astype(np.uint32)
+            k, v = _sort(ctx, keys, 12)
+            assert np.array_equal(v, orc.stable_sort_perm(keys))
+    finally:
+        ctx.set_tuning(**before)
