@@ -295,9 +295,13 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
     ensure(ctx, perm, nn * 4);
     ensure(ctx, alt_k, nn * 4);
     ensure(ctx, alt_v, nn * 4);
+    // One scale for all three axes (the largest centroid extent): a thin axis then contributes
+    // few Morton bits, so the hierarchy does not split flat geometry (a height field) along its
+    // thin direction.
     float3 flo = make_float3(lo[0], lo[1], lo[2]);
-    auto inv = [](float a, float b) { return b > a ? 1.f / (b - a) : 0.f; };
-    float3 inv_ext = make_float3(inv(lo[0], hi[0]), inv(lo[1], hi[1]), inv(lo[2], hi[2]));
+    const float ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    const float inv = ext > 0.f ? 1.f / ext : 0.f;
+    float3 inv_ext = make_float3(inv, inv, inv);
     morton_kernel<<<blocks(n), 256, 0, s>>>(pos, tri, n, flo, inv_ext, static_cast<uint32_t*>(codes.ptr));
     uint32_t *sk = nullptr, *sv = nullptr;
     sort_pairs(ctx, static_cast<uint32_t*>(codes.ptr), static_cast<uint32_t*>(perm.ptr),

@@ -7,17 +7,17 @@
 // queue of 2^bin_exponent rays replaces the bins and batches:
 //   raygen      tops the queue up with camera rays for the next sample ids (Camera.cpp:6-59);
 //   ray sort    octant + Morton keys (written by raygen and shade) -> stable radix sort;
-//   trace       closest hit in ray-key order -> hit record + hit key per queue slot;
+//   trace       closest hit in ray-key order -> (t, leaf) into the ray's record + hit key;
 //   hit sort    stable radix sort by (geomID, primID) (CompareHit, RayUncompressed.h:54-60);
 //   shade       MIS shading in hit order (Integrator.cpp:12-255): emission and environment
-//               contributions, one shadow ray per NEE sample, one continuation per survivor,
-//               appended with one atomic per wave;
-//   shadow      any-hit traversal of the shadow queue; unoccluded ones become contributions;
-// and when the contribution buffer fills (and at the end) a stable radix sort by sample id and
-// a segmented fold adds each sample's contributions into its radiance in bounce order, the
-// order of the reference's samples[sampleID] += (Integrator.cpp:66-71, 171-173). No float
-// atomics anywhere: the image is a pure function of the inputs. The tent/box resolve gathers
-// each pixel's 3x3 neighbourhood in the reference's loop order (TentFilter.cpp:17-61).
+//               radiance, one shadow ray per NEE sample at the same hit-order position, one
+//               continuation per survivor appended with one atomic per 1024-lane workgroup;
+//   shadow      any-hit traversal; unoccluded rays add their contribution.
+// A path has one live ray, so within a launch no two lanes touch the same sample: radiance is
+// added to the sample in place, in bounce order, as the reference's samples[sampleID] +=
+// (Integrator.cpp:66-71, 171-173) -- no float atomics, and the image is a pure function of the
+// inputs whatever the wave composition. The tent/box resolve gathers each pixel's 3x3
+// neighbourhood in the reference's loop order (TentFilter.cpp:17-61).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -317,18 +317,24 @@ __global__ __launch_bounds__(kBlock) void trace_queue_kernel(const Node4* __rest
 }
 
 struct ShadeOut {
-  RayRec* next;          // continuation queue
+  RayRec* next;          // continuation queue (compacted)
   uint32_t* next_keys;
-  float4* shadow_rays;   // 2 float4 per shadow ray
+  float4* shadow_rays;   // 2 float4 per hit-order position m; tmax < 0 marks "no shadow ray"
   float4* shadow_col;    // rgb contribution (already times throughput), sample slot bits in w
-  uint32_t* contrib_slot;
-  float4* contrib_rgb;
-  uint32_t* counters;    // [0] continuations, [1] shadow rays, [2] contributions
+  float4* radiance;      // per-sample radiance, accumulated in place
+  uint64_t* counters;    // low word: continuations, high word: shadow rays
 };
 
+constexpr int kShadeBlock = 1024;
+
 // Integrator::operator() for every slot, in hit-key order (Integrator.cpp:12-255).
-__global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p, const uint32_t* __restrict__ perm,
-                                                    uint32_t n, const RayRec* __restrict__ q, ShadeOut out) {
+// Radiance is added in place: a path has one live ray, so no two lanes of a launch touch the
+// same sample and each sample receives its contributions in bounce order, exactly as the
+// reference's samples[sampleID] += (Integrator.cpp:66-71, 171-173), without atomics.
+__global__ __launch_bounds__(kShadeBlock) void shade_kernel(DeviceScene s, FrameParams p,
+                                                            const uint32_t* __restrict__ perm, uint32_t n,
+                                                            const RayRec* __restrict__ q, ShadeOut out) {
+  __shared__ uint32_t s_append[2 * (kShadeBlock / 64) + 1];
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = m < n;
   bool has_c = false, has_s = false, has_n = false;
@@ -418,8 +424,7 @@ __global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p
       }
     }
   }
-  const uint32_t cslot = wave_append(&out.counters[0], has_n);
-  const uint32_t sslot = wave_append(&out.counters[1], has_s);
+  const uint32_t cslot = block_append<kShadeBlock>(out.counters, has_n, has_s, s_append);
   if (has_n) {
     RayRec r;
     r.o = make_float4(P.x, P.y, P.z, 0.001f);
@@ -429,62 +434,44 @@ __global__ __launch_bounds__(256) void shade_kernel(DeviceScene s, FrameParams p
     out.next[cslot] = r;
     out.next_keys[cslot] = ray_key(P.x, P.y, P.z, ndir.x, ndir.y, ndir.z, p.key_lo, p.key_scale) >> p.ray_key_shift;
   }
-  const uint32_t sample = valid ? slot_of_sid(p, sid) : 0u;
-  if (has_s) {
-    out.shadow_rays[2 * (size_t)sslot] = make_float4(P.x, P.y, P.z, 0.001f);
-    out.shadow_rays[2 * (size_t)sslot + 1] = make_float4(sdir.x, sdir.y, sdir.z, sdist);
-    out.shadow_col[sslot] = make_float4(srgb.x, srgb.y, srgb.z, __uint_as_float(sample));
-  }
-  const uint32_t cs = wave_append(&out.counters[2], has_c);
+  if (!valid) return;
+  const uint32_t sample = slot_of_sid(p, sid);
+  // The shadow ray stays at hit-order position m (no compaction); tmax < 0 marks none.
+  out.shadow_rays[2 * (size_t)m] = make_float4(P.x, P.y, P.z, 0.001f);
+  out.shadow_rays[2 * (size_t)m + 1] = make_float4(sdir.x, sdir.y, sdir.z, has_s ? sdist : -1.f);
+  out.shadow_col[m] = make_float4(srgb.x, srgb.y, srgb.z, __uint_as_float(sample));
   if (has_c) {
-    out.contrib_slot[cs] = sample;
-    out.contrib_rgb[cs] = make_float4(contrib.x, contrib.y, contrib.z, 0.f);
+    float4 acc = out.radiance[sample];
+    acc.x = acc.x + contrib.x;
+    acc.y = acc.y + contrib.y;
+    acc.z = acc.z + contrib.z;
+    out.radiance[sample] = acc;
   }
 }
 
-// Shadow rays (mask 0x0FFFFFFF, Integrator.cpp:153-174); unoccluded ones append their
-// contribution. The queue length is read on the device.
+// Shadow rays (mask 0x0FFFFFFF, Integrator.cpp:153-174), one per hit-order position; positions
+// without a shadow ray carry tmax < 0. An unoccluded ray adds its contribution to its sample in
+// place (one live path per sample, so no two lanes share a sample).
 template <int kStack, int kBlock>
 __global__ __launch_bounds__(kBlock) void shadow_kernel(const Node4* __restrict__ nodes,
-                                                        const float4* __restrict__ leaf_tris,
-                                                        const uint32_t* __restrict__ count,
+                                                        const float4* __restrict__ leaf_tris, uint32_t n,
                                                         const float4* __restrict__ rays,
                                                         const float4* __restrict__ col,
-                                                        uint32_t* __restrict__ contrib_slot,
-                                                        float4* __restrict__ contrib_rgb,
-                                                        uint32_t* __restrict__ contrib_count) {
+                                                        float4* __restrict__ radiance) {
   __shared__ uint32_t s_stack[kStack * kBlock];
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  bool visible = false;
-  float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (i < *count) {
-    const RayState r = make_ray(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]);
-    const HitResult h = traverse<kBlock, true>(nodes, leaf_tris, r, SWR_MASK_SHADOW, s_stack + threadIdx.x);
-    visible = h.id == kMiss;
-    if (visible) c = col[i];
-  }
-  const uint32_t slot = wave_append(contrib_count, visible);
-  if (visible) {
-    contrib_slot[slot] = __float_as_uint(c.w);
-    contrib_rgb[slot] = make_float4(c.x, c.y, c.z, 0.f);
-  }
-}
-
-// Segmented fold over contributions sorted by sample slot: each segment head adds its run, in
-// order, onto the sample's radiance (samples[sampleID] += ..., Integrator.cpp:66-71, 171-173).
-__global__ void fold_kernel(uint32_t n, const uint32_t* __restrict__ sorted_slot, const uint32_t* __restrict__ perm,
-                            const float4* __restrict__ rgb, float4* __restrict__ radiance) {
-  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  const uint32_t s = sorted_slot[m];
-  if (m > 0 && sorted_slot[m - 1] == s) return;
+  if (i >= n) return;
+  const float4 d = rays[2 * (size_t)i + 1];
+  if (!(d.w >= 0.f)) return;
+  const RayState r = make_ray(rays[2 * (size_t)i], d);
+  const HitResult h = traverse<kBlock, true>(nodes, leaf_tris, r, SWR_MASK_SHADOW, s_stack + threadIdx.x);
+  if (h.id != kMiss) return;
+  const float4 c = col[i];
+  const uint32_t s = __float_as_uint(c.w);
   float4 acc = radiance[s];
-  for (uint32_t k = m; k < n && sorted_slot[k] == s; ++k) {
-    const float4 c = rgb[perm[k]];
-    acc.x = acc.x + c.x;
-    acc.y = acc.y + c.y;
-    acc.z = acc.z + c.z;
-  }
+  acc.x = acc.x + c.x;
+  acc.y = acc.y + c.y;
+  acc.z = acc.z + c.z;
   radiance[s] = acc;
 }
 
@@ -670,7 +657,7 @@ void with_traversal(uint32_t max_depth, F&& f) {
 
 enum Ws {
   kQueueA, kQueueB, kKeysA, kKeysB, kHitKeys, kPerm, kSortAltK, kSortAltV,
-  kShadowRays, kShadowCol, kContribSlot, kContribRgb, kContribPerm, kContribAltK, kContribAltV,
+  kShadowRays, kShadowCol,
   kRadiance, kSampleXY, kCounters, kPixels
 };
 
@@ -705,7 +692,6 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   if (total > 0xFFFFFFFFull) fail(SWR_E_INVALID, "render: more than 2^32 - 1 samples in one iteration");
   const int exp = std::max(10, std::min(desc.bin_exponent > 0 ? desc.bin_exponent : 20, 24));
   const uint32_t cap = (uint32_t)std::min<uint64_t>(uint64_t(1) << exp, std::max<uint64_t>(total, 64));
-  const uint32_t contrib_cap = 4u * cap;
   hipStream_t st = ctx->stream;
 
   // Queues (two for the current and the next wave) and per-sample state.
@@ -720,16 +706,11 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   need(kSortAltV, 4ull * cap);
   need(kShadowRays, 32ull * cap);
   need(kShadowCol, 16ull * cap);
-  need(kContribSlot, 4ull * contrib_cap);
-  need(kContribRgb, 16ull * contrib_cap);
-  need(kContribPerm, 4ull * contrib_cap);
-  need(kContribAltK, 4ull * contrib_cap);
-  need(kContribAltV, 4ull * contrib_cap);
   need(kRadiance, 16ull * total);
   need(kSampleXY, 8ull * total);
   need(kCounters, 64);
   need(kPixels, 12ull * p.width * p.height);
-  sort_reserve(ctx, contrib_cap);
+  sort_reserve(ctx, cap);
 
   uint32_t* counters = ws<uint32_t>(ctx, kCounters);
   RayRec* queue[2] = {ws<RayRec>(ctx, kQueueA), ws<RayRec>(ctx, kQueueB)};
@@ -739,7 +720,6 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   const int hit_bits_full = bits_for((uint64_t)s.num_tris);  // keys are prim + 1 <= num_tris
   const int hit_bits = ctx->tuning.hit_key_bits > 0 ? std::min(ctx->tuning.hit_key_bits, hit_bits_full) : hit_bits_full;
   const uint32_t hit_shift = (uint32_t)(hit_bits_full - hit_bits);
-  const int slot_bits = bits_for(total - 1);
 
   StageClock clk;
   swr_stats acc{};
@@ -754,23 +734,7 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     }
   } evg{ev_start, ev_end};
 
-  zero_u32_kernel<<<1, 64, 0, st>>>(counters, 16);
   SWR_HIP(hipEventRecord(ev_start, st));
-
-  // Folds the pending contributions into the per-sample radiance.
-  uint32_t pending = 0;
-  auto flush = [&]() {
-    if (!pending) return;
-    uint32_t *sk = nullptr, *sv = nullptr;
-    sort_pairs(ctx, ws<uint32_t>(ctx, kContribSlot), ws<uint32_t>(ctx, kContribPerm), ws<uint32_t>(ctx, kContribAltK),
-               ws<uint32_t>(ctx, kContribAltV), pending, slot_bits, true, &sk, &sv, st);
-    fold_kernel<<<blocks(pending), 256, 0, st>>>(pending, sk, sv, ws<float4>(ctx, kContribRgb), radiance);
-    SWR_HIP(hipGetLastError());
-    acc.contributions += pending;
-    pending = 0;
-    zero_u32_kernel<<<1, 64, 0, st>>>(counters + 2, 1);
-  };
-
   uint32_t cur = 0, n_active = 0;
   uint64_t next_sid = 0;
   while (true) {
@@ -778,7 +742,6 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     const uint32_t k = (uint32_t)std::min<uint64_t>(cap - n_active, total - next_sid);
     const uint32_t n = n_active + k;
     if (n == 0) break;
-    if (pending + 2ull * n > contrib_cap) flush();
     SWR_HIP(hipEventRecord(clk.ev[0], st));
     if (k)
       raygen_kernel<<<blocks(k), 256, 0, st>>>(p, (uint32_t)next_sid, k, n_active, queue[cur], keys[cur], sxy,
@@ -805,15 +768,13 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     so.next_keys = keys[nxt];
     so.shadow_rays = ws<float4>(ctx, kShadowRays);
     so.shadow_col = ws<float4>(ctx, kShadowCol);
-    so.contrib_slot = ws<uint32_t>(ctx, kContribSlot);
-    so.contrib_rgb = ws<float4>(ctx, kContribRgb);
-    so.counters = counters;
-    shade_kernel<<<blocks(n), 256, 0, st>>>(s, p, hperm, n, queue[cur], so);
+    so.radiance = radiance;
+    so.counters = reinterpret_cast<uint64_t*>(counters);
+    shade_kernel<<<blocks(n, kShadeBlock), kShadeBlock, 0, st>>>(s, p, hperm, n, queue[cur], so);
     SWR_HIP(hipEventRecord(clk.ev[5], st));
     with_traversal(s.max_depth, [&](auto kStack, auto kBlock) {
       shadow_kernel<kStack.value, kBlock.value><<<blocks(n, kBlock.value), kBlock.value, 0, st>>>(
-          s.nodes, s.leaf_tris, counters + 1, so.shadow_rays, so.shadow_col, so.contrib_slot, so.contrib_rgb,
-          counters + 2);
+          s.nodes, s.leaf_tris, n, so.shadow_rays, so.shadow_col, radiance);
     });
     SWR_HIP(hipEventRecord(clk.ev[6], st));
     SWR_HIP(hipGetLastError());
@@ -831,13 +792,10 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     acc.shadow_rays += host_counts[1];
     acc.waves += 1;
     acc.max_wave = std::max(acc.max_wave, n);
-    pending = host_counts[2];
     n_active = host_counts[0];
     next_sid += k;
     cur ^= 1;
   }
-  SWR_HIP(hipEventRecord(clk.ev[0], st));
-  flush();
   SWR_HIP(hipEventRecord(clk.ev[1], st));
   dim3 fgrid(blocks(p.height, 64), p.width);
   filter_kernel<<<fgrid, 64, 0, st>>>(p, radiance, sxy, ws<float>(ctx, kPixels));
@@ -856,7 +814,6 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     release(ctx, staging);
   }
   SWR_HIP(hipStreamSynchronize(st));
-  acc.ms_accumulate += clk.ms(0, 1);
   acc.ms_filter = clk.ms(1, 2);
   float total_ms = 0.f;
   SWR_HIP(hipEventElapsedTime(&total_ms, ev_start, ev_end));
