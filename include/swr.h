@@ -114,6 +114,8 @@ typedef struct {
   int32_t sort_keys_per_thread;  /* onesweep tile = 256 * this: 8, 12, 16, 20 or 24 */
   int32_t ray_key_bits;          /* top bits of the octant + Morton ray key the ray sort orders by (3..32) */
   int32_t hit_key_bits;          /* top bits of the hit key the hit sort orders by (0 = all bits) */
+  int32_t trace_stack_entries;   /* LDS ring of the traversal stack per ray: 8, 16, 32 or 64 entries, deeper
+                                    entries spill to device memory (0 = sized from the BVH depth) */
 } swr_tuning;
 
 /* ---- context (Pathtracer ctor/dtor, Pathtracer.cpp:369-392) ------------------------------ */

@@ -300,18 +300,20 @@ __global__ void raygen_rays_kernel(FrameParams p, uint32_t sid0, uint32_t k, flo
 
 // Closest hit of every queue slot, visited in ray-key order (perm). The hit (t, leaf) goes into
 // the slot's record, the hit key (global triangle id + 1, misses 0) into keys.
-template <int kStack, int kBlock>
-__global__ __launch_bounds__(kBlock) void trace_queue_kernel(const Node4* __restrict__ nodes,
-                                                             const float4* __restrict__ leaf_tris,
-                                                             const uint32_t* __restrict__ perm, uint32_t n,
-                                                             RayRec* __restrict__ q,
-                                                             uint32_t* __restrict__ hit_keys, uint32_t hit_shift) {
-  __shared__ uint32_t s_stack[kStack * kBlock];
-  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+template <int kShort>
+__global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const Node4* __restrict__ nodes,
+                                                                  const float4* __restrict__ leaf_tris,
+                                                                  const uint32_t* __restrict__ perm, uint32_t n,
+                                                                  RayRec* __restrict__ q,
+                                                                  uint32_t* __restrict__ hit_keys, uint32_t hit_shift,
+                                                                  uint32_t* __restrict__ spill, uint32_t spill_stride) {
+  __shared__ uint32_t s_ring[kShort * kTraceBlock];
+  const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = perm[j];
   const RayState r = make_ray(q[i].o, q[i].d);
-  const HitResult h = traverse<kBlock, false>(nodes, leaf_tris, r, SWR_MASK_ALL, s_stack + threadIdx.x);
+  const HitResult h = traverse<kTraceBlock, kShort, false>(nodes, leaf_tris, r, SWR_MASK_ALL, s_ring + threadIdx.x,
+                                                           spill + (size_t)j * spill_stride);
   reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
   hit_keys[i] = (h.id + 1u) >> hit_shift;
 }
@@ -452,19 +454,21 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(DeviceScene s, Frame
 // Shadow rays (mask 0x0FFFFFFF, Integrator.cpp:153-174), one per hit-order position; positions
 // without a shadow ray carry tmax < 0. An unoccluded ray adds its contribution to its sample in
 // place (one live path per sample, so no two lanes share a sample).
-template <int kStack, int kBlock>
-__global__ __launch_bounds__(kBlock) void shadow_kernel(const Node4* __restrict__ nodes,
-                                                        const float4* __restrict__ leaf_tris, uint32_t n,
-                                                        const float4* __restrict__ rays,
-                                                        const float4* __restrict__ col,
-                                                        float4* __restrict__ radiance) {
-  __shared__ uint32_t s_stack[kStack * kBlock];
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+template <int kShort>
+__global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const Node4* __restrict__ nodes,
+                                                             const float4* __restrict__ leaf_tris, uint32_t n,
+                                                             const float4* __restrict__ rays,
+                                                             const float4* __restrict__ col,
+                                                             float4* __restrict__ radiance,
+                                                             uint32_t* __restrict__ spill, uint32_t spill_stride) {
+  __shared__ uint32_t s_ring[kShort * kTraceBlock];
+  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
   if (i >= n) return;
   const float4 d = rays[2 * (size_t)i + 1];
   if (!(d.w >= 0.f)) return;
   const RayState r = make_ray(rays[2 * (size_t)i], d);
-  const HitResult h = traverse<kBlock, true>(nodes, leaf_tris, r, SWR_MASK_SHADOW, s_stack + threadIdx.x);
+  const HitResult h = traverse<kTraceBlock, kShort, true>(nodes, leaf_tris, r, SWR_MASK_SHADOW, s_ring + threadIdx.x,
+                                                          spill + (size_t)i * spill_stride);
   if (h.id != kMiss) return;
   const float4 c = col[i];
   const uint32_t s = __float_as_uint(c.w);
@@ -642,22 +646,9 @@ int bits_for(uint64_t max_value) {
   return b;
 }
 
-template <class F>
-void with_traversal(uint32_t max_depth, F&& f) {
-  const uint32_t need = stack_need(max_depth);
-  if (need <= 32)
-    f(std::integral_constant<int, 32>(), std::integral_constant<int, 256>());
-  else if (need <= 64)
-    f(std::integral_constant<int, 64>(), std::integral_constant<int, 128>());
-  else if (need <= 128)
-    f(std::integral_constant<int, 128>(), std::integral_constant<int, 64>());
-  else
-    fail(SWR_E_INVALID, "render: BVH too deep for the traversal stack");
-}
-
 enum Ws {
   kQueueA, kQueueB, kKeysA, kKeysB, kHitKeys, kPerm, kSortAltK, kSortAltV,
-  kShadowRays, kShadowCol,
+  kShadowRays, kShadowCol, kSpill,
   kRadiance, kSampleXY, kCounters, kPixels
 };
 
@@ -720,6 +711,14 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   const int hit_bits_full = bits_for((uint64_t)s.num_tris);  // keys are prim + 1 <= num_tris
   const int hit_bits = ctx->tuning.hit_key_bits > 0 ? std::min(ctx->tuning.hit_key_bits, hit_bits_full) : hit_bits_full;
   const uint32_t hit_shift = (uint32_t)(hit_bits_full - hit_bits);
+  // Traversal stack: an LDS ring per lane, spilling to HBM only past the ring.
+  const int ring = choose_ring(ctx->tuning.trace_stack, s.max_depth);
+  const uint32_t spill_stride = stack_need(s.max_depth);
+  uint32_t* spill = nullptr;
+  if ((uint32_t)ring < spill_stride) {
+    need(kSpill, 4ull * cap * spill_stride);
+    spill = ws<uint32_t>(ctx, kSpill);
+  }
 
   StageClock clk;
   swr_stats acc{};
@@ -751,9 +750,9 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     sort_pairs(ctx, keys[cur], ws<uint32_t>(ctx, kPerm), ws<uint32_t>(ctx, kSortAltK), ws<uint32_t>(ctx, kSortAltV), n,
                ray_bits, true, &rk, &perm, st);
     SWR_HIP(hipEventRecord(clk.ev[2], st));
-    with_traversal(s.max_depth, [&](auto kStack, auto kBlock) {
-      trace_queue_kernel<kStack.value, kBlock.value><<<blocks(n, kBlock.value), kBlock.value, 0, st>>>(
-          s.nodes, s.leaf_tris, perm, n, queue[cur], ws<uint32_t>(ctx, kHitKeys), hit_shift);
+    with_ring(ring, [&](auto kShort) {
+      trace_queue_kernel<kShort.value><<<blocks(n, kTraceBlock), kTraceBlock, 0, st>>>(
+          s.nodes, s.leaf_tris, perm, n, queue[cur], ws<uint32_t>(ctx, kHitKeys), hit_shift, spill, spill_stride);
     });
     SWR_HIP(hipEventRecord(clk.ev[3], st));
     // The ray permutation is consumed; the hit sort reuses its buffers.
@@ -772,9 +771,9 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     so.counters = reinterpret_cast<uint64_t*>(counters);
     shade_kernel<<<blocks(n, kShadeBlock), kShadeBlock, 0, st>>>(s, p, hperm, n, queue[cur], so);
     SWR_HIP(hipEventRecord(clk.ev[5], st));
-    with_traversal(s.max_depth, [&](auto kStack, auto kBlock) {
-      shadow_kernel<kStack.value, kBlock.value><<<blocks(n, kBlock.value), kBlock.value, 0, st>>>(
-          s.nodes, s.leaf_tris, n, so.shadow_rays, so.shadow_col, radiance);
+    with_ring(ring, [&](auto kShort) {
+      shadow_kernel<kShort.value><<<blocks(n, kTraceBlock), kTraceBlock, 0, st>>>(
+          s.nodes, s.leaf_tris, n, so.shadow_rays, so.shadow_col, radiance, spill, spill_stride);
     });
     SWR_HIP(hipEventRecord(clk.ev[6], st));
     SWR_HIP(hipGetLastError());
@@ -894,6 +893,12 @@ int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc
       if (hb < 0 || (hb > 0 && hb < 4)) continue;
       swr::Tuning t = best;
       t.hit_key_bits = hb;
+      const double ms = time_it(t);
+      if (ms < best_ms) best_ms = ms, best = t;
+    }
+    for (int ring : {8, 16, 32, 64}) {
+      swr::Tuning t = best;
+      t.trace_stack = ring;
       const double ms = time_it(t);
       if (ms < best_ms) best_ms = ms, best = t;
     }

@@ -419,7 +419,7 @@ int swr_intersect(swr_context* ctx, const swr_scene* s, const float* rays_dev, s
       swr::fail(SWR_E_INVALID, "intersect: rays and hits must be 16-byte aligned device buffers");
     if (n > 0xFFFFFFFFull) swr::fail(SWR_E_INVALID, "intersect: too many rays");
     swr::Timer timer(ctx);
-    swr::launch_intersect(s->dev, reinterpret_cast<const float4*>(rays_dev), (uint32_t)n, mask,
+    swr::launch_intersect(ctx, s->dev, reinterpret_cast<const float4*>(rays_dev), (uint32_t)n, mask,
                           reinterpret_cast<uint4*>(hits_dev), nullptr, ctx->stream);
     timer.stop();
     SWR_HIP(hipStreamSynchronize(ctx->stream));
@@ -436,7 +436,7 @@ int swr_occluded(swr_context* ctx, const swr_scene* s, const float* rays_dev, si
       swr::fail(SWR_E_INVALID, "occluded: rays must be a 16-byte aligned device buffer");
     if (n > 0xFFFFFFFFull) swr::fail(SWR_E_INVALID, "occluded: too many rays");
     swr::Timer timer(ctx);
-    swr::launch_occluded(s->dev, reinterpret_cast<const float4*>(rays_dev), (uint32_t)n, mask, flags_dev,
+    swr::launch_occluded(ctx, s->dev, reinterpret_cast<const float4*>(rays_dev), (uint32_t)n, mask, flags_dev,
                          ctx->stream);
     timer.stop();
     SWR_HIP(hipStreamSynchronize(ctx->stream));

@@ -92,10 +92,10 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
 // ---- trace.hip ----
 // Closest hit (hits: t, u, v bits and global triangle id) and any-hit occlusion queries.
 // keys (optional) receives the hit shading key (prim + 1, misses 0) of every ray.
-void launch_intersect(const DeviceScene& s, const float4* rays, uint32_t n, uint32_t mask, uint4* hits,
-                      uint32_t* keys, hipStream_t stream);
-void launch_occluded(const DeviceScene& s, const float4* rays, uint32_t n, uint32_t mask, uint32_t* flags,
-                     hipStream_t stream);
+void launch_intersect(swr_context* ctx, const DeviceScene& s, const float4* rays, uint32_t n, uint32_t mask,
+                      uint4* hits, uint32_t* keys, hipStream_t stream);
+void launch_occluded(swr_context* ctx, const DeviceScene& s, const float4* rays, uint32_t n, uint32_t mask,
+                     uint32_t* flags, hipStream_t stream);
 // Stack entries one traversal needs for a BVH4 of the given depth: every level leaves at most
 // three siblings behind, and the deepest node may push four before popping one.
 inline uint32_t stack_need(uint32_t max_depth) { return 3u * max_depth + 4u; }

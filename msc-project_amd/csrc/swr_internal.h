@@ -56,6 +56,7 @@ struct Tuning {
   int sort_kpt = 16;       // keys per thread of a onesweep tile (tile = 256 * kpt)
   int ray_key_bits = 32;   // top bits of the octant + Morton ray key that the ray sort orders by
   int hit_key_bits = 0;    // top bits of the hit key sorted (0 = all: full (geomID, primID) order)
+  int trace_stack = 0;     // LDS ring entries of the traversal stack (0 = from the BVH depth)
 };
 }  // namespace swr
 
@@ -70,6 +71,7 @@ struct swr_context {
   swr::SortWorkspace sort;
   swr::Tuning tuning;
   swr::DeviceBuffer render_ws[24];           // wavefront queues of the render loop (render.hip)
+  swr::DeviceBuffer trace_spill;             // traversal stack spill of the stand-alone queries
   std::unordered_set<swr_scene*> scenes;     // live scenes; destroying the context detaches them
   int num_cus = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // brackets of the last timed entry point

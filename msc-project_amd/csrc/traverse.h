@@ -1,10 +1,10 @@
 // traverse.h -- device-side BVH4 traversal of one ray (closest hit and any hit).
 //
 // Replaces Embree's rtcIntersect (src/core/RayIntersect.cpp:7-12) and rtcOccluded
-// (src/core/Integrator.cpp:153-174). One lane per ray, wave64; each lane keeps its traversal
-// stack in LDS (entry k of lane l at stack[k * BLOCK + l], so a wave's pushes and pops touch
-// 64 consecutive words). Closest-hit traversal visits the children of a node nearest first
-// and pushes the rest far to near.
+// (src/core/Integrator.cpp:153-174). One lane per ray, wave64; each lane keeps the top of its
+// traversal stack in an LDS ring and spills deeper entries to HBM (ShortStack below).
+// Closest-hit traversal visits the children of a node nearest first and pushes the rest far
+// to near.
 //
 // Arithmetic follows the CPU restatement exactly (no contraction): slabs are (lo - o) * (1/d)
 // with zero direction components replaced by +-2^-80, and the triangle test is Moller-Trumbore
@@ -12,6 +12,8 @@
 // tmin < t < tmax). The closest hit is the smallest t, and at exactly equal t the smaller global
 // triangle id, so the result does not depend on the tree or the visiting order.
 #pragma once
+
+#include <type_traits>
 
 #include "scene.h"
 
@@ -89,18 +91,49 @@ struct HitResult {
   uint32_t leaf;  // position of that triangle in the BVH-order leaf array
 };
 
+// Traversal stack of one lane: the top kShort entries live in an LDS ring (entry k at
+// lds[(k % kShort) * kBlock], so a wave's accesses hit 64 consecutive words); entries pushed
+// deeper than that spill, oldest first, to the lane's column in HBM and come back on pop.
+// A small ring keeps occupancy high; the spill makes any BVH depth correct.
+template <int kBlock, int kShort>
+struct ShortStack {
+  static_assert((kShort & (kShort - 1)) == 0, "ring size must be a power of two");
+  uint32_t* lds;    // this lane's ring column (stride kBlock)
+  uint32_t* spill;  // this lane's spill column (entries [0, base))
+  int sp, base;
+  __device__ ShortStack(uint32_t* l, uint32_t* s) : lds(l), spill(s), sp(0), base(0) {}
+  __device__ void push(uint32_t v) {
+    if (sp - base == kShort) {
+      spill[base] = lds[(base & (kShort - 1)) * kBlock];
+      ++base;
+    }
+    lds[(sp & (kShort - 1)) * kBlock] = v;
+    ++sp;
+  }
+  __device__ bool empty() const { return sp == 0; }
+  __device__ uint32_t pop() {
+    --sp;
+    if (sp < base) {
+      base = sp;
+      return spill[sp];
+    }
+    return lds[(sp & (kShort - 1)) * kBlock];
+  }
+};
+
 // Traverses one ray. kAny: returns on the first hit (id of that triangle), else the closest.
-// st points at this lane's stack column in LDS (stride kBlock).
-template <int kBlock, bool kAny>
+// lds_col: this lane's LDS ring column (kShort entries, stride kBlock); spill_col: this lane's
+// HBM spill column (stack_need(depth) entries; unused when the ring is deep enough).
+template <int kBlock, int kShort, bool kAny>
 __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const float4* __restrict__ leaf_tris,
-                                     const RayState& r, uint32_t mask, uint32_t* st) {
+                                     const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col) {
   HitResult best;
   best.t = r.tmax;
   best.u = best.v = __int_as_float(0x7FC00000);
   best.id = kMiss;
   best.leaf = kMiss;
   const float inf = __int_as_float(0x7F800000);
-  int sp = 0;
+  ShortStack<kBlock, kShort> st(lds_col, spill_col);
   uint32_t node = 0;  // root
   for (;;) {
     if (!(node & kLeafBit)) {
@@ -113,10 +146,10 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
       const float t2 = ch.z == kNoChild ? inf : slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, best.t);
       const float t3 = ch.w == kNoChild ? inf : slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, best.t);
       if (kAny) {
-        if (t0 != inf) st[(sp++) * kBlock] = ch.x;
-        if (t1 != inf) st[(sp++) * kBlock] = ch.y;
-        if (t2 != inf) st[(sp++) * kBlock] = ch.z;
-        if (t3 != inf) st[(sp++) * kBlock] = ch.w;
+        if (t0 != inf) st.push(ch.x);
+        if (t1 != inf) st.push(ch.y);
+        if (t2 != inf) st.push(ch.z);
+        if (t3 != inf) st.push(ch.w);
       } else {
         // Entry distances are >= tmin >= 0, so their bit patterns order like the floats; the low
         // two bits carry the child slot.
@@ -133,9 +166,9 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
         const uint32_t cs[4] = {ch.x, ch.y, ch.z, ch.w};
         // Push hit children far to near except the nearest (keys ascend, so a hit k3 implies
         // hits k0..k2), then descend into the nearest.
-        if ((k3 & ~3u) < miss_key) st[(sp++) * kBlock] = cs[k3 & 3u];
-        if ((k2 & ~3u) < miss_key) st[(sp++) * kBlock] = cs[k2 & 3u];
-        if ((k1 & ~3u) < miss_key) st[(sp++) * kBlock] = cs[k1 & 3u];
+        if ((k3 & ~3u) < miss_key) st.push(cs[k3 & 3u]);
+        if ((k2 & ~3u) < miss_key) st.push(cs[k2 & 3u]);
+        if ((k1 & ~3u) < miss_key) st.push(cs[k1 & 3u]);
         if ((k0 & ~3u) < miss_key) {
           node = cs[k0 & 3u];
           continue;
@@ -169,10 +202,35 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
         }
       }
     }
-    if (sp == 0) break;
-    node = st[(--sp) * kBlock];
+    if (st.empty()) break;
+    node = st.pop();
   }
   return best;
+}
+
+// Launch shape of the traversal kernels: workgroups of 128 lanes, an LDS ring of `ring`
+// entries per lane (8, 16, 32 or 64).
+constexpr int kTraceBlock = 128;
+
+// The ring size to use: the tuned one, or else the smallest power of two >= the stack the BVH
+// can need, capped at 64.
+inline int choose_ring(int tuned, uint32_t max_depth) {
+  if (tuned > 0) return tuned;
+  const uint32_t need = stack_need(max_depth);
+  int r = 8;
+  while ((uint32_t)r < need && r < 64) r <<= 1;
+  return r;
+}
+
+template <class F>
+inline void with_ring(int ring, F&& f) {
+  switch (ring) {
+    case 8: f(std::integral_constant<int, 8>()); break;
+    case 16: f(std::integral_constant<int, 16>()); break;
+    case 32: f(std::integral_constant<int, 32>()); break;
+    case 64: f(std::integral_constant<int, 64>()); break;
+    default: fail(SWR_E_INVALID, "traversal: stack ring must be 8, 16, 32 or 64 entries");
+  }
 }
 
 }  // namespace swr

@@ -140,21 +140,24 @@ def test_budget_refuses_large_render(ctx):
 
 
 def test_tuning_changes_speed_not_pixels(ctx):
-    """Sort tile sizes and key widths only reorder rays and hits: every tuning gives the same
-    image, bit for bit."""
+    """Sort tile sizes, key widths and traversal stack rings only reorder rays and hits or move
+    stack entries between LDS and device memory: every tuning gives the same image, bit for bit."""
     desc = scenes.box_room(width=40, height=40, base=2, sphere_level=3)
     sc = swr.Scene(ctx, desc)
     rd, _ = _pair(desc)
     before = ctx.tuning()
     ref = sc.render(rd)[0]
     try:
-        for kpt, rb, hb in [(8, 32, 0), (12, 24, 0), (20, 16, 8), (24, 3, 4)]:
-            ctx.set_tuning(kpt, rb, hb)
+        for kpt, rb, hb, ring in [(8, 32, 0, 8), (12, 24, 0, 16), (20, 16, 8, 64), (24, 3, 4, 32)]:
+            ctx.set_tuning(kpt, rb, hb, ring)
             np.testing.assert_array_equal(sc.render(rd)[0], ref)
         with pytest.raises(swr.SwrError):
             ctx.set_tuning(10, 32, 0)
+        with pytest.raises(swr.SwrError):
+            ctx.set_tuning(16, 32, 0, 12)
         t = ctx.autotune(sc, rd)
         assert t["sort_keys_per_thread"] in (8, 12, 16, 20, 24)
+        assert t["trace_stack_entries"] in (8, 16, 32, 64)
         np.testing.assert_array_equal(sc.render(rd)[0], ref)
     finally:
         ctx.set_tuning(**before)

@@ -141,3 +141,27 @@ def test_terrain_full_size(ctx):
     rays[:, 1] = rng.uniform(0.05, 0.5, 50000)
     check_against_oracle(sc.intersect(rays), oc.intersect(rays))
     assert np.array_equal(sc.occluded(rays), oc.occluded(rays))
+
+
+@pytest.mark.parametrize("ring", [8, 16, 32, 64])
+def test_stack_ring_sizes_agree(ctx, ring):
+    """The traversal keeps the top of its stack in an LDS ring and spills the rest to device
+    memory. A 30000-triangle soup needs more entries than the small rings hold, so rings of 8 and
+    16 spill on many rays; every ring size must give the oracle's hits bit for bit."""
+    desc = scenes.random_soup(11, 30000)
+    sc = swr.Scene(ctx, desc)
+    info = sc.bvh_info()
+    assert 3 * info["max_depth"] + 4 > 16, info  # the small rings do spill
+    oc = orc.Scene(desc)
+    rays = random_rays(np.random.default_rng(11), 20000)
+    before = ctx.tuning()
+    try:
+        ctx.set_tuning(16, 32, 0, 64)
+        g64 = sc.intersect(rays)
+        ctx.set_tuning(16, 32, 0, ring)
+        g = sc.intersect(rays)
+        assert np.array_equal(sc.occluded(rays), oc.occluded(rays))
+    finally:
+        ctx.set_tuning(**before)
+    assert np.array_equal(g, g64)
+    check_against_oracle(g, oc.intersect(rays))
