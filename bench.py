@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sort-bench", action="store_true", help="also time the 16M-key sort of the sort/traversal bench")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch shapes instead of the autotuner: keys per thread, ray key bits, hit key bits, "
+                         "stack ring entries (e.g. 16,16,16,16)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain copy of known size (gather_kernel<4>, identity permutation, 2^26 "
                          "records: 1280 MiB read, 1024 MiB written) to scale rocprof FETCH_SIZE/WRITE_SIZE")
@@ -85,7 +88,11 @@ def main():
     info = scene.bvh_info()
     rd = swr.render_desc(desc["render"])
     t0 = time.time()
-    tuning = ctx.tuning() if args.no_autotune else ctx.autotune(scene, rd)
+    if args.tuning:
+        ctx.set_tuning(*[int(v) for v in args.tuning.split(",")])
+        tuning = ctx.tuning()
+    else:
+        tuning = ctx.tuning() if args.no_autotune else ctx.autotune(scene, rd)
     tune_s = time.time() - t0
 
     for w in range(args.warmup):
@@ -142,6 +149,8 @@ def main():
             "unit": "GB/s",
             "frac": round(trace_gbs / HBM_PEAK_GBS, 5),
             "traffic": None,
+            "avg_launch_ms": round(agg["ms_trace"] / agg["waves"], 4),
+            "rays_per_launch": agg["closest_rays"] // agg["waves"],
             "note": "algorithmic 48 B/ray (ray in + hit out); traversal is latency/L2-bound, node traffic excluded",
         },
         "sort_roofline": {
@@ -167,13 +176,15 @@ def main():
 
     if copy_cal:
         out["copy_calibration"] = copy_cal
-    traffic_file = os.path.join(ROOT, "profiles", "trace_queue_kernel_terrain_traffic.json")
+    traffic_file = os.path.join(ROOT, "profiles", "trace_queue_kernel_terrain_traffic.md")
     if os.path.exists(traffic_file) and args.workload == "terrain":
         # HBM bytes per traversal ray from the committed rocprofv3 --pmc summary (FETCH_SIZE +
         # WRITE_SIZE per dispatch, scaled by the copy calibration), times this run's rays per launch.
+        per_ray = None
         with open(traffic_file) as f:
-            tr = json.load(f)
-        per_ray = tr.get("hbm_bytes_per_ray")
+            for line in f:
+                if line.startswith("hbm_bytes_per_ray:"):
+                    per_ray = float(line.split(":", 1)[1])
         if per_ray:
             out["roofline"]["traffic"] = round(per_ray * agg["closest_rays"] / agg["waves"], 1)
             out["roofline"]["traffic_per_ray"] = per_ray

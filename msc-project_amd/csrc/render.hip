@@ -17,7 +17,7 @@
 // added to the sample in place, in bounce order, as the reference's samples[sampleID] +=
 // (Integrator.cpp:66-71, 171-173) -- no float atomics, and the image is a pure function of the
 // inputs whatever the wave composition. The tent/box resolve gathers each pixel's 3x3
-// neighbourhood in the reference's loop order (TentFilter.cpp:17-61).
+// neighbourhood (TentFilter.cpp:17-61) from LDS tiles, sample index outermost.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -479,55 +479,115 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const Node4* __rest
   radiance[s] = acc;
 }
 
-// Tent / box resolve (TentFilter.cpp:17-61, BoxFilter.cpp:17-36), one lane per pixel with the
-// lanes of a wave on consecutive y of one column, summing in the reference's order (mask x,
-// mask y, sample). Writes the iteration's value to pixels[y * W + x].
-__global__ __launch_bounds__(64) void filter_kernel(FrameParams p, const float4* __restrict__ radiance,
-                                                    const float2* __restrict__ sxy, float* __restrict__ pixels) {
-  const uint32_t py = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t px = blockIdx.y;
-  if (py >= p.height) return;
+// Tent / box resolve (TentFilter.cpp:17-61, BoxFilter.cpp:17-36). A workgroup owns a tile of
+// kFilterCols columns x 64 rows of pixels, one lane per pixel (a wave per column). For each sample
+// index k it stages sample k of the tile and its one-pixel halo, (kFilterCols + 2) x 66 slots, in
+// LDS with row-contiguous loads (slots of one column and sample index are consecutive in y), and
+// every lane adds its 3 x 3 neighbourhood from LDS. Each sample is read from HBM about
+// (kFilterCols + 2) * 66 / (kFilterCols * 64) times instead of nine.
+//
+// Summation order per pixel: sample index k outer, then mask x, mask y (the reference sums mask
+// x, mask y, then k). The terms and weights are the reference's; only the order of the float
+// additions differs. Out-of-image neighbours are staged with weight 0, which adds exactly zero.
+constexpr int kFilterCols = 4;
+constexpr int kFilterRows = 64;
+constexpr int kFilterBlock = kFilterCols * kFilterRows;
+constexpr int kHaloRows = kFilterRows + 2, kHaloCols = kFilterCols + 2;
+constexpr int kHaloSlots = kHaloRows * kHaloCols;
+constexpr int kHaloPerLane = (kHaloSlots + kFilterBlock - 1) / kFilterBlock;
+
+__global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, const float4* __restrict__ radiance,
+                                                              const float2* __restrict__ sxy,
+                                                              float* __restrict__ pixels) {
+  __shared__ float4 s_rad[2][kHaloSlots];
+  __shared__ float2 s_xy[2][kHaloSlots];
   const int W = (int)p.width, H = (int)p.height, spp = (int)p.spp;
-  float out0, out1, out2;
-  if (p.filter == SWR_FILTER_BOX) {
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-    for (int i = 0; i < spp; ++i) {
-      const float4 L = radiance[sample_slot(px, py, (uint32_t)i, H, spp)];
-      s0 += L.x;
-      s1 += L.y;
-      s2 += L.z;
+  const int x0 = (int)blockIdx.y * kFilterCols, y0 = (int)blockIdx.x * kFilterRows;
+  const int lx = (int)threadIdx.x / kFilterRows, ly = (int)threadIdx.x % kFilterRows;
+  const int px = x0 + lx, py = y0 + ly;
+  const bool tent = p.filter != SWR_FILTER_BOX;
+
+  // The halo slots this lane stages: slot e = c * kHaloRows + r is pixel (x0 - 1 + c, y0 - 1 + r).
+  int64_t base[kHaloPerLane];  // slot of sample 0, or -1 outside the image / past the halo
+#pragma unroll
+  for (int j = 0; j < kHaloPerLane; ++j) {
+    const int e = (int)threadIdx.x + j * kFilterBlock;
+    const int c = e / kHaloRows, r = e - c * kHaloRows;
+    const int qx = x0 - 1 + c, qy = y0 - 1 + r;
+    const bool in = e < kHaloSlots && qx >= 0 && qx < W && qy >= 0 && qy < H && (tent || (c >= 1 && c <= kFilterCols && r >= 1 && r <= kFilterRows));
+    base[j] = in ? (int64_t)qx * spp * H + qy : -1;
+  }
+  float4 pr[kHaloPerLane];
+  float2 pq[kHaloPerLane];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int j = 0; j < kHaloPerLane; ++j) {
+      if (base[j] >= 0) {
+        const size_t sl = (size_t)base[j] + (size_t)k * H;
+        pr[j] = radiance[sl];
+        pq[j] = sxy[sl];
+      } else {
+        pr[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pq[j] = make_float2(-1e30f, -1e30f);  // weight 0 for every pixel of the tile
+      }
     }
-    out0 = s0 / (float)spp;
-    out1 = s1 / (float)spp;
-    out2 = s2 / (float)spp;
-  } else {
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, den = 0.f;
-    const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
-    for (int mx = -1; mx < 2; ++mx)
-      for (int my = -1; my < 2; ++my) {
-        const int qx = (int)px + mx, qy = (int)py + my;
-        if (qx < 0 || qx >= W || qy < 0 || qy >= H) continue;
-        for (int i = 0; i < spp; ++i) {
-          const uint32_t s = sample_slot((uint32_t)qx, (uint32_t)qy, (uint32_t)i, H, spp);
-          const float2 q = sxy[s];
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < kHaloPerLane; ++j) {
+      const int e = (int)threadIdx.x + j * kFilterBlock;
+      if (e < kHaloSlots) {
+        s_rad[buf][e] = pr[j];
+        s_xy[buf][e] = pq[j];
+      }
+    }
+  };
+
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, den = 0.f;
+  const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+  if (spp > 0) {
+    fetch(0);
+    stage(0);
+  }
+  for (int k = 0; k < spp; ++k) {
+    const int buf = k & 1;
+    if (k + 1 < spp) fetch(k + 1);  // in flight while this sample index is summed
+    __syncthreads();
+    if (tent) {
+#pragma unroll
+      for (int mx = 0; mx < 3; ++mx)
+#pragma unroll
+        for (int my = 0; my < 3; ++my) {
+          const int e = (lx + mx) * kHaloRows + ly + my;
+          const float2 q = s_xy[buf][e];
           const float dx = fmaxf(0.f, 1.f - fabsf(q.x - cx));
           const float dy = fmaxf(0.f, 1.f - fabsf(q.y - cy));
           const float w = dx * dy;
           den = den + w;
-          const float4 L = radiance[s];
+          const float4 L = s_rad[buf][e];
           s0 += L.x * w;
           s1 += L.y * w;
           s2 += L.z * w;
         }
-      }
-    out0 = s0 / den;
-    out1 = s1 / den;
-    out2 = s2 / den;
+    } else {
+      const float4 L = s_rad[buf][(lx + 1) * kHaloRows + ly + 1];
+      s0 += L.x;
+      s1 += L.y;
+      s2 += L.z;
+    }
+    if (k + 1 < spp) stage(buf ^ 1);
   }
+  if (px >= W || py >= H) return;
   float* o = pixels + 3 * ((size_t)py * W + px);
-  o[0] = out0;
-  o[1] = out1;
-  o[2] = out2;
+  if (tent) {
+    o[0] = s0 / den;
+    o[1] = s1 / den;
+    o[2] = s2 / den;
+  } else {
+    o[0] = s0 / (float)spp;
+    o[1] = s1 / (float)spp;
+    o[2] = s2 / (float)spp;
+  }
 }
 
 // Sample records (x, y, r, g, b) in reference sample-id order (Image.h:10-14).
@@ -796,8 +856,8 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     cur ^= 1;
   }
   SWR_HIP(hipEventRecord(clk.ev[1], st));
-  dim3 fgrid(blocks(p.height, 64), p.width);
-  filter_kernel<<<fgrid, 64, 0, st>>>(p, radiance, sxy, ws<float>(ctx, kPixels));
+  dim3 fgrid(blocks(p.height, kFilterRows), blocks(p.width, kFilterCols));
+  filter_kernel<<<fgrid, kFilterBlock, 0, st>>>(p, radiance, sxy, ws<float>(ctx, kPixels));
   SWR_HIP(hipGetLastError());
   SWR_HIP(hipEventRecord(clk.ev[2], st));
   SWR_HIP(hipEventRecord(ev_end, st));
