@@ -30,6 +30,7 @@ extern "C" {
 #define SWR_E_CANCELLED (-4) /* swr_cancel() was called */
 #define SWR_E_IO (-5)        /* file read/write failure */
 #define SWR_E_PARSE (-6)     /* scene file could not be decoded */
+#define SWR_E_BUSY (-7)      /* no free job slot or table entry right now; retry after jobs finish */
 
 /* Ray masks used by the reference: camera and continuation rays see everything
  * (RayDecompress.cpp:20), shadow rays skip the lights (Integrator.cpp:156), and light
@@ -115,7 +116,8 @@ typedef struct {
   int32_t ray_key_bits;          /* top bits of the octant + Morton ray key the ray sort orders by (3..32) */
   int32_t hit_key_bits;          /* top bits of the hit key the hit sort orders by (0 = all bits) */
   int32_t trace_stack_entries;   /* LDS ring of the traversal stack per ray: 8, 16, 32 or 64 entries, deeper
-                                    entries spill to device memory (0 = sized from the BVH depth) */
+                                    entries spill to device memory (0 = per scene: the ring
+                                    swr_batch_autotune chose for it, else sized from its BVH depth) */
 } swr_tuning;
 
 /* ---- context (Pathtracer ctor/dtor, Pathtracer.cpp:369-392) ------------------------------ */
@@ -221,6 +223,27 @@ int swr_raygen(swr_context* ctx, const swr_render_desc* desc, uint32_t sid0, siz
  * order (Camera.cpp:28). stats may be NULL. */
 int swr_render(swr_context* ctx, const swr_scene* scene, const swr_render_desc* desc, float* pixels, float* samples,
                swr_stats* stats);
+
+/* ---- batched renders (Pathtracer::process of many cameras in one loop) ----------------------
+ * The reference renders one image per Pathtracer and drains its bins batch by batch
+ * (Pathtracer.cpp:451-475). A batch traces the rays of many render jobs (any resolution, spp
+ * and resident scene) through one persistent wavefront loop: every wave is filled from all jobs,
+ * the per-wave kernels are captured once as HIP graphs and replayed without host round trips.
+ * Each job's image is bit-identical to swr_render of the same job alone. */
+typedef struct swr_batch swr_batch;
+int swr_batch_create(swr_context* ctx, swr_batch** out);
+void swr_batch_destroy(swr_batch* batch);
+/* Adds a job (at most 65536; scenes: at most 256 distinct). *job receives its index. */
+int swr_batch_add(swr_batch* batch, const swr_scene* scene, const swr_render_desc* desc, int32_t* job);
+/* Traces every job to completion with waves of wave_rays rays (0 = 2^23). stats: the whole
+ * batch (ms_total = first raygen to the last resolved framebuffer; no per-stage times). */
+int swr_batch_run(swr_batch* batch, uint32_t wave_rays, swr_stats* stats);
+/* A finished job's filtered pixels (W*H*3 floats) and optionally its samples (as swr_render). */
+int swr_batch_result(swr_batch* batch, int32_t job, float* pixels, float* samples);
+/* Chooses the traversal stack ring per scene (recorded on each scene, used by every later loop
+ * while the context's trace_stack_entries is 0), then the sort tile and key widths, on reduced
+ * copies of the batch's jobs (about 1/16 of their samples). */
+int swr_batch_autotune(swr_batch* batch, uint32_t wave_rays, swr_tuning* out);
 
 /* ---- file output (main.cpp:197-208) ------------------------------------------------------ */
 /* Writes W*H*3 floats as an uncompressed scanline OpenEXR file of half RGBA with A = 1. */

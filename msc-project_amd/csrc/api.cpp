@@ -110,6 +110,7 @@ void swr_destroy(swr_context* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (swr_scene* s : ctx->scenes) swr::detach_scene(s);
   ctx->scenes.clear();
+  swr::destroy_loop_buffers(ctx);
   for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
   ctx->allocs.clear();
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -1,288 +1,133 @@
-// render.hip -- the sorted wavefront loop of one render on gfx950 (Pathtracer::process,
-// src/core/Pathtracer.cpp:425-493).
+// render.hip -- the sorted wavefront loop on gfx950 (Pathtracer::process,
+// src/core/Pathtracer.cpp:425-493), shared by the plain render, the batched loop and the
+// serving loop.
 //
 // The reference generates every camera ray up front, bins rays into six direction files,
 // then repeatedly pops a batch, sorts it, traces it with Embree, sorts the hits and shades
 // them, writing continuation rays back to the bins (Pathtracer.cpp:451-475). Here one device
-// queue of 2^bin_exponent rays replaces the bins and batches:
-//   raygen      tops the queue up with camera rays for the next sample ids (Camera.cpp:6-59);
-//   ray sort    octant + Morton keys (written by raygen and shade) -> stable radix sort;
+// queue of rays replaces the bins and batches, and one wave is:
+//   wave begin  continuations of the last wave + room for camera rays -> wave size (device);
+//   raygen      tops the queue up with camera rays for the next global path indices, which map
+//               to (job, sample id) through the job range table (Camera.cpp:6-59);
+//   ray sort    scene + octant + Morton keys (written by raygen and shade) -> stable radix sort;
 //   trace       closest hit in ray-key order -> (t, leaf) into the ray's record + hit key;
-//   hit sort    stable radix sort by (geomID, primID) (CompareHit, RayUncompressed.h:54-60);
+//   hit sort    stable radix sort by (scene, geomID, primID) (CompareHit, RayUncompressed.h:54-60);
 //   shade       MIS shading in hit order (Integrator.cpp:12-255): emission and environment
 //               radiance, one shadow ray per NEE sample at the same hit-order position, one
 //               continuation per survivor appended with one atomic per 1024-lane workgroup;
-//   shadow      any-hit traversal; unoccluded rays add their contribution.
+//   shadow      any-hit traversal; unoccluded rays add their contribution;
+//   wave end    the oldest job that still has a live path (job completion for the drivers).
 // A path has one live ray, so within a launch no two lanes touch the same sample: radiance is
 // added to the sample in place, in bounce order, as the reference's samples[sampleID] +=
-// (Integrator.cpp:66-71, 171-173) -- no float atomics, and the image is a pure function of the
-// inputs whatever the wave composition. The tent/box resolve gathers each pixel's 3x3
+// (Integrator.cpp:66-71, 171-173) -- no float atomics, and every job's image is a pure function
+// of its inputs whatever else shares its waves. The tent/box resolve gathers each pixel's 3x3
 // neighbourhood (TentFilter.cpp:17-61) from LDS tiles, sample index outermost.
+//
+// Every kernel reads the wave size from device memory, so one wave can be launched with sizes
+// known on the host (the plain, host-paced loop) or captured once into a HIP graph sized for the
+// queue capacity and replayed (the persistent loop of the batched and serving drivers).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
 #include "device_util.h"
 #include "traverse.h"
+#include "wavefront.h"
 
 namespace swr {
 
-void commit_scene(swr_scene* s);
-
 namespace {
 
-const float kPi = 3.14159265358979323846f;  // Constants.h:10-15
-const float kInvPi = 0.31830988618379067154f;
-const float kPi180 = 0.01745329251994329577f;
-const float kEps = 1.19209290e-07f;         // FLT_EPSILON
-
-// ---- small vector helpers (no contraction: -ffp-contract=off) ----------------------------
-struct V3 {
-  float x, y, z;
-};
-__host__ __device__ inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
-__device__ inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ inline V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
-__device__ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ inline V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-__device__ inline float norm(V3 a) { return sqrtf(dot(a, a)); }
-__device__ inline V3 normalize(V3 a) {
-  const float n = norm(a);
-  return n > 0.f ? a * (1.f / n) : a;
-}
-__device__ inline float maxc(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
-__device__ inline V3 apply(const float* m, V3 v) {
-  return v3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
-            m[6] * v.x + m[7] * v.y + m[8] * v.z);
-}
-
-// ---- frame constants -----------------------------------------------------------------------
-struct CameraRec {
-  float rot[9];  // Ry * Rx * Rz (PinHoleCamera.cpp:7-11)
-  float t[3];
-  float n[3];    // R * x-hat, normalised (PinHoleCamera.cpp:13)
-  int32_t type;
-  float focal_length, focal_distance, aperture;
-};
-
-struct FrameParams {
-  uint32_t width, height, spp, base;
-  int32_t sampler, filter, min_depth, max_depth;
-  float threshold;
-  uint32_t iteration;
-  uint32_t seed_lo, seed_hi;
-  uint32_t num_lights;
-  float pick;           // 1 / number of lights
-  float3 key_lo, key_scale;
-  uint32_t ray_key_shift;  // the ray sort orders by the top (32 - shift) key bits
-  V3 env;
-  int32_t env_nonzero;
-  CameraRec cam;
-};
-
-// ---- BSDFs (LambertShader.cpp, NullShader.cpp; Glossy = GGX reflection) -----------------
-struct Eval {
-  V3 f;
-  float cos;
-  float pdf;
-};
-
-// Local frame of the samplers (LambertShader.cpp:65-67).
-__device__ inline void shading_frame(V3 n, V3* s, V3* t) {
-  const V3 temp = (n.x > 0.99f) ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
-  *s = normalize(cross(temp, n));
-  *t = normalize(cross(n, *s));
-}
-
-__device__ inline float ggx_d(float a2, float ch) {
-  const float k = ch * ch * (a2 - 1.f) + 1.f;
-  return a2 / (kPi * k * k);
-}
-__device__ inline float ggx_g1(float a2, float c) { return 2.f * c / (c + sqrtf(a2 + (1.f - a2) * c * c)); }
-
-__device__ inline Eval bsdf_eval(const ShaderRec& sh, V3 in, V3 out, V3 n) {
-  Eval e;
-  if (sh.type == SWR_SHADER_NULL) {  // NullShader::evaluate (NullShader.cpp:25-38)
-    e.f = v3(0.f, 0.f, 0.f);
-    e.cos = 1.f;
-    e.pdf = 1.f * kInvPi * 0.5f;
-    return e;
-  }
-  const float ci = dot(n, in), co = dot(n, out);
-  const V3 col = v3(sh.colour[0], sh.colour[1], sh.colour[2]);
-  if (sh.type == SWR_SHADER_LAMBERT) {  // LambertShader::evaluate (LambertShader.cpp:27-47)
-    e.f = (ci < 0.f || co < 0.f) ? v3(0.f, 0.f, 0.f) : col * sh.reflectance * kInvPi;
-    e.cos = ci < 0.f ? 0.f : ci;
-    e.pdf = e.cos * kInvPi;
-    return e;
-  }
-  e.cos = ci < 0.f ? 0.f : ci;
-  if (ci <= 0.f || co <= 0.f) {
-    e.f = v3(0.f, 0.f, 0.f);
-    e.pdf = 0.f;
-    return e;
-  }
-  const float a = sh.roughness, a2 = a * a;
-  const V3 h = normalize(in + out);
-  const float ch = dot(n, h), oh = dot(out, h);
-  const float D = ggx_d(a2, ch);
-  const float G = ggx_g1(a2, ci) * ggx_g1(a2, co);
-  e.f = col * (sh.reflectance * D * G / (4.f * ci * co));
-  e.pdf = oh > 0.f ? D * ch / (4.f * oh) : 0.f;
-  return e;
-}
-
-// Samples an input direction; false when the sample carries no energy (the path ends).
-__device__ inline bool bsdf_sample(const ShaderRec& sh, float u1, float u2, V3 out, V3 n, V3* in, Eval* e) {
-  if (sh.type == SWR_SHADER_NULL) return false;  // NullShader::sample: zero weight
-  V3 s, t;
-  shading_frame(n, &s, &t);
-  const V3 col = v3(sh.colour[0], sh.colour[1], sh.colour[2]);
-  if (sh.type == SWR_SHADER_LAMBERT) {  // LambertShader::sample (LambertShader.cpp:49-73)
-    const float azimuth = 2.f * kPi * u1;
-    const float zenith = acosf(sqrtf(u2));
-    // sphericalToCartesianCoord (Common.h:28-33)
-    const float x = cosf(azimuth) * sinf(zenith);
-    const float y = sinf(azimuth) * sinf(zenith);
-    const float z = cosf(zenith);
-    *in = t * x + s * y + n * z;
-    e->f = col * sh.reflectance * kInvPi;
-    e->cos = dot(n, *in);
-    e->pdf = e->cos * kInvPi;
-    return e->pdf > 0.f;
-  }
-  const float a = sh.roughness, a2 = a * a;
-  const float phi = 2.f * kPi * u1;
-  const float tan2 = a2 * u2 / (1.f - u2);
-  const float ch = 1.f / sqrtf(1.f + tan2);
-  const float sh_ = sqrtf(fmaxf(0.f, 1.f - ch * ch));
-  const V3 h = t * (cosf(phi) * sh_) + s * (sinf(phi) * sh_) + n * ch;
-  const float oh = dot(out, h);
-  *in = h * (2.f * oh) - out;
-  *e = bsdf_eval(sh, *in, out, n);
-  return e->pdf > 0.f && maxc(e->f) > 0.f;
-}
-
-// QuadLight::illuminate (QuadLight.cpp:41-67).
-__device__ inline void light_illuminate(const LightRec& L, float ux, float uy, V3 p, V3* dir, float* dist, V3* rad,
-                                        float* pdfw) {
-  const V3 local = v3(0.f, L.sx * (ux - 0.5f), L.sy * (uy - 0.5f));
-  const V3 smp = v3(L.t[0], L.t[1], L.t[2]) + apply(L.rot, local);
-  const V3 ln = v3(L.n[0], L.n[1], L.n[2]);
-  float c = dot(normalize(p - smp), ln);
-  *rad = c < 0.f ? v3(0.f, 0.f, 0.f) : v3(L.intensity, L.intensity, L.intensity);
-  c = c < 0.f ? 0.f : c;
-  const V3 dv = smp - p;
-  *dir = normalize(dv);
-  *dist = norm(dv);
-  *pdfw = (dot(dv, dv) / c) / (L.sx * L.sy);
-}
-
-// ---- camera (Camera.cpp:15-38, PinHoleCamera.cpp:16-32, ThinLensCamera.cpp:16-56) -------
-__device__ inline void camera_ray(const FrameParams& p, uint32_t sid, V3* org, V3* dir, float* sx, float* sy) {
-  const uint32_t W = p.width, H = p.height, spp = p.spp;
-  const uint32_t x = sid / (H * spp);
-  const uint32_t rem = sid - x * H * spp;
-  const uint32_t y = rem / spp;
-  const uint32_t k = rem - y * spp;
-  const Uniform4 r = philox_uniform4(p.seed_lo, p.seed_hi, sid, p.iteration, 0u, 0u);
-  const uint32_t base = p.base > 0 ? p.base : 1u;
-  const float cell = 1.f / (float)base;
-  const uint32_t ix = k / base, iy = k % base;
-  float jx, jy;
-  if (p.sampler == SWR_SAMPLER_INDEPENDENT) {  // IndependentSampler.cpp:11-12
-    jx = r.r[0];
-    jy = r.r[1];
-  } else if (p.sampler == SWR_SAMPLER_GRID) {  // GridSampler.cpp:13-14
-    jx = (float)ix * cell + 0.5f * cell;
-    jy = (float)iy * cell + 0.5f * cell;
-  } else {  // StratifiedSampler.cpp:13-14
-    jx = (float)ix * cell + r.r[0] * cell;
-    jy = (float)iy * cell + r.r[1] * cell;
-  }
-  *sx = (float)x + jx;
-  *sy = (float)y + jy;
-  const float fx = (((float)x + jx) * 2.f - (float)W) / (float)W * 36.f;  // Camera.cpp:34
-  const float fy = (((float)y + jy) * 2.f - (float)H) / (float)W * 36.f;  // Camera.cpp:35
-  const CameraRec& c = p.cam;
-  const V3 ct = v3(c.t[0], c.t[1], c.t[2]), cn = v3(c.n[0], c.n[1], c.n[2]);
-  const V3 nodal = ct + cn * c.focal_length;
-  const V3 film = ct + apply(c.rot, v3(0.f, fy, fx));
-  if (c.type == SWR_CAMERA_PINHOLE) {
-    *org = nodal;
-    *dir = normalize(nodal - film);
-    return;
-  }
-  // Thin lens: rejection-sampled aperture disk, rays through the focal-plane point.
-  const float ratio = c.focal_distance / c.focal_length;
-  const V3 focal = film + (nodal - film) * ratio;
-  float ax = 0.f, ay = 0.f;
-  float cx0 = r.r[2] - 0.5f, cy0 = r.r[3] - 0.5f;
-  for (uint32_t attempt = 0; attempt < 64; ++attempt) {
-    if (attempt) {
-      const Uniform4 q = philox_uniform4(p.seed_lo, p.seed_hi, sid, p.iteration, 0u, attempt);
-      cx0 = q.r[0] - 0.5f;
-      cy0 = q.r[1] - 0.5f;
-    }
-    if (sqrtf(cx0 * cx0 + cy0 * cy0) < 0.5f) {
-      ax = cx0;
-      ay = cy0;
-      break;
-    }
-  }
-  const float scale = c.focal_length / c.aperture;
-  const V3 ap = ct + apply(c.rot, v3(0.f, ax * scale, ay * scale));
-  *org = ap + cn * c.focal_length;
-  *dir = normalize(focal - *org);
-}
-
-// ---- queue records ----------------------------------------------------------------------------
-// One queued ray is one 64-byte line: origin + tmin, direction + tmax, throughput + lastPdf, and
-// (sample id, depth, hit t, hit leaf). Traversal fills the last two words, so shading gathers
-// each ray with a single line read.
-struct alignas(16) RayRec {
-  float4 o;
-  float4 d;
-  float4 w;
-  uint4 id;
-};
-static_assert(sizeof(RayRec) == 64, "ray record must be one 64-byte line");
-
-// Per-sample state is stored column-major by sample: slot = (x * spp + k) * H + y, so that the
-// 64 lanes of a resolve wave (consecutive y) read consecutive slots.
-__host__ __device__ inline uint32_t sample_slot(uint32_t x, uint32_t y, uint32_t k, uint32_t H, uint32_t spp) {
-  return (x * spp + k) * H + y;
-}
-__device__ inline uint32_t slot_of_sid(const FrameParams& p, uint32_t sid) {
-  const uint32_t x = sid / (p.height * p.spp);
-  const uint32_t rem = sid - x * p.height * p.spp;
-  const uint32_t y = rem / p.spp;
-  return sample_slot(x, y, rem - y * p.spp, p.height, p.spp);
-}
-
 // ---- kernels --------------------------------------------------------------------------------
-// Camera rays for sample ids [sid0, sid0 + k) into queue slots [slot0, slot0 + k); resets each
-// sample's radiance and records its film position (Camera.cpp:23-33).
-__global__ void raygen_kernel(FrameParams p, uint32_t sid0, uint32_t k, uint32_t slot0, RayRec* __restrict__ q,
-                              uint32_t* __restrict__ keys, float2* __restrict__ sample_xy,
-                              float4* __restrict__ radiance) {
+__global__ void wave_begin_kernel(WaveState* w) {
+  const uint64_t c = w->counters;
+  const uint32_t n_cont = (uint32_t)c;
+  w->shadow_rays += c >> 32;
+  const uint64_t left = w->total - w->cursor;
+  const uint32_t k = (uint32_t)min((uint64_t)(w->cap - n_cont), left);
+  const uint32_t n = n_cont + k;
+  w->raygen_base = w->cursor;
+  w->cursor += k;
+  w->n = n;
+  w->k = k;
+  w->counters = 0;
+  w->camera_rays += k;
+  w->closest_rays += n;
+  if (n) w->waves += 1;
+  w->max_wave = max(w->max_wave, n);
+}
+
+// Index (in [range_lo, num_ranges)) of the range holding global path g.
+__device__ inline uint32_t find_range(const JobRange* __restrict__ ranges, uint32_t mask, uint32_t lo, uint32_t hi_excl,
+                                      uint64_t g) {
+  uint32_t hi = hi_excl - 1;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo + 1) >> 1);
+    if (ranges[mid & mask].path_begin <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo & mask;
+}
+
+// Lanes of a wave take turns by `value` (a scene or a job): each turn serves the value of the
+// lowest pending lane, read into a wave-uniform register so that what it indexes is read with
+// scalar loads. Queues are sorted by scene first and camera rays come in job order, so a wave
+// almost always finishes in one turn.
+#define SWR_BY_UNIFORM(pending, value, u, ...)                                                    \
+  for (;;) {                                                                                     \
+    const uint64_t m_ = __ballot(pending);                                                       \
+    if (m_ == 0) break;                                                                          \
+    const uint32_t u = __builtin_amdgcn_readfirstlane(                                           \
+        __shfl((int)(value), __ffsll((unsigned long long)m_) - 1, 64));                          \
+    if ((pending) && (value) == u) {                                                             \
+      pending = false;                                                                           \
+      __VA_ARGS__                                                                                \
+    }                                                                                            \
+  }
+
+// Camera rays for the wave's k new paths into its last k queue slots; resets each sample's
+// radiance and records its film position (Camera.cpp:23-33).
+template <bool kOne>
+__global__ __launch_bounds__(256) void raygen_kernel(const FrameParams* __restrict__ jobs,
+                                                     const JobRange* __restrict__ ranges, uint32_t range_mask,
+                                                     const WaveState* __restrict__ w, RayRec* __restrict__ q,
+                                                     uint32_t* __restrict__ keys, float2* __restrict__ sample_xy,
+                                                     float4* __restrict__ radiance) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= k) return;
-  const uint32_t sid = sid0 + t, slot = slot0 + t;
-  V3 o, d;
-  float sx, sy;
-  camera_ray(p, sid, &o, &d, &sx, &sy);
-  RayRec r;
-  r.o = make_float4(o.x, o.y, o.z, 0.001f);    // RayDecompress.cpp:15
-  r.d = make_float4(d.x, d.y, d.z, 100000.f);  // RayDecompress.cpp:16
-  r.w = make_float4(1.f, 1.f, 1.f, 1.f);       // weight 1, lastPdf 1 (Camera.cpp:23-26)
-  r.id = make_uint4(sid, 0u, 0u, kMiss);
-  q[slot] = r;
-  keys[slot] = ray_key(o.x, o.y, o.z, d.x, d.y, d.z, p.key_lo, p.key_scale) >> p.ray_key_shift;
-  const uint32_t s = slot_of_sid(p, sid);
-  sample_xy[s] = make_float2(sx, sy);
-  radiance[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t k = w->k;
+  if (blockIdx.x * blockDim.x >= k) return;
+  bool pending = t < k;
+  uint32_t job = 0, sid = 0;
+  if (pending) {
+    const uint64_t g = w->raygen_base + t;
+    const JobRange rg = ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, g)];
+    job = rg.job;
+    sid = (uint32_t)(g - rg.path_begin);
+  }
+  const uint32_t slot = w->n - k + t;
+  auto body = [&](uint32_t j0) {
+    const FrameParams& p = jobs[j0];
+    V3 o, d;
+    float sx, sy;
+    camera_ray(p, sid, &o, &d, &sx, &sy);
+    RayRec r;
+    r.o = make_float4(o.x, o.y, o.z, 0.001f);    // RayDecompress.cpp:15
+    r.d = make_float4(d.x, d.y, d.z, 100000.f);  // RayDecompress.cpp:16
+    r.w = make_float4(1.f, 1.f, 1.f, 1.f);       // weight 1, lastPdf 1 (Camera.cpp:23-26)
+    r.id = make_uint4(sid, j0 << 16, p.scene, kMiss);
+    q[slot] = r;
+    keys[slot] = queue_key(p, o, d);
+    const uint64_t s = p.sample_base + slot_of_sid(p, sid);
+    sample_xy[s] = make_float2(sx, sy);
+    radiance[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+  };
+  if (kOne) {
+    if (pending) body(0u);
+  } else {
+    SWR_BY_UNIFORM(pending, job, j0, { body(j0); })
+  }
 }
 
 // Stand-alone camera rays (swr_raygen): ray t as 8 floats, film position t.
@@ -298,57 +143,102 @@ __global__ void raygen_rays_kernel(FrameParams p, uint32_t sid0, uint32_t k, flo
   if (sample_xy) sample_xy[t] = make_float2(sx, sy);
 }
 
-// Closest hit of every queue slot, visited in ray-key order (perm). The hit (t, leaf) goes into
-// the slot's record, the hit key (global triangle id + 1, misses 0) into keys.
-template <int kShort>
-__global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const Node4* __restrict__ nodes,
-                                                                  const float4* __restrict__ leaf_tris,
-                                                                  const uint32_t* __restrict__ perm, uint32_t n,
+
+// Closest hit of every queued ray, visited in ray-key order (perm). The hit (t, leaf) goes into
+// the ray's record, the hit key ((scene, global triangle id + 1), misses 0) into hit_keys. With
+// kRingFilter, only rays whose scene uses this launch's ring size are traced.
+// The scene of ray j is the top scene_bits bits of its sorted key (sorted_keys[j] >> scene_shift):
+// the ray record's scene word is overwritten with the hit t by whichever launch traces it.
+template <int kShort, bool kRingFilter, bool kOne>
+__global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const DeviceScene* __restrict__ scenes,
+                                                                  const uint8_t* __restrict__ scene_ring, int ring,
+                                                                  const uint32_t* __restrict__ perm,
+                                                                  const uint32_t* __restrict__ sorted_keys,
+                                                                  uint32_t scene_shift,
+                                                                  const WaveState* __restrict__ w,
                                                                   RayRec* __restrict__ q,
-                                                                  uint32_t* __restrict__ hit_keys, uint32_t hit_shift,
+                                                                  uint32_t* __restrict__ hit_keys,
+                                                                  uint32_t hit_tri_bits, uint32_t hit_shift,
                                                                   uint32_t* __restrict__ spill, uint32_t spill_stride) {
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
   const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t i = perm[j];
-  const RayState r = make_ray(q[i].o, q[i].d);
-  const HitResult h = traverse<kTraceBlock, kShort, false>(nodes, leaf_tris, r, SWR_MASK_ALL, s_ring + threadIdx.x,
-                                                           spill + (size_t)j * spill_stride);
-  reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
-  hit_keys[i] = (h.id + 1u) >> hit_shift;
+  const uint32_t n = w->n;
+  if (blockIdx.x * kTraceBlock >= n) return;
+  bool pending = j < n;
+  uint32_t i = 0, scene = 0;
+  float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
+  if (pending) {
+    i = perm[j];
+    if (!kOne) scene = sorted_keys[j] >> scene_shift;
+    if (kRingFilter && scene_ring[scene] != (uint32_t)ring) pending = false;
+  }
+  if (pending) {
+    o = q[i].o;
+    d = q[i].d;
+  }
+  auto body = [&](uint32_t s0) {
+    const RayState r = make_ray(o, d);
+    const HitResult h = traverse<kTraceBlock, kShort, false>(scenes[s0].nodes, scenes[s0].leaf_tris, r,
+                                                             SWR_MASK_ALL, s_ring + threadIdx.x,
+                                                             spill + (size_t)j * spill_stride);
+    reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
+    hit_keys[i] = ((s0 << hit_tri_bits) | (h.id + 1u)) >> hit_shift;
+  };
+  if (kOne) {
+    if (pending) body(0u);
+  } else {
+    SWR_BY_UNIFORM(pending, scene, s0, { body(s0); })
+  }
 }
 
 struct ShadeOut {
   RayRec* next;          // continuation queue (compacted)
   uint32_t* next_keys;
-  float4* shadow_rays;   // 2 float4 per hit-order position m; tmax < 0 marks "no shadow ray"
-  float4* shadow_col;    // rgb contribution (already times throughput), sample slot bits in w
-  float4* radiance;      // per-sample radiance, accumulated in place
-  uint64_t* counters;    // low word: continuations, high word: shadow rays
+  float4* shadow;        // 3 float4 per hit-order position m: origin + tag, direction + tmax, rgb + slot
+  float4* radiance;      // per-sample radiance arena, accumulated in place
+  uint32_t* block_min;   // per workgroup: smallest admission number among its continuations
 };
 
 constexpr int kShadeBlock = 1024;
 
-// Integrator::operator() for every slot, in hit-key order (Integrator.cpp:12-255).
+// Integrator::operator() for every queued ray, in hit-key order (Integrator.cpp:12-255).
 // Radiance is added in place: a path has one live ray, so no two lanes of a launch touch the
 // same sample and each sample receives its contributions in bounce order, exactly as the
 // reference's samples[sampleID] += (Integrator.cpp:66-71, 171-173), without atomics.
-__global__ __launch_bounds__(kShadeBlock) void shade_kernel(DeviceScene s, FrameParams p,
-                                                            const uint32_t* __restrict__ perm, uint32_t n,
+// Shadow record at position m: origin (tmin is the reference's 0.001, Integrator.cpp:154; w
+// carries scene << 24 | sample slot bits 55..32), direction + tmax (< 0: no shadow ray), rgb
+// contribution (already times throughput) + sample slot bits 31..0.
+//
+// kOne: the loop holds one job on one scene, whose constants are then wave-uniform (scalar
+// loads). Otherwise each lane reads its job's constants and scene pointers.
+template <bool kOne>
+__global__ __launch_bounds__(kShadeBlock) void shade_kernel(const DeviceScene* __restrict__ scenes,
+                                                            const FrameParams* __restrict__ jobs,
+                                                            const uint32_t* __restrict__ perm, WaveState* w,
                                                             const RayRec* __restrict__ q, ShadeOut out) {
   __shared__ uint32_t s_append[2 * (kShadeBlock / 64) + 1];
+  __shared__ uint32_t s_min[kShadeBlock / 64];
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = m < n;
+  const bool valid = m < w->n;
   bool has_c = false, has_s = false, has_n = false;
   V3 contrib = v3(0.f, 0.f, 0.f), srgb = v3(0.f, 0.f, 0.f);
   V3 P = v3(0.f, 0.f, 0.f), sdir = v3(0.f, 0.f, 0.f), ndir = v3(0.f, 0.f, 0.f), nw = v3(0.f, 0.f, 0.f);
   float sdist = 0.f, npdf = 0.f;
-  uint32_t sid = 0, depth = 0;
+  uint32_t sid = 0, depth = 0, job = 0, scene = 0, seq = 0xFFFFFFFFu;
+  uint64_t sample = 0;
+  uint32_t nkey = 0;
+  RayRec rec;
   if (valid) {
-    const RayRec rec = q[perm[m]];
+    rec = q[perm[m]];
     sid = rec.id.x;
-    depth = rec.id.y;
+    job = kOne ? 0u : rec.id.y >> 16;
+    depth = rec.id.y & 0xFFFFu;
+    if (!kOne) scene = jobs[job].scene;
+  }
+  auto body = [&](const FrameParams& p, const DeviceScene& s) {
     const uint32_t leaf = rec.id.w;
+    seq = p.seq;
+    sample = p.sample_base + slot_of_sid(p, sid);
     const V3 w = v3(rec.w.x, rec.w.y, rec.w.z);
     const float last_pdf = rec.w.w;
     if (leaf == kMiss) {
@@ -360,7 +250,8 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(DeviceScene s, Frame
       }
     } else {
       const float t = __uint_as_float(rec.id.z);
-      const float4 e1 = s.leaf_tris[3 * (size_t)leaf + 1], e2 = s.leaf_tris[3 * (size_t)leaf + 2];
+      const float4* lt = s.leaf_tris + 3 * (size_t)leaf;
+      const float4 e1 = lt[1], e2 = lt[2];
       const GeomRec g = s.geoms[__float_as_uint(e2.w)];
       const V3 dn = normalize(v3(rec.d.x, rec.d.y, rec.d.z));
       if (g.light >= 0) {  // light hit (Integrator.cpp:37-75)
@@ -419,29 +310,45 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(DeviceScene s, Frame
                 has_n = true;
                 ndir = in;
                 npdf = e.pdf;
+                nkey = queue_key(p, P, ndir);
               }
             }
           }
         }
       }
     }
+  };
+  if (valid) body(jobs[kOne ? 0u : job], scenes[kOne ? 0u : scene]);
+  // Oldest job with a live continuation in this workgroup (job completion).
+  uint32_t live = has_n ? seq : 0xFFFFFFFFu;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) live = min(live, (uint32_t)__shfl_xor((int)live, off, 64));
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = live;
+  const uint32_t cslot = block_append<kShadeBlock>(&w->counters, has_n, has_s, s_append);
+  if (threadIdx.x == 0) {
+    uint32_t b = 0xFFFFFFFFu;
+    for (int i = 0; i < kShadeBlock / 64; ++i) b = min(b, s_min[i]);
+    out.block_min[blockIdx.x] = b;
   }
-  const uint32_t cslot = block_append<kShadeBlock>(out.counters, has_n, has_s, s_append);
   if (has_n) {
     RayRec r;
     r.o = make_float4(P.x, P.y, P.z, 0.001f);
     r.d = make_float4(ndir.x, ndir.y, ndir.z, 100000.f);
     r.w = make_float4(nw.x, nw.y, nw.z, npdf);
-    r.id = make_uint4(sid, depth + 1u, 0u, kMiss);
+    r.id = make_uint4(sid, rec.id.y + 1u, scene, kMiss);  // same job, depth + 1
     out.next[cslot] = r;
-    out.next_keys[cslot] = ray_key(P.x, P.y, P.z, ndir.x, ndir.y, ndir.z, p.key_lo, p.key_scale) >> p.ray_key_shift;
+    out.next_keys[cslot] = nkey;
   }
   if (!valid) return;
-  const uint32_t sample = slot_of_sid(p, sid);
   // The shadow ray stays at hit-order position m (no compaction); tmax < 0 marks none.
-  out.shadow_rays[2 * (size_t)m] = make_float4(P.x, P.y, P.z, 0.001f);
-  out.shadow_rays[2 * (size_t)m + 1] = make_float4(sdir.x, sdir.y, sdir.z, has_s ? sdist : -1.f);
-  out.shadow_col[m] = make_float4(srgb.x, srgb.y, srgb.z, __uint_as_float(sample));
+  float4* sr = out.shadow + 3 * (size_t)m;
+  if (has_s) {
+    sr[0] = make_float4(P.x, P.y, P.z, __uint_as_float((scene << 24) | (uint32_t)(sample >> 32)));
+    sr[1] = make_float4(sdir.x, sdir.y, sdir.z, sdist);
+    sr[2] = make_float4(srgb.x, srgb.y, srgb.z, __uint_as_float((uint32_t)sample));
+  } else {
+    sr[1] = make_float4(0.f, 0.f, 0.f, -1.f);
+  }
   if (has_c) {
     float4 acc = out.radiance[sample];
     acc.x = acc.x + contrib.x;
@@ -454,30 +361,83 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(DeviceScene s, Frame
 // Shadow rays (mask 0x0FFFFFFF, Integrator.cpp:153-174), one per hit-order position; positions
 // without a shadow ray carry tmax < 0. An unoccluded ray adds its contribution to its sample in
 // place (one live path per sample, so no two lanes share a sample).
-template <int kShort>
-__global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const Node4* __restrict__ nodes,
-                                                             const float4* __restrict__ leaf_tris, uint32_t n,
-                                                             const float4* __restrict__ rays,
-                                                             const float4* __restrict__ col,
+template <int kShort, bool kRingFilter, bool kOne>
+__global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const DeviceScene* __restrict__ scenes,
+                                                             const uint8_t* __restrict__ scene_ring, int ring,
+                                                             const WaveState* __restrict__ w,
+                                                             const float4* __restrict__ shadow,
                                                              float4* __restrict__ radiance,
                                                              uint32_t* __restrict__ spill, uint32_t spill_stride) {
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
   const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-  if (i >= n) return;
-  const float4 d = rays[2 * (size_t)i + 1];
-  if (!(d.w >= 0.f)) return;
-  const RayState r = make_ray(rays[2 * (size_t)i], d);
-  const HitResult h = traverse<kTraceBlock, kShort, true>(nodes, leaf_tris, r, SWR_MASK_SHADOW, s_ring + threadIdx.x,
-                                                          spill + (size_t)i * spill_stride);
-  if (h.id != kMiss) return;
-  const float4 c = col[i];
-  const uint32_t s = __float_as_uint(c.w);
-  float4 acc = radiance[s];
-  acc.x = acc.x + c.x;
-  acc.y = acc.y + c.y;
-  acc.z = acc.z + c.z;
-  radiance[s] = acc;
+  const uint32_t n = w->n;
+  if (blockIdx.x * kTraceBlock >= n) return;
+  const float4* sr = shadow + 3 * (size_t)i;
+  float4 d = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (i < n) d = sr[1];
+  bool pending = d.w >= 0.f;
+  float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t tag = 0, scene = 0;
+  if (pending) {
+    o = sr[0];
+    tag = __float_as_uint(o.w);
+    scene = tag >> 24;
+  }
+  auto body = [&](uint32_t s0) {
+    if (!kRingFilter || scene_ring[s0] == (uint32_t)ring) {
+      const RayState r = make_ray(make_float4(o.x, o.y, o.z, 0.001f), d);
+      const HitResult h = traverse<kTraceBlock, kShort, true>(scenes[s0].nodes, scenes[s0].leaf_tris, r,
+                                                              SWR_MASK_SHADOW, s_ring + threadIdx.x,
+                                                              spill + (size_t)i * spill_stride);
+      if (h.id == kMiss) {
+        const float4 c = sr[2];
+        const uint64_t s = ((uint64_t)(tag & 0xFFFFFFu) << 32) | __float_as_uint(c.w);
+        float4 acc = radiance[s];
+        acc.x = acc.x + c.x;
+        acc.y = acc.y + c.y;
+        acc.z = acc.z + c.z;
+        radiance[s] = acc;
+      }
+    }
+  };
+  if (kOne) {
+    if (pending) body(0u);
+  } else {
+    SWR_BY_UNIFORM(pending, scene, s0, { body(s0); })
+  }
 }
+
+// After a wave: every job admitted before the oldest job that still has a live path (a
+// continuation in the next queue, or paths not yet generated) has finished -- its last shadow
+// rays were traced in this wave.
+__global__ __launch_bounds__(1024) void wave_end_kernel(const uint32_t* __restrict__ block_min, uint32_t nblocks,
+                                                        const JobRange* __restrict__ ranges, uint32_t range_mask,
+                                                        WaveState* w) {
+  __shared__ uint32_t s_min[1024 / 64];
+  uint32_t m = 0xFFFFFFFFu;
+  for (uint32_t i = threadIdx.x; i < nblocks; i += 1024) m = min(m, block_min[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off, 64));
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < 1024 / 64; ++i) m = min(m, s_min[i]);
+    if (w->cursor < w->total)
+      m = min(m, ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, w->cursor)].seq);
+    w->done_seq = m;
+  }
+}
+
+// Stream-ordered admission of one job: its frame constants, its range and the new path total.
+__global__ void admit_kernel(FrameParams* jobs, FrameParams p, JobRange* ranges, uint32_t range_index, JobRange r,
+                             WaveState* w, uint32_t num_ranges, uint64_t total) {
+  jobs[p.job] = p;
+  ranges[range_index] = r;
+  w->num_ranges = num_ranges;
+  w->total = total;
+}
+
+__global__ void set_range_lo_kernel(WaveState* w, uint32_t range_lo) { w->range_lo = range_lo; }
 
 // Tent / box resolve (TentFilter.cpp:17-61, BoxFilter.cpp:17-36). A workgroup owns a tile of
 // kFilterCols columns x 64 rows of pixels, one lane per pixel (a wave per column). For each sample
@@ -515,7 +475,7 @@ __global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, con
     const int c = e / kHaloRows, r = e - c * kHaloRows;
     const int qx = x0 - 1 + c, qy = y0 - 1 + r;
     const bool in = e < kHaloSlots && qx >= 0 && qx < W && qy >= 0 && qy < H && (tent || (c >= 1 && c <= kFilterCols && r >= 1 && r <= kFilterRows));
-    base[j] = in ? (int64_t)qx * spp * H + qy : -1;
+    base[j] = in ? (int64_t)p.sample_base + (int64_t)qx * spp * H + qy : -1;
   }
   float4 pr[kHaloPerLane];
   float2 pq[kHaloPerLane];
@@ -578,7 +538,7 @@ __global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, con
     if (k + 1 < spp) stage(buf ^ 1);
   }
   if (px >= W || py >= H) return;
-  float* o = pixels + 3 * ((size_t)py * W + px);
+  float* o = pixels + 3 * (p.pixel_base + (size_t)py * W + px);
   if (tent) {
     o[0] = s0 / den;
     o[1] = s1 / den;
@@ -590,12 +550,13 @@ __global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, con
   }
 }
 
-// Sample records (x, y, r, g, b) in reference sample-id order (Image.h:10-14).
+
+// Sample records (x, y, r, g, b) of one job in reference sample-id order (Image.h:10-14).
 __global__ void pack_samples_kernel(FrameParams p, size_t n, const float2* __restrict__ sxy,
                                     const float4* __restrict__ radiance, float* __restrict__ out) {
   const size_t sid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sid >= n) return;
-  const uint32_t s = slot_of_sid(p, (uint32_t)sid);
+  const uint64_t s = p.sample_base + slot_of_sid(p, (uint32_t)sid);
   const float2 q = sxy[s];
   const float4 L = radiance[s];
   float* o = out + 5 * sid;
@@ -606,15 +567,12 @@ __global__ void pack_samples_kernel(FrameParams p, size_t n, const float2* __res
   o[4] = L.z;
 }
 
-__global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0u;
-}
-
 inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
 
+}  // namespace
+
 // ---- host side ------------------------------------------------------------------------------
-void axis_rotation(int axis, float deg, float m[9]) {
+static void axis_rotation(int axis, float deg, float m[9]) {
   const float a = deg * kPi180, c = std::cos(a), s = std::sin(a);
   std::memset(m, 0, 9 * sizeof(float));
   const int i = (axis + 1) % 3, j = (axis + 2) % 3;
@@ -625,7 +583,7 @@ void axis_rotation(int axis, float deg, float m[9]) {
   m[j * 3 + j] = c;
 }
 
-void mat_mul(const float a[9], const float b[9], float out[9]) {
+static void mat_mul(const float a[9], const float b[9], float out[9]) {
   float r[9];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) r[i * 3 + j] = a[i * 3 + 0] * b[0 * 3 + j] + a[i * 3 + 1] * b[1 * 3 + j] + a[i * 3 + 2] * b[2 * 3 + j];
@@ -706,178 +664,444 @@ int bits_for(uint64_t max_value) {
   return b;
 }
 
-enum Ws {
-  kQueueA, kQueueB, kKeysA, kKeysB, kHitKeys, kPerm, kSortAltK, kSortAltV,
-  kShadowRays, kShadowCol, kSpill,
-  kRadiance, kSampleXY, kCounters, kPixels
-};
 
-template <class T>
-T* ws(swr_context* ctx, Ws w) {
-  return static_cast<T*>(ctx->render_ws[w].ptr);
+// ---- WaveLoop --------------------------------------------------------------------------------
+void LoopBuffers::release(swr_context* ctx) {
+  for (DeviceBuffer* b : {&queue[0], &queue[1], &keys[0], &keys[1], &hit_keys, &perm, &alt_k, &alt_v, &shadow, &spill,
+                          &jobs, &ranges, &scene_tab, &scene_ring, &wave, &block_min, &radiance, &sample_xy, &pixels,
+                          &staging})
+    swr::release(ctx, *b);
+  if (host_state) (void)hipHostFree(host_state);
+  host_state = nullptr;
+  for (auto& e : snap_ev)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
+  for (auto& e : stage_ev)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
 }
 
-struct StageClock {
-  hipEvent_t ev[8];
-  StageClock() {
-    for (auto& e : ev) SWR_HIP(hipEventCreate(&e));
+uint64_t WaveLoop::job_samples(const swr_render_desc& d) {
+  return (uint64_t)std::max(d.width, 0) * (uint64_t)std::max(d.height, 0) * spp_of(d);
+}
+uint64_t WaveLoop::job_pixels(const swr_render_desc& d) {
+  return (uint64_t)std::max(d.width, 0) * (uint64_t)std::max(d.height, 0);
+}
+
+WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const swr_scene*>& scenes,
+                   uint32_t max_jobs, uint32_t capacity)
+    : ctx_(ctx), b_(bufs), stream_(ctx->stream), scenes_(scenes), max_jobs_(max_jobs), cap_(capacity) {
+  if (scenes.empty() || scenes.size() > 256) fail(SWR_E_INVALID, "wavefront: 1 to 256 scenes");
+  if (max_jobs == 0 || max_jobs > 65536) fail(SWR_E_INVALID, "wavefront: 1 to 65536 job slots");
+  if (capacity < 64 || capacity > (1u << 24)) fail(SWR_E_INVALID, "wavefront: wave capacity must be 64 .. 2^24");
+  uint32_t max_tris = 1, max_depth = 0;
+  for (const swr_scene* s : scenes) {
+    if (!s || s->ctx != ctx) fail(SWR_E_INVALID, "wavefront: scene belongs to another (or a destroyed) context");
+    if (!s->committed) fail(SWR_E_INVALID, "wavefront: scene not committed");
+    max_tris = std::max(max_tris, s->dev.num_tris);
+    max_depth = std::max(max_depth, s->dev.max_depth);
   }
-  ~StageClock() {
-    for (auto& e : ev) (void)hipEventDestroy(e);
+  scene_bits_ = scenes.size() > 1 ? bits_for(scenes.size() - 1) : 0;
+  hit_tri_bits_ = bits_for((uint64_t)max_tris);  // keys are prim + 1 <= num_tris
+  const int hit_full = scene_bits_ + hit_tri_bits_;
+  if (hit_full > 32) fail(SWR_E_INVALID, "wavefront: scene count and triangle count exceed a 32-bit hit key");
+  const Tuning& tu = ctx->tuning;
+  ray_bits_ = std::max(scene_bits_ + 3, std::min(tu.ray_key_bits, 32));
+  hit_bits_ = tu.hit_key_bits > 0 ? std::min(tu.hit_key_bits, hit_full) : hit_full;
+  hit_shift_ = hit_full - hit_bits_;
+  passes_per_wave_ = (ray_bits_ + 7) / 8 + (hit_bits_ + 7) / 8;
+  spill_stride_ = stack_need(max_depth);
+  std::vector<uint8_t> ring_tab(scenes.size());
+  for (size_t i = 0; i < scenes.size(); ++i) {
+    const int ring = tu.trace_stack > 0 ? tu.trace_stack
+                                        : choose_ring(scenes[i]->tuned_ring, scenes[i]->dev.max_depth);
+    rings_.push_back(ring);
+    ring_tab[i] = (uint8_t)ring;
+    if (std::find(ring_set_.begin(), ring_set_.end(), ring) == ring_set_.end()) ring_set_.push_back(ring);
+    if ((uint32_t)ring < stack_need(scenes[i]->dev.max_depth)) spill_ = true;
   }
-  float ms(int a, int b) {
-    float v = 0.f;
-    SWR_HIP(hipEventElapsedTime(&v, ev[a], ev[b]));
-    return v;
+  range_cap_ = 1;
+  while (range_cap_ < 2 * max_jobs) range_cap_ <<= 1;
+
+  ensure(ctx, b_.queue[0], sizeof(RayRec) * (size_t)cap_);
+  ensure(ctx, b_.queue[1], sizeof(RayRec) * (size_t)cap_);
+  ensure(ctx, b_.keys[0], 4ull * cap_);
+  ensure(ctx, b_.keys[1], 4ull * cap_);
+  ensure(ctx, b_.hit_keys, 4ull * cap_);
+  ensure(ctx, b_.perm, 4ull * cap_);
+  ensure(ctx, b_.alt_k, 4ull * cap_);
+  ensure(ctx, b_.alt_v, 4ull * cap_);
+  ensure(ctx, b_.shadow, 48ull * cap_);
+  if (spill_) ensure(ctx, b_.spill, 4ull * cap_ * spill_stride_);
+  ensure(ctx, b_.jobs, sizeof(FrameParams) * (size_t)max_jobs_);
+  ensure(ctx, b_.ranges, sizeof(JobRange) * (size_t)range_cap_);
+  ensure(ctx, b_.scene_tab, sizeof(DeviceScene) * scenes.size());
+  ensure(ctx, b_.scene_ring, scenes.size());
+  ensure(ctx, b_.wave, sizeof(WaveState));
+  ensure(ctx, b_.block_min, 4ull * blocks(cap_, kShadeBlock));
+  sort_reserve(ctx, cap_);
+  if (!b_.host_state) {
+    SWR_HIP(hipHostMalloc((void**)&b_.host_state, 8 * sizeof(WaveState), hipHostMallocDefault));
+    for (auto& e : b_.snap_ev) SWR_HIP(hipEventCreate(&e));
+    for (auto& e : b_.stage_ev) SWR_HIP(hipEventCreate(&e));
   }
-};
+  std::vector<DeviceScene> tab;
+  for (const swr_scene* s : scenes) tab.push_back(s->dev);
+  SWR_HIP(hipMemcpyAsync(b_.scene_tab.ptr, tab.data(), sizeof(DeviceScene) * tab.size(), hipMemcpyHostToDevice,
+                         stream_));
+  SWR_HIP(hipMemcpyAsync(b_.scene_ring.ptr, ring_tab.data(), ring_tab.size(), hipMemcpyHostToDevice, stream_));
+  WaveState w{};
+  w.cap = cap_;
+  w.done_seq = 0;
+  SWR_HIP(hipMemcpyAsync(b_.wave.ptr, &w, sizeof w, hipMemcpyHostToDevice, stream_));
+  SWR_HIP(hipStreamSynchronize(stream_));
+}
+
+WaveLoop::~WaveLoop() {
+  (void)hipStreamSynchronize(stream_);  // replays may still be in flight
+  for (int c = 0; c < 2; ++c) {
+    if (exec_[c]) (void)hipGraphExecDestroy(exec_[c]);
+    if (graph_[c]) (void)hipGraphDestroy(graph_[c]);
+  }
+}
+
+FrameParams WaveLoop::params(uint32_t scene, const swr_render_desc& desc, uint64_t sample_base,
+                             uint64_t pixel_base) const {
+  if (scene >= scenes_.size()) fail(SWR_E_INVALID, "wavefront: scene index out of range");
+  FrameParams p = make_params(desc, &scenes_[scene]->dev);
+  if ((uint64_t)p.width * p.height * p.spp > 0xFFFFFFFFull)
+    fail(SWR_E_INVALID, "render: more than 2^32 - 1 samples in one iteration");
+  if (p.max_depth > 65534) fail(SWR_E_INVALID, "render: max depth above 65534");
+  p.ray_key_shift = (uint32_t)(32 - ray_bits_);
+  p.key_scene_bits = (uint32_t)scene_bits_;
+  p.key_prefix = scene_bits_ ? scene << (32 - scene_bits_) : 0u;
+  p.scene = scene;
+  p.sample_base = sample_base;
+  p.pixel_base = pixel_base;
+  return p;
+}
+
+void WaveLoop::set_arenas(float4* radiance, float2* sample_xy, float* pixels) {
+  radiance_ = radiance;
+  sxy_ = sample_xy;
+  pixels_ = pixels;
+}
+
+uint32_t WaveLoop::admit(uint32_t slot, FrameParams p) {
+  if (slot >= max_jobs_) fail(SWR_E_INVALID, "wavefront: job slot out of range");
+  if (num_ranges_ - range_lo_ >= range_cap_) fail(SWR_E_BUSY, "wavefront: job range table full");
+  p.job = slot;
+  p.seq = seq_++;
+  const uint64_t paths = (uint64_t)p.width * p.height * p.spp;
+  JobRange r{total_, slot, p.seq};
+  total_ += paths;
+  range_begin_.push_back(r.path_begin);
+  ++num_ranges_;
+  admit_kernel<<<1, 1, 0, stream_>>>(static_cast<FrameParams*>(b_.jobs.ptr), p, static_cast<JobRange*>(b_.ranges.ptr),
+                                     (num_ranges_ - 1) & (range_cap_ - 1), r, static_cast<WaveState*>(b_.wave.ptr),
+                                     num_ranges_, total_);
+  SWR_HIP(hipGetLastError());
+  return p.seq;
+}
+
+void WaveLoop::retire_ranges(uint64_t cursor) {
+  // Keep the range holding `cursor` (and everything after it).
+  uint32_t lo = range_lo_;
+  while (lo + 1 < num_ranges_ && range_begin_[lo + 1] <= cursor) ++lo;
+  if (lo != range_lo_) {
+    range_lo_ = lo;
+    set_range_lo_kernel<<<1, 1, 0, stream_>>>(static_cast<WaveState*>(b_.wave.ptr), range_lo_);
+    SWR_HIP(hipGetLastError());
+  }
+}
+
+void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times) {
+  hipStream_t st = stream_;
+  const uint32_t nb = exact ? n : cap_, kb = exact ? k : cap_;
+  WaveState* w = static_cast<WaveState*>(b_.wave.ptr);
+  const uint32_t* n_dev = exact ? nullptr : &w->n;
+  RayRec* queue[2] = {static_cast<RayRec*>(b_.queue[0].ptr), static_cast<RayRec*>(b_.queue[1].ptr)};
+  uint32_t* keys[2] = {static_cast<uint32_t*>(b_.keys[0].ptr), static_cast<uint32_t*>(b_.keys[1].ptr)};
+  uint32_t* perm = static_cast<uint32_t*>(b_.perm.ptr);
+  uint32_t* alt_k = static_cast<uint32_t*>(b_.alt_k.ptr);
+  uint32_t* alt_v = static_cast<uint32_t*>(b_.alt_v.ptr);
+  uint32_t* hit_keys = static_cast<uint32_t*>(b_.hit_keys.ptr);
+  const FrameParams* jobs = static_cast<const FrameParams*>(b_.jobs.ptr);
+  const JobRange* ranges = static_cast<const JobRange*>(b_.ranges.ptr);
+  const DeviceScene* stab = static_cast<const DeviceScene*>(b_.scene_tab.ptr);
+  const uint8_t* sring = static_cast<const uint8_t*>(b_.scene_ring.ptr);
+  uint32_t* spill = static_cast<uint32_t*>(b_.spill.ptr);
+  const uint32_t mask = range_cap_ - 1;
+  const bool filter = ring_set_.size() > 1;
+  auto mark = [&](int i) {
+    if (times) SWR_HIP(hipEventRecord(b_.stage_ev[i], st));
+  };
+
+  wave_begin_kernel<<<1, 1, 0, st>>>(w);
+  mark(0);
+  // Several scenes always carry scene bits in the ray key (the multi-scene trace reads them).
+  const bool one_job = max_jobs_ == 1 && scenes_.size() == 1, one_scene = scenes_.size() == 1;
+  if (kb) {
+    if (one_job)
+      raygen_kernel<true><<<blocks(kb), 256, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
+    else
+      raygen_kernel<false><<<blocks(kb), 256, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
+  }
+  mark(1);
+  const uint32_t shade_blocks = blocks(nb, kShadeBlock);
+  if (nb) {
+    uint32_t *rk = nullptr, *rperm = nullptr;
+    sort_pairs(ctx_, keys[cur], perm, alt_k, alt_v, nb, ray_bits_, true, &rk, &rperm, st, n_dev);
+    mark(2);
+    const uint32_t scene_shift = (uint32_t)(ray_bits_ - scene_bits_);  // used only with scene_bits_ > 0
+    for (int ring : ring_set_) {
+      with_ring(ring, [&](auto kShort) {
+        if (filter)
+          trace_queue_kernel<kShort.value, true, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
+              spill_stride_);
+        else if (one_scene)
+          trace_queue_kernel<kShort.value, false, true><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
+              spill_stride_);
+        else
+          trace_queue_kernel<kShort.value, false, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
+              spill_stride_);
+      });
+    }
+    mark(3);
+    // The ray permutation is consumed; the hit sort reuses its buffers.
+    uint32_t *hk = nullptr, *hperm = nullptr;
+    uint32_t* spare = rperm == perm ? alt_v : perm;
+    sort_pairs(ctx_, hit_keys, spare, keys[cur], rperm, nb, hit_bits_, true, &hk, &hperm, st, n_dev);
+    mark(4);
+    ShadeOut so;
+    so.next = queue[cur ^ 1];
+    so.next_keys = keys[cur ^ 1];
+    so.shadow = static_cast<float4*>(b_.shadow.ptr);
+    so.radiance = radiance_;
+    so.block_min = static_cast<uint32_t*>(b_.block_min.ptr);
+    if (one_job)
+      shade_kernel<true><<<shade_blocks, kShadeBlock, 0, st>>>(stab, jobs, hperm, w, queue[cur], so);
+    else
+      shade_kernel<false><<<shade_blocks, kShadeBlock, 0, st>>>(stab, jobs, hperm, w, queue[cur], so);
+    mark(5);
+    for (int ring : ring_set_) {
+      with_ring(ring, [&](auto kShort) {
+        if (filter)
+          shadow_kernel<kShort.value, true, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+              stab, sring, ring, w, so.shadow, radiance_, spill, spill_stride_);
+        else if (one_scene)
+          shadow_kernel<kShort.value, false, true><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+              stab, sring, ring, w, so.shadow, radiance_, spill, spill_stride_);
+        else
+          shadow_kernel<kShort.value, false, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+              stab, sring, ring, w, so.shadow, radiance_, spill, spill_stride_);
+      });
+    }
+    mark(6);
+  }
+  wave_end_kernel<<<1, 1024, 0, st>>>(static_cast<const uint32_t*>(b_.block_min.ptr), nb ? shade_blocks : 0u, ranges,
+                                      mask, w);
+  SWR_HIP(hipGetLastError());
+}
+
+void WaveLoop::run_paced(StageTimes* times) {
+  WaveState* hs = b_.host_state;
+  SWR_HIP(hipMemcpyAsync(hs, b_.wave.ptr, sizeof(WaveState), hipMemcpyDeviceToHost, stream_));
+  SWR_HIP(hipStreamSynchronize(stream_));
+  uint32_t n_cont = (uint32_t)hs->counters;
+  uint64_t cursor = hs->cursor;
+  while (true) {
+    if (ctx_->cancel.load()) fail(SWR_E_CANCELLED, "render cancelled");
+    const uint32_t k = (uint32_t)std::min<uint64_t>(cap_ - n_cont, total_ - cursor);
+    const uint32_t n = n_cont + k;
+    if (n == 0) break;
+    enqueue_wave(cur_, true, n, k, times);
+    SWR_HIP(hipMemcpyAsync(hs, b_.wave.ptr, sizeof(WaveState), hipMemcpyDeviceToHost, stream_));
+    SWR_HIP(hipStreamSynchronize(stream_));
+    if (times) {
+      auto ms = [&](int a, int b) {
+        float v = 0.f;
+        SWR_HIP(hipEventElapsedTime(&v, b_.stage_ev[a], b_.stage_ev[b]));
+        return (double)v;
+      };
+      times->raygen += ms(0, 1);
+      times->sort_rays += ms(1, 2);
+      times->trace += ms(2, 3);
+      times->sort_hits += ms(3, 4);
+      times->shade += ms(4, 5);
+      times->shadow += ms(5, 6);
+    }
+    n_cont = (uint32_t)hs->counters;
+    cursor += k;
+    cur_ ^= 1;
+  }
+}
+
+void WaveLoop::capture() {
+  if (exec_[0]) return;
+  const uint64_t epoch_mirror = ctx_->sort.epoch;
+  for (int c = 0; c < 2; ++c) {
+    SWR_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_wave(c, false, cap_, cap_, nullptr);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      ctx_->sort.epoch = epoch_mirror;
+      throw;
+    }
+    SWR_HIP(hipStreamEndCapture(stream_, &graph_[c]));
+    SWR_HIP(hipGraphInstantiate(&exec_[c], graph_[c], nullptr, nullptr, 0));
+  }
+  ctx_->sort.epoch = epoch_mirror;  // capturing issued no tags; replays do
+}
+
+void WaveLoop::launch_graph_waves(int waves) {
+  capture();
+  sort_epoch_guard(ctx_, (uint64_t)passes_per_wave_ * (uint64_t)(waves + 16), stream_);
+  for (int i = 0; i < waves; ++i) {
+    SWR_HIP(hipGraphLaunch(exec_[cur_], stream_));
+    ctx_->sort.epoch += (uint64_t)passes_per_wave_;
+    cur_ ^= 1;
+  }
+}
+
+int WaveLoop::snapshot() {
+  const int t = snap_next_;
+  snap_next_ = (snap_next_ + 1) & 7;
+  SWR_HIP(hipMemcpyAsync(b_.host_state + t, b_.wave.ptr, sizeof(WaveState), hipMemcpyDeviceToHost, stream_));
+  SWR_HIP(hipEventRecord(b_.snap_ev[t], stream_));
+  return t;
+}
+
+const WaveState& WaveLoop::wait_snapshot(int ticket) {
+  SWR_HIP(hipEventSynchronize(b_.snap_ev[ticket]));
+  return b_.host_state[ticket];
+}
+
+void WaveLoop::run_graphs(int check_every) {
+  check_every = std::max(1, check_every);
+  capture();
+  int pending = -1;
+  while (true) {
+    if (ctx_->cancel.load()) {
+      SWR_HIP(hipStreamSynchronize(stream_));
+      fail(SWR_E_CANCELLED, "render cancelled");
+    }
+    launch_graph_waves(check_every);
+    const int t = snapshot();
+    if (pending >= 0) {
+      const WaveState& s = wait_snapshot(pending);
+      if (s.cursor == s.total && s.total == total_ && (uint32_t)s.counters == 0) break;
+    }
+    pending = t;
+  }
+  SWR_HIP(hipStreamSynchronize(stream_));
+}
+
+void WaveLoop::filter(const FrameParams& p) {
+  dim3 fgrid(blocks(p.height, kFilterRows), blocks(p.width, kFilterCols));
+  filter_kernel<<<fgrid, kFilterBlock, 0, stream_>>>(p, radiance_, sxy_, pixels_);
+  SWR_HIP(hipGetLastError());
+}
+
+void WaveLoop::copy_out(const FrameParams& p, float* pixels_host, float* samples_host) {
+  hipStream_t st = stream_;
+  if (pixels_host)
+    SWR_HIP(hipMemcpyAsync(pixels_host, pixels_ + 3 * p.pixel_base, 12ull * p.width * p.height,
+                           hipMemcpyDeviceToHost, st));
+  if (samples_host) {
+    const uint64_t total = (uint64_t)p.width * p.height * p.spp;
+    ensure(ctx_, b_.staging, 20ull * total);
+    pack_samples_kernel<<<blocks(total), 256, 0, st>>>(p, total, sxy_, radiance_, static_cast<float*>(b_.staging.ptr));
+    SWR_HIP(hipGetLastError());
+    SWR_HIP(hipMemcpyAsync(samples_host, b_.staging.ptr, 20ull * total, hipMemcpyDeviceToHost, st));
+    SWR_HIP(hipStreamSynchronize(st));
+    release(ctx_, b_.staging);
+  }
+  SWR_HIP(hipStreamSynchronize(st));
+}
+
+LoopTotals WaveLoop::totals() {
+  const WaveState& s = wait_snapshot(snapshot());
+  LoopTotals t;
+  t.camera_rays = s.camera_rays;
+  t.closest_rays = s.closest_rays;
+  t.shadow_rays = s.shadow_rays + (s.counters >> 32);
+  t.waves = s.waves;
+  t.max_wave = s.max_wave;
+  return t;
+}
+
+void destroy_loop_buffers(swr_context* ctx) {
+  if (!ctx->loop) return;
+  ctx->loop->release(ctx);
+  delete ctx->loop;
+  ctx->loop = nullptr;
+}
+
+// ---- plain render ------------------------------------------------------------------------------
+namespace {
+
+uint32_t wave_capacity(const swr_render_desc& desc, uint64_t total) {
+  const int exp = std::max(10, std::min(desc.bin_exponent > 0 ? desc.bin_exponent : 20, 24));
+  return (uint32_t)std::min<uint64_t>(uint64_t(1) << exp, std::max<uint64_t>(total, 64));
+}
 
 void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_desc& desc, float* pixels_host,
                   float* samples_host, swr_stats* stats) {
   SWR_HIP(hipSetDevice(ctx->device));
-  const DeviceScene& s = scene->dev;
-  FrameParams p = make_params(desc, &s);
-  const int ray_bits = std::max(3, std::min(ctx->tuning.ray_key_bits, 32));
-  p.ray_key_shift = (uint32_t)(32 - ray_bits);
-  const uint64_t total = (uint64_t)p.width * p.height * p.spp;
-  if (total > 0xFFFFFFFFull) fail(SWR_E_INVALID, "render: more than 2^32 - 1 samples in one iteration");
-  const int exp = std::max(10, std::min(desc.bin_exponent > 0 ? desc.bin_exponent : 20, 24));
-  const uint32_t cap = (uint32_t)std::min<uint64_t>(uint64_t(1) << exp, std::max<uint64_t>(total, 64));
+  if (!ctx->loop) ctx->loop = new LoopBuffers();
+  LoopBuffers& b = *ctx->loop;
+  const uint64_t total = WaveLoop::job_samples(desc);
+  WaveLoop loop(ctx, b, {scene}, 1, wave_capacity(desc, total));
+  const FrameParams p = loop.params(0, desc, 0, 0);
+  ensure(ctx, b.radiance, 16ull * total);
+  ensure(ctx, b.sample_xy, 8ull * total);
+  ensure(ctx, b.pixels, 12ull * p.width * p.height);
+  loop.set_arenas(static_cast<float4*>(b.radiance.ptr), static_cast<float2*>(b.sample_xy.ptr),
+                  static_cast<float*>(b.pixels.ptr));
   hipStream_t st = ctx->stream;
-
-  // Queues (two for the current and the next wave) and per-sample state.
-  auto need = [&](Ws w, size_t bytes) { ensure(ctx, ctx->render_ws[w], bytes); };
-  need(kQueueA, sizeof(RayRec) * (size_t)cap);
-  need(kQueueB, sizeof(RayRec) * (size_t)cap);
-  need(kKeysA, 4ull * cap);
-  need(kKeysB, 4ull * cap);
-  need(kHitKeys, 4ull * cap);
-  need(kPerm, 4ull * cap);
-  need(kSortAltK, 4ull * cap);
-  need(kSortAltV, 4ull * cap);
-  need(kShadowRays, 32ull * cap);
-  need(kShadowCol, 16ull * cap);
-  need(kRadiance, 16ull * total);
-  need(kSampleXY, 8ull * total);
-  need(kCounters, 64);
-  need(kPixels, 12ull * p.width * p.height);
-  sort_reserve(ctx, cap);
-
-  uint32_t* counters = ws<uint32_t>(ctx, kCounters);
-  RayRec* queue[2] = {ws<RayRec>(ctx, kQueueA), ws<RayRec>(ctx, kQueueB)};
-  uint32_t* keys[2] = {ws<uint32_t>(ctx, kKeysA), ws<uint32_t>(ctx, kKeysB)};
-  float4* radiance = ws<float4>(ctx, kRadiance);
-  float2* sxy = ws<float2>(ctx, kSampleXY);
-  const int hit_bits_full = bits_for((uint64_t)s.num_tris);  // keys are prim + 1 <= num_tris
-  const int hit_bits = ctx->tuning.hit_key_bits > 0 ? std::min(ctx->tuning.hit_key_bits, hit_bits_full) : hit_bits_full;
-  const uint32_t hit_shift = (uint32_t)(hit_bits_full - hit_bits);
-  // Traversal stack: an LDS ring per lane, spilling to HBM only past the ring.
-  const int ring = choose_ring(ctx->tuning.trace_stack, s.max_depth);
-  const uint32_t spill_stride = stack_need(s.max_depth);
-  uint32_t* spill = nullptr;
-  if ((uint32_t)ring < spill_stride) {
-    need(kSpill, 4ull * cap * spill_stride);
-    spill = ws<uint32_t>(ctx, kSpill);
-  }
-
-  StageClock clk;
-  swr_stats acc{};
-  hipEvent_t ev_start, ev_end;
-  SWR_HIP(hipEventCreate(&ev_start));
-  SWR_HIP(hipEventCreate(&ev_end));
+  hipEvent_t ev[4];
+  for (auto& e : ev) SWR_HIP(hipEventCreate(&e));
   struct EvGuard {
-    hipEvent_t a, b;
+    hipEvent_t* e;
     ~EvGuard() {
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
+      for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
     }
-  } evg{ev_start, ev_end};
-
-  SWR_HIP(hipEventRecord(ev_start, st));
-  uint32_t cur = 0, n_active = 0;
-  uint64_t next_sid = 0;
-  while (true) {
-    if (ctx->cancel.load()) fail(SWR_E_CANCELLED, "render cancelled");
-    const uint32_t k = (uint32_t)std::min<uint64_t>(cap - n_active, total - next_sid);
-    const uint32_t n = n_active + k;
-    if (n == 0) break;
-    SWR_HIP(hipEventRecord(clk.ev[0], st));
-    if (k)
-      raygen_kernel<<<blocks(k), 256, 0, st>>>(p, (uint32_t)next_sid, k, n_active, queue[cur], keys[cur], sxy,
-                                               radiance);
-    SWR_HIP(hipEventRecord(clk.ev[1], st));
-    uint32_t *rk = nullptr, *perm = nullptr;
-    sort_pairs(ctx, keys[cur], ws<uint32_t>(ctx, kPerm), ws<uint32_t>(ctx, kSortAltK), ws<uint32_t>(ctx, kSortAltV), n,
-               ray_bits, true, &rk, &perm, st);
-    SWR_HIP(hipEventRecord(clk.ev[2], st));
-    with_ring(ring, [&](auto kShort) {
-      trace_queue_kernel<kShort.value><<<blocks(n, kTraceBlock), kTraceBlock, 0, st>>>(
-          s.nodes, s.leaf_tris, perm, n, queue[cur], ws<uint32_t>(ctx, kHitKeys), hit_shift, spill, spill_stride);
-    });
-    SWR_HIP(hipEventRecord(clk.ev[3], st));
-    // The ray permutation is consumed; the hit sort reuses its buffers.
-    uint32_t *hk = nullptr, *hperm = nullptr;
-    uint32_t* spare = perm == ws<uint32_t>(ctx, kPerm) ? ws<uint32_t>(ctx, kSortAltV) : ws<uint32_t>(ctx, kPerm);
-    sort_pairs(ctx, ws<uint32_t>(ctx, kHitKeys), spare, keys[cur], perm, n, hit_bits, true, &hk, &hperm, st);
-    SWR_HIP(hipEventRecord(clk.ev[4], st));
-    zero_u32_kernel<<<1, 64, 0, st>>>(counters, 2);
-    const int nxt = cur ^ 1;
-    ShadeOut so;
-    so.next = queue[nxt];
-    so.next_keys = keys[nxt];
-    so.shadow_rays = ws<float4>(ctx, kShadowRays);
-    so.shadow_col = ws<float4>(ctx, kShadowCol);
-    so.radiance = radiance;
-    so.counters = reinterpret_cast<uint64_t*>(counters);
-    shade_kernel<<<blocks(n, kShadeBlock), kShadeBlock, 0, st>>>(s, p, hperm, n, queue[cur], so);
-    SWR_HIP(hipEventRecord(clk.ev[5], st));
-    with_ring(ring, [&](auto kShort) {
-      shadow_kernel<kShort.value><<<blocks(n, kTraceBlock), kTraceBlock, 0, st>>>(
-          s.nodes, s.leaf_tris, n, so.shadow_rays, so.shadow_col, radiance, spill, spill_stride);
-    });
-    SWR_HIP(hipEventRecord(clk.ev[6], st));
-    SWR_HIP(hipGetLastError());
-    uint32_t host_counts[4];
-    SWR_HIP(hipMemcpyAsync(host_counts, counters, 16, hipMemcpyDeviceToHost, st));
-    SWR_HIP(hipStreamSynchronize(st));
-    acc.ms_raygen += clk.ms(0, 1);
-    acc.ms_sort_rays += clk.ms(1, 2);
-    acc.ms_trace += clk.ms(2, 3);
-    acc.ms_sort_hits += clk.ms(3, 4);
-    acc.ms_shade += clk.ms(4, 5);
-    acc.ms_shadow += clk.ms(5, 6);
-    acc.camera_rays += k;
-    acc.closest_rays += n;
-    acc.shadow_rays += host_counts[1];
-    acc.waves += 1;
-    acc.max_wave = std::max(acc.max_wave, n);
-    n_active = host_counts[0];
-    next_sid += k;
-    cur ^= 1;
-  }
-  SWR_HIP(hipEventRecord(clk.ev[1], st));
-  dim3 fgrid(blocks(p.height, kFilterRows), blocks(p.width, kFilterCols));
-  filter_kernel<<<fgrid, kFilterBlock, 0, st>>>(p, radiance, sxy, ws<float>(ctx, kPixels));
-  SWR_HIP(hipGetLastError());
-  SWR_HIP(hipEventRecord(clk.ev[2], st));
-  SWR_HIP(hipEventRecord(ev_end, st));
-  if (pixels_host)
-    SWR_HIP(hipMemcpyAsync(pixels_host, ws<float>(ctx, kPixels), 12ull * p.width * p.height, hipMemcpyDeviceToHost,
-                           st));
-  if (samples_host) {
-    DeviceBuffer staging;
-    ensure(ctx, staging, 20ull * total);
-    pack_samples_kernel<<<blocks(total), 256, 0, st>>>(p, total, sxy, radiance, static_cast<float*>(staging.ptr));
-    SWR_HIP(hipMemcpyAsync(samples_host, staging.ptr, 20ull * total, hipMemcpyDeviceToHost, st));
-    SWR_HIP(hipStreamSynchronize(st));
-    release(ctx, staging);
-  }
+  } guard{ev};
+  StageTimes times;
+  SWR_HIP(hipEventRecord(ev[0], st));
+  loop.admit(0, p);
+  loop.run_paced(&times);
+  SWR_HIP(hipEventRecord(ev[1], st));
+  loop.filter(p);
+  SWR_HIP(hipEventRecord(ev[2], st));
+  loop.copy_out(p, pixels_host, samples_host);
   SWR_HIP(hipStreamSynchronize(st));
-  acc.ms_filter = clk.ms(1, 2);
-  float total_ms = 0.f;
-  SWR_HIP(hipEventElapsedTime(&total_ms, ev_start, ev_end));
-  acc.ms_total = total_ms;
-  if (stats) *stats = acc;
+  const LoopTotals t = loop.totals();
+  if (stats) {
+    swr_stats acc{};
+    acc.camera_rays = t.camera_rays;
+    acc.closest_rays = t.closest_rays;
+    acc.shadow_rays = t.shadow_rays;
+    acc.waves = (uint32_t)t.waves;
+    acc.max_wave = t.max_wave;
+    acc.ms_raygen = times.raygen;
+    acc.ms_sort_rays = times.sort_rays;
+    acc.ms_trace = times.trace;
+    acc.ms_sort_hits = times.sort_hits;
+    acc.ms_shade = times.shade;
+    acc.ms_shadow = times.shadow;
+    float f = 0.f, tot = 0.f;
+    SWR_HIP(hipEventElapsedTime(&f, ev[1], ev[2]));
+    SWR_HIP(hipEventElapsedTime(&tot, ev[0], ev[2]));
+    acc.ms_filter = f;
+    acc.ms_total = tot;
+    *stats = acc;
+  }
 }
 
 }  // namespace
@@ -992,3 +1216,4 @@ int swr_render(swr_context* ctx, const swr_scene* scene, const swr_render_desc* 
 }
 
 }  // extern "C"
+

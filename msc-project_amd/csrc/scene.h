@@ -117,6 +117,7 @@ struct swr_scene {
   float env[3] = {0.f, 0.f, 0.f};
   bool committed = false;
   int leaf_max = 4;
+  int tuned_ring = 0;  // traversal stack ring chosen for this scene by the batch autotuner (0 = none)
 
   // committed
   std::vector<swr::GeomRec> geoms;

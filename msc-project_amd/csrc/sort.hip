@@ -16,6 +16,8 @@
 // Look-back status words are 64-bit: [63:32] epoch tag of the pass, [31:30] flag
 // (1 aggregate, 2 inclusive prefix), [29:0] count. Words from older passes carry an older
 // epoch and read as "not ready", so the status buffer never needs clearing between passes.
+// The epoch counter and the element count can live in device memory, so that a captured
+// graph of the wavefront loop replays the sort with fresh tags and the current wave size.
 #include "swr_internal.h"
 
 namespace swr {
@@ -27,11 +29,16 @@ constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPre = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1;
 
-__global__ void sort_prepare_kernel(uint32_t* hist, uint32_t* counters) {
+// counters: [0..3] tile tickets of the passes; [kEpochWord] epoch of the last pass of the last
+// sort, on its own cache line: every tile reads it while the tickets take atomics, and a read
+// of a line under atomics waits behind them.
+constexpr int kEpochWord = 64;
+__global__ void sort_prepare_kernel(uint32_t* hist, uint32_t* counters, int passes) {
   const int t = threadIdx.x;
 #pragma unroll
   for (int p = 0; p < 4; ++p) hist[p * kRadix + t] = 0;
   if (t < 4) counters[t] = 0;
+  if (t == 0) counters[kEpochWord] += (uint32_t)passes;  // pass p uses that - passes + 1 + p
 }
 
 // Lanes holding the same 8-bit digit (a 64-lane match-any from 8 ballots), among active lanes.
@@ -50,9 +57,11 @@ __device__ inline uint64_t match_digit(uint32_t d, uint64_t active) {
 // in LDS; per key and digit position only the lowest lane of each group of equal digits adds
 // the group's size, so skewed digits (e.g. a few direction octants) do not serialise on
 // atomics.
-__global__ __launch_bounds__(256) void sort_histogram_kernel(const uint32_t* __restrict__ keys, size_t n, int passes,
+__global__ __launch_bounds__(256) void sort_histogram_kernel(const uint32_t* __restrict__ keys, size_t n_host,
+                                                             const uint32_t* __restrict__ n_dev, int passes,
                                                              uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_hist[kWaves][4 * kRadix];
+  const size_t n = n_dev ? (size_t)*n_dev : n_host;
   for (int i = threadIdx.x; i < kWaves * 4 * kRadix; i += blockDim.x) (&s_hist[0][0])[i] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -100,25 +109,35 @@ template <bool kIota, int kKpt>
 __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __restrict__ keys_in,
                                                               const uint32_t* __restrict__ vals_in,
                                                               uint32_t* __restrict__ keys_out,
-                                                              uint32_t* __restrict__ vals_out, uint32_t n, int shift,
+                                                              uint32_t* __restrict__ vals_out, uint32_t n_host,
+                                                              const uint32_t* __restrict__ n_dev, int shift,
                                                               const uint32_t* __restrict__ hist,
-                                                              uint32_t* __restrict__ counter,
-                                                              uint64_t* __restrict__ status, uint32_t epoch) {
+                                                              uint32_t* __restrict__ counters, int pass, int passes,
+                                                              uint64_t* __restrict__ status) {
   constexpr int kTile = kSortBlock * kKpt;
   __shared__ uint32_t s_buf[kTile];
   __shared__ uint32_t s_wave[kWaves][kRadix];
   __shared__ uint32_t s_start[kRadix];
   __shared__ uint32_t s_global[kRadix];
   __shared__ uint32_t s_tmp[kWaves];
-  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_tile, s_epoch;
 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (t == 0) s_tile = atomicAdd(counter, 1u);
+  if (t == 0) {
+    s_tile = atomicAdd(counters + pass, 1u);
+    s_epoch = counters[kEpochWord] - (uint32_t)passes + 1u + (uint32_t)pass;
+  }
   for (int i = t; i < kWaves * kRadix; i += kSortBlock) (&s_wave[0][0])[i] = 0;
+  const uint32_t n = n_dev ? *n_dev : n_host;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t epoch = s_epoch;
+  const uint32_t tile_base = tile * (uint32_t)kTile;
+  // Launches are sized for the largest wave; tickets past the current count have no keys, and
+  // no earlier tile waits on them.
+  if (tile_base >= n) return;
   // Global digit bases of this pass: exclusive scan of the histogram.
   const uint32_t digit_base = block_exclusive_scan(hist[t], s_tmp);
-  const uint32_t tile = s_tile;  // ordered by the barrier inside the scan
-  const uint32_t tile_base = tile * (uint32_t)kTile;
   const uint32_t valid = min((uint32_t)kTile, n - tile_base);
   const uint32_t wbase = (uint32_t)w * 64u * kKpt;
 
@@ -247,48 +266,68 @@ size_t sort_status_bytes(size_t n) {
 void sort_reserve(swr_context* ctx, size_t n) {
   SortWorkspace& ws = ctx->sort;
   ensure(ctx, ws.hist, 4 * kRadix * sizeof(uint32_t));
-  ensure(ctx, ws.counters, 4 * sizeof(uint32_t));
+  if (!ws.counters.ptr) {
+    ensure(ctx, ws.counters, 2 * kEpochWord * sizeof(uint32_t));
+    SWR_HIP(hipMemset(ws.counters.ptr, 0, ws.counters.bytes));
+  }
   const size_t need = sort_status_bytes(n);
   if (ws.status.bytes < need) {
     ensure(ctx, ws.status, need);
-    // Fresh memory may hold any bit pattern, including the current epoch tag.
-    SWR_HIP(hipMemset(ws.status.ptr, 0, ws.status.bytes));
-    ws.epoch = 0;
+    sort_reset_epochs(ctx, ctx->stream);
+    SWR_HIP(hipStreamSynchronize(ctx->stream));
   }
 }
 
+void sort_reset_epochs(swr_context* ctx, hipStream_t stream) {
+  // Fresh or recycled memory may hold any bit pattern, including a current epoch tag.
+  SortWorkspace& ws = ctx->sort;
+  SWR_HIP(hipMemsetAsync(ws.status.ptr, 0, ws.status.bytes, stream));
+  SWR_HIP(hipMemsetAsync(ws.counters.ptr, 0, ws.counters.bytes, stream));
+  ws.epoch = 0;
+}
+
+void sort_epoch_guard(swr_context* ctx, uint64_t passes_ahead, hipStream_t stream) {
+  if (ctx->sort.epoch + passes_ahead >= 0xFFFFFF00ull) sort_reset_epochs(ctx, stream);
+}
+
 void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals, size_t n,
-                int key_bits, bool iota, uint32_t** out_keys, uint32_t** out_vals, hipStream_t stream) {
+                int key_bits, bool iota, uint32_t** out_keys, uint32_t** out_vals, hipStream_t stream,
+                const uint32_t* n_dev) {
   if (key_bits < 1 || key_bits > 32) fail(SWR_E_INVALID, "sort: key_bits must be in 1..32");
   if (n > (size_t(1) << 30)) fail(SWR_E_INVALID, "sort: at most 2^30 elements");
-  *out_keys = keys;
-  *out_vals = vals;
-  if (n == 0) return;
-  sort_reserve(ctx, n);
+  const int passes = (key_bits + 7) / 8;
+  // The final buffers depend only on the pass count, so a captured sort returns them too.
+  *out_keys = (passes & 1) ? alt_keys : keys;
+  *out_vals = (passes & 1) ? alt_vals : vals;
+  if (n == 0) {
+    *out_keys = keys;
+    *out_vals = vals;
+    return;
+  }
   SortWorkspace& ws = ctx->sort;
+  if (!n_dev) {
+    sort_reserve(ctx, n);
+    sort_epoch_guard(ctx, (uint64_t)passes, stream);
+  }
+  ws.epoch += (uint64_t)passes;  // host mirror of the device epoch counter
   uint32_t* hist = static_cast<uint32_t*>(ws.hist.ptr);
   uint32_t* counters = static_cast<uint32_t*>(ws.counters.ptr);
   uint64_t* status = static_cast<uint64_t*>(ws.status.ptr);
-  const int passes = (key_bits + 7) / 8;
-  sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters);
+  sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters, passes);
   const int hist_blocks = (int)std::min<size_t>((n + 4095) / 4096, (size_t)std::max(ctx->num_cus, 1) * 8);
-  sort_histogram_kernel<<<hist_blocks, 256, 0, stream>>>(keys, n, passes, hist);
+  sort_histogram_kernel<<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
   const int kpt = ctx->tuning.sort_kpt;
   const uint32_t tiles = (uint32_t)((n + (size_t)kSortBlock * kpt - 1) / ((size_t)kSortBlock * kpt));
   uint32_t *sk = keys, *sv = vals, *dk = alt_keys, *dv = alt_vals;
   for (int p = 0; p < passes; ++p) {
-    if (++ws.epoch == 0) {  // tag space exhausted: start over from a cleared buffer
-      SWR_HIP(hipMemsetAsync(ws.status.ptr, 0, ws.status.bytes, stream));
-      ws.epoch = 1;
-    }
     const bool first_iota = iota && p == 0;
-#define SWR_ONESWEEP(KPT)                                                                                     \
-  if (first_iota)                                                                                             \
-    onesweep_kernel<true, KPT><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, 8 * p,          \
-                                                                 hist + p * kRadix, counters + p, status, ws.epoch); \
-  else                                                                                                        \
-    onesweep_kernel<false, KPT><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, 8 * p,         \
-                                                                  hist + p * kRadix, counters + p, status, ws.epoch);
+#define SWR_ONESWEEP(KPT)                                                                                      \
+  if (first_iota)                                                                                              \
+    onesweep_kernel<true, KPT><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev, 8 * p,    \
+                                                                 hist + p * kRadix, counters, p, passes, status); \
+  else                                                                                                         \
+    onesweep_kernel<false, KPT><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev, 8 * p,   \
+                                                                  hist + p * kRadix, counters, p, passes, status);
     switch (kpt) {
       case 8: SWR_ONESWEEP(8) break;
       case 12: SWR_ONESWEEP(12) break;
@@ -302,8 +341,6 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
     std::swap(sv, dv);
   }
   SWR_HIP(hipGetLastError());
-  *out_keys = sk;
-  *out_vals = sv;
 }
 
 }  // namespace swr

@@ -41,9 +41,9 @@ struct DeviceBuffer {
 struct SortWorkspace {
   DeviceBuffer alt_keys, alt_vals;   // ping-pong partners of the caller's buffers
   DeviceBuffer hist;                 // 4 passes x 256 digits
-  DeviceBuffer counters;             // 4 tile tickets
+  DeviceBuffer counters;             // 4 tile tickets and the device epoch counter
   DeviceBuffer status;               // look-back words, tiles x 256 x u64
-  uint32_t epoch = 0;                // look-back tag, bumped per pass
+  uint64_t epoch = 0;                // host mirror of the device epoch counter (tags issued so far)
 };
 
 }  // namespace swr
@@ -51,6 +51,8 @@ struct SortWorkspace {
 struct swr_scene;
 
 namespace swr {
+struct LoopBuffers;
+void destroy_loop_buffers(swr_context* ctx);
 // Launch shapes chosen per context (by swr_autotune or swr_set_tuning).
 struct Tuning {
   int sort_kpt = 16;       // keys per thread of a onesweep tile (tile = 256 * kpt)
@@ -70,7 +72,7 @@ struct swr_context {
   std::unordered_map<void*, size_t> allocs;
   swr::SortWorkspace sort;
   swr::Tuning tuning;
-  swr::DeviceBuffer render_ws[24];           // wavefront queues of the render loop (render.hip)
+  swr::LoopBuffers* loop = nullptr;          // wavefront buffers of the plain render (render.hip)
   swr::DeviceBuffer trace_spill;             // traversal stack spill of the stand-alone queries
   std::unordered_set<swr_scene*> scenes;     // live scenes; destroying the context detaches them
   int num_cus = 0;
@@ -108,8 +110,15 @@ void release(swr_context* ctx, DeviceBuffer& b);
 // Stable sort of n (key, value) pairs on key bits [0, key_bits). keys/vals hold the input; the
 // sorted output lands in whichever of (keys, vals) or (alt_keys, alt_vals) the last pass wrote,
 // returned through out_keys / out_vals. iota: values are 0..n-1 (vals is not read).
+// With n_dev, the element count is read on the device at run time and n is only its upper
+// bound (launch sizes); such a sort may be captured into a graph once sort_reserve(n) has run.
 void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals, size_t n,
-                int key_bits, bool iota, uint32_t** out_keys, uint32_t** out_vals, hipStream_t stream);
+                int key_bits, bool iota, uint32_t** out_keys, uint32_t** out_vals, hipStream_t stream,
+                const uint32_t* n_dev = nullptr);
+// The look-back tags come from a 32-bit device counter; before it could wrap (checked against
+// a host mirror), the status words are cleared. Graph replays call the guard between launches.
+void sort_epoch_guard(swr_context* ctx, uint64_t passes_ahead, hipStream_t stream);
+void sort_reset_epochs(swr_context* ctx, hipStream_t stream);
 // Scratch sizes the sort needs for n elements.
 size_t sort_status_bytes(size_t n);
 void sort_reserve(swr_context* ctx, size_t n);

@@ -1,0 +1,150 @@
+// wavefront.h -- the sorted wavefront loop (render.hip) as its three drivers use it: one plain
+// render with a host-paced loop, a batch of renders in one persistent loop replayed as HIP
+// graphs, and the serving loop that admits and retires jobs while the loop runs.
+//
+// A loop owns a fixed set of resident scenes, a table of job slots (one render job = one
+// Pathtracer::process() iteration of one camera), the ray queues of one wave and the look-up
+// tables the kernels read. Job state (per-sample radiance and film positions, pixels) lives in
+// arenas whose offsets the driver chooses, so that jobs of any size can come and go.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "integrator.h"
+#include "scene.h"
+
+namespace swr {
+
+// Device bookkeeping of a loop (one copy in device memory). The append counter that shading
+// hits with atomics has a cache line of its own: reads of a line under atomics wait behind them,
+// and every wave of every kernel reads the wave size.
+struct alignas(128) WaveState {
+  uint32_t n, k;          // rays in the wave in flight; camera rays among them (its last k slots)
+  uint32_t cap;           // queue capacity
+  uint32_t range_lo, num_ranges;  // live part of the job range table
+  uint32_t done_seq;      // every job with a smaller admission number has finished
+  uint32_t max_wave;
+  uint32_t pad0;
+  uint64_t raygen_base;   // global path index of the wave's first camera ray
+  uint64_t cursor;        // next global path index to generate
+  uint64_t total;         // global path indices admitted so far
+  uint64_t camera_rays, closest_rays, shadow_rays, waves;  // running totals
+  alignas(128) uint64_t counters;  // appends of the wave in flight: low word continuations, high word shadow rays
+};
+
+// Global path indices [path_begin, next range's path_begin) belong to job slot `job`.
+struct JobRange {
+  uint64_t path_begin;
+  uint32_t job;
+  uint32_t seq;
+};
+
+struct LoopTotals {
+  uint64_t camera_rays = 0, closest_rays = 0, shadow_rays = 0, waves = 0;
+  uint32_t max_wave = 0;
+};
+
+// Per-stage device times of a host-paced wave (events on the loop's stream).
+struct StageTimes {
+  double raygen = 0, sort_rays = 0, trace = 0, sort_hits = 0, shade = 0, shadow = 0;
+};
+
+// Device buffers of a loop, owned by its driver so that they can outlive one loop: the plain
+// render keeps one set per context and reuses it frame after frame.
+struct LoopBuffers {
+  DeviceBuffer queue[2], keys[2], hit_keys, perm, alt_k, alt_v, shadow, spill;
+  DeviceBuffer jobs, ranges, scene_tab, scene_ring, wave, block_min;
+  DeviceBuffer radiance, sample_xy, pixels, staging;  // arenas, when the driver keeps them here
+  WaveState* host_state = nullptr;                     // pinned snapshot slots
+  hipEvent_t snap_ev[8] = {};
+  hipEvent_t stage_ev[8] = {};
+  void release(swr_context* ctx);
+};
+
+class WaveLoop {
+ public:
+  // scenes: the resident scenes (at most 256); max_jobs: job slots (at most 65536);
+  // capacity: rays per wave (64 .. 2^24).
+  WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const swr_scene*>& scenes, uint32_t max_jobs,
+           uint32_t capacity);
+  ~WaveLoop();
+  WaveLoop(const WaveLoop&) = delete;
+  WaveLoop& operator=(const WaveLoop&) = delete;
+
+  // Samples and pixels of a job (arena sizes the driver must provide).
+  static uint64_t job_samples(const swr_render_desc& d);
+  static uint64_t job_pixels(const swr_render_desc& d);
+
+  // Validates desc and fills the frame constants of a job on scene `scene` (an index into the
+  // constructor's list) whose state starts at the given arena offsets.
+  FrameParams params(uint32_t scene, const swr_render_desc& desc, uint64_t sample_base, uint64_t pixel_base) const;
+
+  // Arenas shared by the jobs: per-sample radiance (16 B) and film position (8 B), pixels (12 B).
+  void set_arenas(float4* radiance, float2* sample_xy, float* pixels);
+
+  // Admits a job into `slot` (stream-ordered, so it is safe while waves are queued): its paths
+  // follow every path admitted before. Returns its admission number. The slot must be free: its
+  // previous job (if any) finished (snapshot done_seq above that job's number).
+  uint32_t admit(uint32_t slot, FrameParams p);
+  // Drops range-table entries of jobs whose paths were all generated before `cursor` (a value
+  // read from a snapshot), making room for more admissions.
+  void retire_ranges(uint64_t cursor);
+
+  // One wave: raygen refill, ray sort, trace, hit sort, shade, shadow. With `exact` the wave
+  // size is known on the host and launches are sized to it; otherwise they are sized for the
+  // capacity and read the size on the device (the form captured into graphs).
+  void enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times);
+
+  // Host-paced loop until every admitted path has finished; per-stage times if asked.
+  void run_paced(StageTimes* times);
+
+  // Persistent loop: captures the wave for both queue parities once, then replays graphs until
+  // every admitted path has finished (checked every `check_every` waves, two checks in flight).
+  void capture();
+  void run_graphs(int check_every = 4);
+  // Replays `waves` graph waves without waiting (serving loop); a snapshot then reads the state.
+  void launch_graph_waves(int waves);
+  // Asynchronously snapshots the device state into host memory; returns a ticket.
+  int snapshot();
+  // Waits for the snapshot with this ticket and returns it.
+  const WaveState& wait_snapshot(int ticket);
+
+  // Filters job `p` into its pixels (in the pixel arena).
+  void filter(const FrameParams& p);
+  // Copies the job's pixels and, if asked, its sample records (x, y, r, g, b in reference
+  // sample-id order) to the host (synchronises).
+  void copy_out(const FrameParams& p, float* pixels_host, float* samples_host);
+
+  LoopTotals totals();  // running totals from a fresh snapshot (synchronises)
+  hipStream_t stream() const { return stream_; }
+  uint32_t capacity() const { return cap_; }
+  int num_scenes() const { return (int)scenes_.size(); }
+  const swr_scene* scene(int i) const { return scenes_[i]; }
+  uint64_t paths_admitted() const { return total_; }
+
+ private:
+  swr_context* ctx_;
+  LoopBuffers& b_;
+  hipStream_t stream_;
+  std::vector<const swr_scene*> scenes_;
+  std::vector<int> rings_;        // ring size per scene
+  std::vector<int> ring_set_;     // distinct ring sizes
+  uint32_t max_jobs_, cap_;
+  int scene_bits_ = 0, hit_tri_bits_ = 0, ray_bits_ = 32, hit_bits_ = 0, hit_shift_ = 0;
+  uint32_t spill_stride_ = 0;
+  bool spill_ = false;
+  uint64_t total_ = 0;            // paths admitted
+  uint32_t seq_ = 0, num_ranges_ = 0, range_lo_ = 0, range_cap_ = 0;
+  float4* radiance_ = nullptr;
+  float2* sxy_ = nullptr;
+  float* pixels_ = nullptr;
+  hipGraphExec_t exec_[2] = {nullptr, nullptr};
+  hipGraph_t graph_[2] = {nullptr, nullptr};
+  int cur_ = 0;
+  int passes_per_wave_ = 0;
+  int snap_next_ = 0;
+  std::vector<uint64_t> range_begin_;  // host copy of the range table (path_begin per range)
+};
+
+}  // namespace swr

@@ -24,6 +24,7 @@ SWR_E_BUDGET = -3
 SWR_E_CANCELLED = -4
 SWR_E_IO = -5
 SWR_E_PARSE = -6
+SWR_E_BUSY = -7
 
 MASK_ALL = 0xFFFFFFFF
 MASK_SHADOW = 0x0FFFFFFF
@@ -120,6 +121,12 @@ _SIG = {
     "swr_raygen": (C.c_int, [_P, C.POINTER(RenderDesc), C.c_uint32, C.c_size_t, _P, _P]),
     "swr_render": (C.c_int, [_P, _P, C.POINTER(RenderDesc), _P, _P, C.POINTER(Stats)]),
     "swr_write_exr": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32]),
+    "swr_batch_create": (C.c_int, [_P, C.POINTER(_P)]),
+    "swr_batch_destroy": (None, [_P]),
+    "swr_batch_add": (C.c_int, [_P, _P, C.POINTER(RenderDesc), C.POINTER(C.c_int32)]),
+    "swr_batch_run": (C.c_int, [_P, C.c_uint32, C.POINTER(Stats)]),
+    "swr_batch_result": (C.c_int, [_P, C.c_int32, _P, _P]),
+    "swr_batch_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(Tuning)]),
 }
 
 _lib = None
@@ -414,6 +421,55 @@ class Scene:
         self.ctx.check(self.lib.swr_render(self.ctx.h, self.h, C.byref(desc), pix.ctypes.data,
                                            smp.ctypes.data if smp is not None else None, C.byref(st)))
         return pix, smp, st.as_dict()
+
+
+class Batch:
+    """Many render jobs traced together in one persistent wavefront loop (swr_batch_*)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = C.c_void_p()
+        ctx.check(self.lib.swr_batch_create(ctx.h, C.byref(h)))
+        self.h = h
+        self.descs = []
+
+    def add(self, scene: "Scene", desc: RenderDesc) -> int:
+        job = C.c_int32()
+        self.ctx.check(self.lib.swr_batch_add(self.h, scene.h, C.byref(desc), C.byref(job)))
+        self.descs.append(RenderDesc.from_buffer_copy(desc))
+        return job.value
+
+    def run(self, wave_rays: int = 0) -> dict:
+        st = Stats()
+        self.ctx.check(self.lib.swr_batch_run(self.h, wave_rays, C.byref(st)))
+        return st.as_dict()
+
+    def result(self, job: int, samples: bool = False):
+        if not 0 <= job < len(self.descs):  # let the library refuse it
+            self.ctx.check(self.lib.swr_batch_result(self.h, job, None, None))
+        d = self.descs[job]
+        spp = d.spp if d.spp > 0 else d.base * d.base
+        pix = np.zeros((d.height, d.width, 3), np.float32)
+        smp = np.zeros((d.width * d.height * spp, 5), np.float32) if samples else None
+        self.ctx.check(self.lib.swr_batch_result(self.h, job, pix.ctypes.data,
+                                                 smp.ctypes.data if smp is not None else None))
+        return pix, smp
+
+    def autotune(self, wave_rays: int = 0) -> dict:
+        t = Tuning()
+        self.ctx.check(self.lib.swr_batch_autotune(self.h, wave_rays, C.byref(t)))
+        return t.as_dict()
+
+    def close(self):
+        if self.h:
+            self.lib.swr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def raygen(ctx: Context, desc: RenderDesc, sid0: int, n: int):
