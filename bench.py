@@ -27,7 +27,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="terrain", choices=["terrain", "box_room"])
+    ap.add_argument("--workload", default="terrain", choices=["terrain", "box_room", "batched"],
+                    help="terrain: the plain render at scale (default); box_room: the CPU-runnable plain case; "
+                         "batched: only the batched-vs-plain comparison, batched Mrays/s as the value")
     ap.add_argument("--spp", type=int, default=None, help="override samples per pixel (must keep the workload name honest)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=0, help="sample ids the CPU baseline traces (0 = auto)")
@@ -37,6 +39,12 @@ def parse():
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes instead of the autotuner: keys per thread, ray key bits, hit key bits, "
                          "stack ring entries (e.g. 16,16,16,16)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
+    ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched workload")
+    ap.add_argument("--no-plain", action="store_true", help="batched workload: skip the job-by-job plain pass")
+    ap.add_argument("--batch-tuning", default=None,
+                    help="batched workload: fixed launch shapes (as --tuning, ring 0 = per scene by depth) "
+                         "instead of the batch autotuner")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain copy of known size (gather_kernel<4>, identity permutation, 2^26 "
                          "records: 1280 MiB read, 1024 MiB written) to scale rocprof FETCH_SIZE/WRITE_SIZE")
@@ -68,6 +76,18 @@ def main():
 
     import swr
     from swr import scenes
+
+    if args.workload == "batched":
+        ctx = swr.Context(0)
+        res = batched_vs_plain(ctx, args, plain=not args.no_plain)
+        ctx.close()
+        print(json.dumps({
+            "metric": "Mrays/s on 1 MI355X (batched persistent loop; closest-hit + shadow rays)",
+            "value": res["batched_mrays_s"], "unit": "Mrays/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": res["batched_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic (seeded procedural scenes, no files)",
+            "config": {"workload": "batched_256_jobs_8_scenes_d6", "method": "batched"}, "batched": res}), flush=True)
+        return
 
     t0 = time.time()
     if args.workload == "terrain":
@@ -205,10 +225,96 @@ def main():
         dk.free()
         dv.free()
 
+    if not args.no_batched:
+        scene.close()
+        out["batched"] = batched_vs_plain(ctx, args)
+
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(desc, rd, args)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def batched_vs_plain(ctx, args, plain=True):
+    """The batched workload (256 jobs over 8 resident scenes of 10k-2M triangles, pixel counts
+    log-uniform in [128^2, 1024^2], spp uniform in {1..64}, depth 6) traced twice: job by job
+    through the plain loop, and all together in one persistent graph-replayed loop. Both times
+    run from each loop's first raygen to its last resolved framebuffer (device events); images
+    of every 8th job are compared bit for bit."""
+    import numpy as np
+
+    import swr
+    from swr import scenes
+
+    t0 = time.time()
+    descs, jobs = scenes.batch_workload(jobs=args.batch_jobs)
+    gen_s = time.time() - t0
+    t0 = time.time()
+    scs = [swr.Scene(ctx, d) for d in descs]
+    build_s = time.time() - t0
+    rds = [swr.render_desc(r) for _, r in jobs]
+    saved = ctx.tuning()
+
+    b = swr.Batch(ctx)
+    for (s, _), rd in zip(jobs, rds):
+        b.add(scs[s], rd)
+    t0 = time.time()
+    if args.batch_tuning:
+        ctx.set_tuning(*[int(v) for v in args.batch_tuning.split(",")])
+        tuning = ctx.tuning()
+    else:
+        tuning = b.autotune()  # per-scene stack rings + sort tile and key widths, used by both passes
+    tune_s = time.time() - t0
+
+    # plain: one render per job through the host-paced loop, same launch shapes
+    plain_ms, plain_rays, plain_pix = 0.0, 0, {}
+    wall = time.perf_counter()
+    for j, ((s, _), rd) in enumerate(zip(jobs, rds)):
+        if not plain:
+            break
+        pix, _, st = scs[s].render(rd)
+        plain_ms += st["ms_total"]
+        plain_rays += st["closest_rays"] + st["shadow_rays"]
+        if j % 8 == 0:
+            plain_pix[j] = pix
+    plain_wall = time.perf_counter() - wall
+
+    paced = b.run(paced=True)  # the same batch launched wave by wave from the host: per-stage times
+    b.run()  # warm-up of the graph path
+    wall = time.perf_counter()
+    st = b.run()
+    batch_wall = time.perf_counter() - wall
+    batch_rays = st["closest_rays"] + st["shadow_rays"]
+    same = all(np.array_equal(b.result(j)[0], p) for j, p in plain_pix.items())
+    b.close()
+    ctx.set_tuning(**saved)
+    rings = [sc.stack_ring() for sc in scs]
+    for sc in scs:
+        sc.close()
+    plain_v = plain_rays / (plain_ms * 1e-3) / 1e6 if plain else None
+    batch_v = batch_rays / (st["ms_total"] * 1e-3) / 1e6
+    return {
+        "workload": f"{len(jobs)} jobs over 8 scenes (10k-2M tris), 128^2-1024^2 px, 1-64 spp, depth 6",
+        "paths": int(sum(rd.width * rd.height * (rd.spp or rd.base * rd.base) for rd in rds)),
+        "rays": int(batch_rays),
+        "plain_mrays_s": round(plain_v, 2) if plain else None,
+        "batched_mrays_s": round(batch_v, 2),
+        "speedup": round(batch_v / plain_v, 3) if plain else None,
+        "plain_ms": round(plain_ms, 2),
+        "batched_ms": round(st["ms_total"], 2),
+        "plain_wall_s": round(plain_wall, 3),
+        "batched_wall_s": round(batch_wall, 3),
+        "waves": {"batched": st["waves"]},
+        "paced_batched_mrays_s": round((paced["closest_rays"] + paced["shadow_rays"]) / (paced["ms_total"] * 1e-3) / 1e6,
+                                       2),
+        "paced_stage_ms": {k[3:]: round(paced[k], 2) for k in paced if k.startswith("ms_")},
+        "images_equal_sampled": bool(same and plain_rays == batch_rays) if plain else None,
+        "scene_rings": rings,
+        "batch_tuning": tuning,
+        "autotune_s": round(tune_s, 2),
+        "scene_gen_s": round(gen_s, 2),
+        "scene_upload_bvh_s": round(build_s, 2),
+    }
 
 
 def cpu_baseline(desc, rd, args):

@@ -209,6 +209,10 @@ int swr_intersect(swr_context* ctx, const swr_scene* scene, const float* rays_de
 /* rtcOccluded (Integrator.cpp:153-174): flags_dev[i] = 1 if any hit in (tmin, tmax). */
 int swr_occluded(swr_context* ctx, const swr_scene* scene, const float* rays_dev, size_t n, uint32_t mask,
                  uint32_t* flags_dev);
+/* Traversal stack ring of this scene (0 = sized from its BVH depth); swr_batch_autotune sets it.
+ * The context's trace_stack_entries, when nonzero, overrides it. Rings: 0, 8, 16, 32 or 64. */
+int swr_scene_set_stack_ring(swr_scene* scene, int32_t ring);
+int swr_scene_stack_ring(const swr_scene* scene, int32_t* ring);
 /* Global triangle id -> (geomID, primID). */
 int swr_scene_prim_to_geom(const swr_scene* scene, uint32_t prim, uint32_t* geom_id, uint32_t* prim_id);
 
@@ -236,8 +240,11 @@ void swr_batch_destroy(swr_batch* batch);
 /* Adds a job (at most 65536; scenes: at most 256 distinct). *job receives its index. */
 int swr_batch_add(swr_batch* batch, const swr_scene* scene, const swr_render_desc* desc, int32_t* job);
 /* Traces every job to completion with waves of wave_rays rays (0 = 2^23). stats: the whole
- * batch (ms_total = first raygen to the last resolved framebuffer; no per-stage times). */
-int swr_batch_run(swr_batch* batch, uint32_t wave_rays, swr_stats* stats);
+ * batch (ms_total = first raygen to the last resolved framebuffer). flags: 0, or
+ * SWR_BATCH_PACED to launch each wave from the host with sizes read back after the previous one
+ * (no graphs; fills the per-stage times; for profiling and comparison). */
+#define SWR_BATCH_PACED 1u
+int swr_batch_run(swr_batch* batch, uint32_t wave_rays, uint32_t flags, swr_stats* stats);
 /* A finished job's filtered pixels (W*H*3 floats) and optionally its samples (as swr_render). */
 int swr_batch_result(swr_batch* batch, int32_t job, float* pixels, float* samples);
 /* Chooses the traversal stack ring per scene (recorded on each scene, used by every later loop

@@ -41,7 +41,7 @@ uint32_t wave_capacity(uint32_t wave_rays, uint64_t total) {
 // runs the persistent loop; returns the device milliseconds from the first raygen to the last
 // resolved framebuffer.
 double run_jobs(swr_batch* b, const std::vector<swr_batch::Job>& jobs, uint32_t wave_rays, bool keep,
-                swr_stats* stats) {
+                swr_stats* stats, bool paced = false) {
   swr_context* ctx = b->ctx;
   SWR_HIP(hipSetDevice(ctx->device));
   uint64_t samples = 0, pixels = 0;
@@ -64,7 +64,7 @@ double run_jobs(swr_batch* b, const std::vector<swr_batch::Job>& jobs, uint32_t 
     sb += WaveLoop::job_samples(j.desc);
     pb += WaveLoop::job_pixels(j.desc);
   }
-  loop->capture();  // outside the timed region, like the plain render's allocations
+  if (!paced) loop->capture();  // outside the timed region, like the plain render's allocations
   hipStream_t st = loop->stream();
   hipEvent_t ev[2];
   for (auto& e : ev) SWR_HIP(hipEventCreate(&e));
@@ -77,7 +77,11 @@ double run_jobs(swr_batch* b, const std::vector<swr_batch::Job>& jobs, uint32_t 
   } guard{ev};
   SWR_HIP(hipEventRecord(ev[0], st));
   for (size_t i = 0; i < jobs.size(); ++i) params[i].seq = loop->admit((uint32_t)i, params[i]);
-  loop->run_graphs(4);
+  StageTimes times;
+  if (paced)
+    loop->run_paced(&times);
+  else
+    loop->run_graphs(4);
   for (const auto& p : params) loop->filter(p);
   SWR_HIP(hipEventRecord(ev[1], st));
   SWR_HIP(hipStreamSynchronize(st));
@@ -92,6 +96,12 @@ double run_jobs(swr_batch* b, const std::vector<swr_batch::Job>& jobs, uint32_t 
     s.waves = (uint32_t)t.waves;
     s.max_wave = t.max_wave;
     s.ms_total = ms;
+    s.ms_raygen = times.raygen;
+    s.ms_sort_rays = times.sort_rays;
+    s.ms_trace = times.trace;
+    s.ms_sort_hits = times.sort_hits;
+    s.ms_shade = times.shade;
+    s.ms_shadow = times.shadow;
     *stats = s;
   }
   if (keep) {
@@ -163,11 +173,12 @@ int swr_batch_add(swr_batch* b, const swr_scene* scene, const swr_render_desc* d
   });
 }
 
-int swr_batch_run(swr_batch* b, uint32_t wave_rays, swr_stats* stats) {
+int swr_batch_run(swr_batch* b, uint32_t wave_rays, uint32_t flags, swr_stats* stats) {
   return swr::guarded_batch(b, [&] {
     if (b->jobs.empty()) swr::fail(SWR_E_INVALID, "batch: no jobs");
+    if (flags & ~SWR_BATCH_PACED) swr::fail(SWR_E_INVALID, "batch: unknown flags");
     b->done = false;
-    swr::run_jobs(b, b->jobs, wave_rays, true, stats);
+    swr::run_jobs(b, b->jobs, wave_rays, true, stats, (flags & SWR_BATCH_PACED) != 0);
     b->done = true;
   });
 }

@@ -313,6 +313,18 @@ int swr_scene_bounds(const swr_scene* s, float out6[6]) {
   return SWR_OK;
 }
 
+int swr_scene_set_stack_ring(swr_scene* s, int32_t ring) {
+  if (!s || !(ring == 0 || ring == 8 || ring == 16 || ring == 32 || ring == 64)) return SWR_E_INVALID;
+  s->tuned_ring = ring;
+  return SWR_OK;
+}
+
+int swr_scene_stack_ring(const swr_scene* s, int32_t* ring) {
+  if (!s || !ring) return SWR_E_INVALID;
+  *ring = s->tuned_ring;
+  return SWR_OK;
+}
+
 int swr_scene_prim_to_geom(const swr_scene* s, uint32_t prim, uint32_t* geom_id, uint32_t* prim_id) {
   if (!s || !s->committed || !geom_id || !prim_id || prim >= s->dev.num_tris) return SWR_E_INVALID;
   auto it = std::upper_bound(s->geoms.begin(), s->geoms.end(), prim,

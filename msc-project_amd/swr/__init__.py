@@ -118,13 +118,15 @@ _SIG = {
     "swr_intersect": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_uint32, _P]),
     "swr_occluded": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_uint32, _P]),
     "swr_scene_prim_to_geom": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "swr_scene_set_stack_ring": (C.c_int, [_P, C.c_int32]),
+    "swr_scene_stack_ring": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "swr_raygen": (C.c_int, [_P, C.POINTER(RenderDesc), C.c_uint32, C.c_size_t, _P, _P]),
     "swr_render": (C.c_int, [_P, _P, C.POINTER(RenderDesc), _P, _P, C.POINTER(Stats)]),
     "swr_write_exr": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32]),
     "swr_batch_create": (C.c_int, [_P, C.POINTER(_P)]),
     "swr_batch_destroy": (None, [_P]),
     "swr_batch_add": (C.c_int, [_P, _P, C.POINTER(RenderDesc), C.POINTER(C.c_int32)]),
-    "swr_batch_run": (C.c_int, [_P, C.c_uint32, C.POINTER(Stats)]),
+    "swr_batch_run": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(Stats)]),
     "swr_batch_result": (C.c_int, [_P, C.c_int32, _P, _P]),
     "swr_batch_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(Tuning)]),
 }
@@ -352,6 +354,14 @@ class Scene:
     def commit(self):
         self._rc(self.lib.swr_scene_commit(self.h))
 
+    def stack_ring(self) -> int:
+        r = C.c_int32()
+        self.ctx.check(self.lib.swr_scene_stack_ring(self.h, C.byref(r)))
+        return r.value
+
+    def set_stack_ring(self, ring: int):
+        self.ctx.check(self.lib.swr_scene_set_stack_ring(self.h, ring))
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.swr_scene_destroy(self.h)
@@ -439,9 +449,9 @@ class Batch:
         self.descs.append(RenderDesc.from_buffer_copy(desc))
         return job.value
 
-    def run(self, wave_rays: int = 0) -> dict:
+    def run(self, wave_rays: int = 0, paced: bool = False) -> dict:
         st = Stats()
-        self.ctx.check(self.lib.swr_batch_run(self.h, wave_rays, C.byref(st)))
+        self.ctx.check(self.lib.swr_batch_run(self.h, wave_rays, 1 if paced else 0, C.byref(st)))
         return st.as_dict()
 
     def result(self, job: int, samples: bool = False):

@@ -201,6 +201,72 @@ def terrain(seed: int = 0x5EED0002, grid: int = 1024, groups: int = 64, width: i
             "render": render}
 
 
+def sphere_cluster(seed: int, count: int, level: int, width: int = 256, height: int = 256, base: int = 2,
+                   max_depth: int = 6, bin_exponent: int = 23):
+    """An icosphere cluster in the box room: `count` spheres of 20 * 4^level triangles with
+    seeded centres, radii and Lambert or glossy materials, under the room's ceiling light."""
+    rng = np.random.default_rng(seed)
+    room = box_room(seed=seed, width=width, height=height, base=base, max_depth=max_depth, sphere_level=0,
+                    bin_exponent=bin_exponent)
+    shaders = room["shaders"][:5]
+    meshes = [m for m in room["meshes"] if m["name"] != "sphere"]
+    unit_pos, idx, uv = icosphere(level, 1.0, (0.0, 0.0, 0.0))
+    for i in range(count):
+        r = float(rng.uniform(0.06, 0.25))
+        c = np.array([rng.uniform(-0.8, 0.8), rng.uniform(r, 1.5), rng.uniform(-0.8, 0.6)])
+        pos = unit_pos.copy()
+        pos[:, :3] = unit_pos[:, :3] * r + c
+        colour = rng.uniform(0.2, 0.8, 3).tolist()
+        if rng.random() < 0.5:
+            shaders.append({"type": SHADER_GLOSSY, "colour": colour, "reflectance": 1.0,
+                            "roughness": float(rng.uniform(0.05, 0.6))})
+        else:
+            shaders.append({"type": SHADER_LAMBERT, "colour": colour, "reflectance": 1.0, "roughness": 0.0})
+        meshes.append({"name": f"sphere_{i}", "pos4": pos, "idx": idx, "uv": uv, "shader": len(shaders) - 1})
+    return {"shaders": shaders, "meshes": meshes, "lights": room["lights"], "environment": (0.0, 0.0, 0.0),
+            "render": room["render"]}
+
+
+def batch_workload(seed: int = 0x5EED0004, jobs: int = 256, max_depth: int = 6):
+    """Batched workload: `jobs` render jobs over 8 resident scenes of 10k-2M triangles (icosphere
+    clusters and terrain variants). Each job: a seeded camera on a uniformly chosen scene, pixel
+    count log-uniform in [128^2, 1024^2] with aspect U[0.75, 1.33], spp uniform in {1..64}
+    (stratified when spp is a square, independent otherwise), max depth 6.
+    Returns (scene descriptions, [(scene index, render dict)])."""
+    descs = [
+        sphere_cluster(seed + 1, 2, 4),      # 10,250 triangles
+        sphere_cluster(seed + 2, 8, 4),      # 40,970
+        sphere_cluster(seed + 3, 6, 5),      # 122,890
+        sphere_cluster(seed + 4, 16, 5),     # 327,690
+        terrain(seed + 5, grid=256, groups=16),   # 131,074
+        terrain(seed + 6, grid=512, groups=64),   # 524,290
+        terrain(seed + 7, grid=768, groups=64),   # 1,179,650
+        terrain(seed + 8, grid=1024, groups=64),  # 2,097,154
+    ]
+    rng = np.random.default_rng(seed)
+    out = []
+    for j in range(jobs):
+        s = int(rng.integers(0, len(descs)))
+        side = math.exp(rng.uniform(math.log(128.0), math.log(1024.0)))
+        aspect = rng.uniform(0.75, 4.0 / 3.0)
+        w = max(1, int(round(side * math.sqrt(aspect))))
+        h = max(1, int(round(side / math.sqrt(aspect))))
+        spp = int(rng.integers(1, 65))
+        b = int(round(math.sqrt(spp)))
+        r = dict(descs[s]["render"])
+        if s < 4:
+            cam = camera_for([rng.uniform(-0.6, 0.6), rng.uniform(0.6, 1.4), rng.uniform(3.0, 3.4)],
+                             rng.uniform(80.0, 100.0), rng.uniform(-10.0, 5.0))
+        else:
+            cam = camera_for([rng.uniform(-0.5, 0.5), rng.uniform(0.6, 1.2), rng.uniform(1.5, 2.0)],
+                             rng.uniform(75.0, 105.0), rng.uniform(-35.0, -20.0))
+        r.update({"width": w, "height": h, "base": b, "spp": spp, "sampler": 0 if b * b == spp else 1,
+                  "max_depth": max_depth, "min_depth": 2, "seed": seed * 1000 + j, "iteration": 0,
+                  "camera": cam})
+        out.append((s, r))
+    return descs, out
+
+
 def random_soup(seed: int, ntris: int, extent: float = 1.0, size: float = 0.1):
     """Random triangles in a cube (intersection tests)."""
     rng = np.random.default_rng(seed)

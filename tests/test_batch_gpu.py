@@ -45,14 +45,14 @@ def mixed(ctx):
         sc.close()
 
 
-@pytest.mark.parametrize("wave_rays", [0, 4096, 777])
-def test_batch_equals_solo_renders(ctx, mixed, wave_rays):
+@pytest.mark.parametrize("wave_rays,paced", [(0, False), (4096, False), (777, False), (4096, True)])
+def test_batch_equals_solo_renders(ctx, mixed, wave_rays, paced):
     scs, jobs, solo = mixed
     b = swr.Batch(ctx)
     try:
         for s, rd in jobs:
             b.add(scs[s], rd)
-        st = b.run(wave_rays)
+        st = b.run(wave_rays, paced=paced)
         assert st["camera_rays"] == sum(rd.width * rd.height * (rd.spp or rd.base * rd.base) for _, rd in jobs)
         assert st["closest_rays"] == sum(x[2]["closest_rays"] for x in solo)
         assert st["shadow_rays"] == sum(x[2]["shadow_rays"] for x in solo)
