@@ -252,6 +252,49 @@ int swr_batch_result(swr_batch* batch, int32_t job, float* pixels, float* sample
  * copies of the batch's jobs (about 1/16 of their samples). */
 int swr_batch_autotune(swr_batch* batch, uint32_t wave_rays, swr_tuning* out);
 
+/* ---- serving loop (progressive renders on resident scenes, continuous batching) ------------
+ * The reference's viewer drives one Pathtracer, calling process() again and again and showing
+ * pixels / iteration (main.cpp:104-124, 158-173). A server keeps scenes resident and runs one
+ * persistent wavefront loop on its own thread; requests (a camera on a scene, M iterations)
+ * are admitted as job slots and device memory free up, and their waves merge with everything
+ * else in flight. A request's result is the mean of its M filtered iterations, accumulated in
+ * iteration order -- bit-identical to the same request served alone.
+ * While a server exists its thread owns the context: use only swr_server_* and swr_cancel. */
+typedef struct swr_server swr_server;
+typedef struct {
+  uint64_t state_bytes;      /* device memory for request state (per-sample arenas, 24 B per sample of one
+                                iteration, and FP32 framebuffers, 12 B per pixel); 0 = 8 GiB */
+  uint32_t wave_rays;        /* rays per wave (0 = 2^23) */
+  uint32_t max_requests;     /* requests on the device at once (0 = 256, at most 65536) */
+  uint32_t waves_per_check;  /* graph-replayed waves between host checks (0 = 2) */
+} swr_server_config;
+typedef struct {
+  int32_t status;            /* SWR_OK, or why the request failed */
+  uint32_t iterations;
+  double queue_ms;           /* submission to admission */
+  double latency_ms;         /* submission to the framebuffer in host memory */
+} swr_request_stats;
+typedef struct {
+  uint64_t submitted, completed, refused;
+  uint64_t camera_rays, closest_rays, shadow_rays, waves;
+  double busy_s;             /* wall time with at least one request on the device */
+  uint64_t state_bytes, queue_bytes;  /* request-state arenas; ray queues and tables */
+} swr_server_totals;
+/* scenes: the resident scenes (their order gives the scene index of a request). */
+int swr_server_create(swr_context* ctx, const swr_scene* const* scenes, int32_t num_scenes,
+                      const swr_server_config* config, swr_server** out);
+/* Queues a request: desc gives one iteration (image size, spp, sampler, filter, camera, depth,
+ * seed; iteration i uses desc->iteration + i as its RNG counter word) and iterations its count.
+ * pixels (host, W*H*3 floats) receives the result and must stay valid until swr_server_wait
+ * returns; it may be NULL. Thread-safe. SWR_E_BUDGET if the request can never fit. */
+int swr_server_submit(swr_server* server, int32_t scene, const swr_render_desc* desc, int32_t iterations,
+                      float* pixels, uint64_t* request);
+/* Blocks until the request has finished (or failed); thread-safe. */
+int swr_server_wait(swr_server* server, uint64_t request, swr_request_stats* stats);
+int swr_server_stats(swr_server* server, swr_server_totals* stats);
+/* Stops the thread; requests not finished fail with SWR_E_CANCELLED. */
+void swr_server_destroy(swr_server* server);
+
 /* ---- file output (main.cpp:197-208) ------------------------------------------------------ */
 /* Writes W*H*3 floats as an uncompressed scanline OpenEXR file of half RGBA with A = 1. */
 int swr_write_exr(const char* path, const float* rgb, int32_t width, int32_t height);

@@ -422,6 +422,7 @@ __global__ __launch_bounds__(1024) void wave_end_kernel(const uint32_t* __restri
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 0; i < 1024 / 64; ++i) m = min(m, s_min[i]);
+    w->live_min = m;
     if (w->cursor < w->total)
       m = min(m, ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, w->cursor)].seq);
     w->done_seq = m;
@@ -438,6 +439,19 @@ __global__ void admit_kernel(FrameParams* jobs, FrameParams p, JobRange* ranges,
 }
 
 __global__ void set_range_lo_kernel(WaveState* w, uint32_t range_lo) { w->range_lo = range_lo; }
+
+// Job completion as of now: admissions since the last wave moved the cursor's job.
+__global__ void done_kernel(const JobRange* __restrict__ ranges, uint32_t range_mask, WaveState* w) {
+  uint32_t m = w->live_min;
+  if (w->cursor < w->total)
+    m = min(m, ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, w->cursor)].seq);
+  w->done_seq = m;
+}
+
+__global__ void clear_kernel(float* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
 
 // Tent / box resolve (TentFilter.cpp:17-61, BoxFilter.cpp:17-36). A workgroup owns a tile of
 // kFilterCols columns x 64 rows of pixels, one lane per pixel (a wave per column). For each sample
@@ -458,7 +472,7 @@ constexpr int kHaloPerLane = (kHaloSlots + kFilterBlock - 1) / kFilterBlock;
 
 __global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, const float4* __restrict__ radiance,
                                                               const float2* __restrict__ sxy,
-                                                              float* __restrict__ pixels) {
+                                                              float* __restrict__ pixels, int accumulate) {
   __shared__ float4 s_rad[2][kHaloSlots];
   __shared__ float2 s_xy[2][kHaloSlots];
   const int W = (int)p.width, H = (int)p.height, spp = (int)p.spp;
@@ -539,17 +553,25 @@ __global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, con
   }
   if (px >= W || py >= H) return;
   float* o = pixels + 3 * (p.pixel_base + (size_t)py * W + px);
+  float v0, v1, v2;
   if (tent) {
-    o[0] = s0 / den;
-    o[1] = s1 / den;
-    o[2] = s2 / den;
+    v0 = s0 / den;
+    v1 = s1 / den;
+    v2 = s2 / den;
   } else {
-    o[0] = s0 / (float)spp;
-    o[1] = s1 / (float)spp;
-    o[2] = s2 / (float)spp;
+    v0 = s0 / (float)spp;
+    v1 = s1 / (float)spp;
+    v2 = s2 / (float)spp;
   }
+  if (accumulate) {
+    v0 = o[0] + v0;
+    v1 = o[1] + v1;
+    v2 = o[2] + v2;
+  }
+  o[0] = v0;
+  o[1] = v1;
+  o[2] = v2;
 }
-
 
 // Sample records (x, y, r, g, b) of one job in reference sample-id order (Image.h:10-14).
 __global__ void pack_samples_kernel(FrameParams p, size_t n, const float2* __restrict__ sxy,
@@ -718,8 +740,9 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
     if (std::find(ring_set_.begin(), ring_set_.end(), ring) == ring_set_.end()) ring_set_.push_back(ring);
     if ((uint32_t)ring < stack_need(scenes[i]->dev.max_depth)) spill_ = true;
   }
+  // Live ranges: jobs not fully generated, plus those a lagging retire_ranges still holds.
   range_cap_ = 1;
-  while (range_cap_ < 2 * max_jobs) range_cap_ <<= 1;
+  while (range_cap_ < 4 * max_jobs) range_cap_ <<= 1;
 
   ensure(ctx, b_.queue[0], sizeof(RayRec) * (size_t)cap_);
   ensure(ctx, b_.queue[1], sizeof(RayRec) * (size_t)cap_);
@@ -751,6 +774,7 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
   WaveState w{};
   w.cap = cap_;
   w.done_seq = 0;
+  w.live_min = 0xFFFFFFFFu;
   SWR_HIP(hipMemcpyAsync(b_.wave.ptr, &w, sizeof w, hipMemcpyHostToDevice, stream_));
   SWR_HIP(hipStreamSynchronize(stream_));
 }
@@ -971,6 +995,8 @@ void WaveLoop::launch_graph_waves(int waves) {
 int WaveLoop::snapshot() {
   const int t = snap_next_;
   snap_next_ = (snap_next_ + 1) & 7;
+  done_kernel<<<1, 1, 0, stream_>>>(static_cast<const JobRange*>(b_.ranges.ptr), range_cap_ - 1,
+                                    static_cast<WaveState*>(b_.wave.ptr));
   SWR_HIP(hipMemcpyAsync(b_.host_state + t, b_.wave.ptr, sizeof(WaveState), hipMemcpyDeviceToHost, stream_));
   SWR_HIP(hipEventRecord(b_.snap_ev[t], stream_));
   return t;
@@ -1001,9 +1027,15 @@ void WaveLoop::run_graphs(int check_every) {
   SWR_HIP(hipStreamSynchronize(stream_));
 }
 
-void WaveLoop::filter(const FrameParams& p) {
+void WaveLoop::filter(const FrameParams& p, bool accumulate) {
   dim3 fgrid(blocks(p.height, kFilterRows), blocks(p.width, kFilterCols));
-  filter_kernel<<<fgrid, kFilterBlock, 0, stream_>>>(p, radiance_, sxy_, pixels_);
+  filter_kernel<<<fgrid, kFilterBlock, 0, stream_>>>(p, radiance_, sxy_, pixels_, accumulate ? 1 : 0);
+  SWR_HIP(hipGetLastError());
+}
+
+void WaveLoop::clear_pixels(const FrameParams& p) {
+  const size_t n = 3ull * p.width * p.height;
+  clear_kernel<<<blocks(n), 256, 0, stream_>>>(pixels_ + 3 * p.pixel_base, n);
   SWR_HIP(hipGetLastError());
 }
 

@@ -25,7 +25,7 @@ struct alignas(128) WaveState {
   uint32_t range_lo, num_ranges;  // live part of the job range table
   uint32_t done_seq;      // every job with a smaller admission number has finished
   uint32_t max_wave;
-  uint32_t pad0;
+  uint32_t live_min;      // smallest admission number among the last wave's continuations
   uint64_t raygen_base;   // global path index of the wave's first camera ray
   uint64_t cursor;        // next global path index to generate
   uint64_t total;         // global path indices admitted so far
@@ -105,13 +105,17 @@ class WaveLoop {
   void run_graphs(int check_every = 4);
   // Replays `waves` graph waves without waiting (serving loop); a snapshot then reads the state.
   void launch_graph_waves(int waves);
-  // Asynchronously snapshots the device state into host memory; returns a ticket.
+  // Asynchronously snapshots the device state into host memory (done_seq recomputed against
+  // every admission enqueued before it); returns a ticket.
   int snapshot();
   // Waits for the snapshot with this ticket and returns it.
   const WaveState& wait_snapshot(int ticket);
 
-  // Filters job `p` into its pixels (in the pixel arena).
-  void filter(const FrameParams& p);
+  // Filters job `p` into its pixels (in the pixel arena); with accumulate, adds the filtered
+  // iteration to them (the reference's pixels += across iterations, TentFilter.cpp:57-60).
+  void filter(const FrameParams& p, bool accumulate = false);
+  // Zeroes job `p`'s pixels (stream-ordered).
+  void clear_pixels(const FrameParams& p);
   // Copies the job's pixels and, if asked, its sample records (x, y, r, g, b in reference
   // sample-id order) to the host (synchronises).
   void copy_out(const FrameParams& p, float* pixels_host, float* samples_host);
@@ -122,6 +126,8 @@ class WaveLoop {
   int num_scenes() const { return (int)scenes_.size(); }
   const swr_scene* scene(int i) const { return scenes_[i]; }
   uint64_t paths_admitted() const { return total_; }
+  uint32_t admissions() const { return seq_; }  // the next admission number
+  float* pixel_arena() const { return pixels_; }
 
  private:
   swr_context* ctx_;

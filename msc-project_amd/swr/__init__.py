@@ -68,6 +68,29 @@ class Tuning(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("state_bytes", C.c_uint64), ("wave_rays", C.c_uint32), ("max_requests", C.c_uint32),
+                ("waves_per_check", C.c_uint32)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_uint32), ("queue_ms", C.c_double),
+                ("latency_ms", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerTotals(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("refused", C.c_uint64),
+                ("camera_rays", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("waves", C.c_uint64), ("busy_s", C.c_double), ("state_bytes", C.c_uint64),
+                ("queue_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [("camera_rays", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("contributions", C.c_uint64), ("waves", C.c_uint32), ("max_wave", C.c_uint32),
@@ -129,6 +152,11 @@ _SIG = {
     "swr_batch_run": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(Stats)]),
     "swr_batch_result": (C.c_int, [_P, C.c_int32, _P, _P]),
     "swr_batch_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(Tuning)]),
+    "swr_server_create": (C.c_int, [_P, C.POINTER(_P), C.c_int32, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "swr_server_submit": (C.c_int, [_P, C.c_int32, C.POINTER(RenderDesc), C.c_int32, _P, C.POINTER(C.c_uint64)]),
+    "swr_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(RequestStats)]),
+    "swr_server_stats": (C.c_int, [_P, C.POINTER(ServerTotals)]),
+    "swr_server_destroy": (None, [_P]),
 }
 
 _lib = None
@@ -473,6 +501,53 @@ class Batch:
     def close(self):
         if self.h:
             self.lib.swr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over resident scenes (swr_server_*): submit progressive requests from any
+    thread, wait for each; results are the mean of the request's filtered iterations."""
+
+    def __init__(self, ctx: Context, scenes, state_bytes: int = 0, wave_rays: int = 0, max_requests: int = 0,
+                 waves_per_check: int = 0):
+        self.ctx, self.lib = ctx, ctx.lib
+        arr = (_P * len(scenes))(*[sc.h for sc in scenes])
+        cfg = ServerConfig(state_bytes, wave_rays, max_requests, waves_per_check)
+        h = _P()
+        ctx.check(self.lib.swr_server_create(ctx.h, arr, len(scenes), C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._out = {}
+
+    def submit(self, scene: int, desc: RenderDesc, iterations: int = 1, keep: bool = True) -> int:
+        pix = np.zeros((desc.height, desc.width, 3), np.float32) if keep else None
+        rid = C.c_uint64()
+        self.ctx.check(self.lib.swr_server_submit(self.h, scene, C.byref(desc), iterations,
+                                                  pix.ctypes.data if pix is not None else None, C.byref(rid)))
+        self._out[rid.value] = pix
+        return rid.value
+
+    def wait(self, request: int):
+        st = RequestStats()
+        rc = self.lib.swr_server_wait(self.h, request, C.byref(st))
+        pix = self._out.pop(request, None)
+        if rc != SWR_OK:
+            raise SwrError(rc, self.lib.swr_last_error(self.ctx.h).decode())
+        return pix, st.as_dict()
+
+    def totals(self) -> dict:
+        t = ServerTotals()
+        self.ctx.check(self.lib.swr_server_stats(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.swr_server_destroy(self.h)
             self.h = None
 
     def __del__(self):
