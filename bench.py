@@ -27,9 +27,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="terrain", choices=["terrain", "box_room", "batched"],
+    ap.add_argument("--workload", default="terrain", choices=["terrain", "box_room", "batched", "serving"],
                     help="terrain: the plain render at scale (default); box_room: the CPU-runnable plain case; "
-                         "batched: only the batched-vs-plain comparison, batched Mrays/s as the value")
+                         "batched: only the batched-vs-plain comparison, batched Mrays/s as the value; "
+                         "serving: only the serving run, its Mrays/s as the value")
     ap.add_argument("--spp", type=int, default=None, help="override samples per pixel (must keep the workload name honest)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=0, help="sample ids the CPU baseline traces (0 = auto)")
@@ -42,6 +43,19 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
     ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched workload")
     ap.add_argument("--no-plain", action="store_true", help="batched workload: skip the job-by-job plain pass")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="measured requests of the serving run")
+    ap.add_argument("--serve-warmup", type=int, default=50, help="requests before the measured ones")
+    ap.add_argument("--serve-rate", type=float, default=16.0,
+                    help="Poisson arrival rate (requests/s); 16/s loads the device to about 63%%")
+    ap.add_argument("--serve-backlog-waves", type=int, default=0,
+                    help="serving: admit while fewer paths than this many waves wait (0 = library default)")
+    ap.add_argument("--serve-waves-per-check", type=int, default=0,
+                    help="serving: graph waves between host checks (0 = library default)")
+    ap.add_argument("--serve-as-batch", type=int, default=0,
+                    help="serving workload: also trace the first N requests' iterations as one batch (capacity check)")
+    ap.add_argument("--serve-state-gib", type=float, default=32.0,
+                    help="device memory budget for request state (per-sample arenas + framebuffers)")
     ap.add_argument("--batch-tuning", default=None,
                     help="batched workload: fixed launch shapes (as --tuning, ring 0 = per scene by depth) "
                          "instead of the batch autotuner")
@@ -76,6 +90,18 @@ def main():
 
     import swr
     from swr import scenes
+
+    if args.workload == "serving":
+        ctx = swr.Context(0)
+        res = serving(ctx, args)
+        ctx.close()
+        print(json.dumps({
+            "metric": "Mrays/s on 1 MI355X (serving loop, Poisson arrivals); p50/p99 render latency",
+            "value": res["mrays_s"], "unit": "Mrays/s", "n_gpus": 1, "steps": 1, "warmup": args.serve_warmup,
+            "ms_per_step": res["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic (seeded procedural scenes, no files)",
+            "config": {"workload": "serving_4_scenes_poisson", "method": "served"}, "serving": res}), flush=True)
+        return
 
     if args.workload == "batched":
         ctx = swr.Context(0)
@@ -228,6 +254,8 @@ def main():
     if not args.no_batched:
         scene.close()
         out["batched"] = batched_vs_plain(ctx, args)
+    if not args.no_serving:
+        out["serving"] = serving(ctx, args)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(desc, rd, args)
@@ -314,6 +342,99 @@ def batched_vs_plain(ctx, args, plain=True):
         "autotune_s": round(tune_s, 2),
         "scene_gen_s": round(gen_s, 2),
         "scene_upload_bvh_s": round(build_s, 2),
+    }
+
+
+def serving(ctx, args):
+    """The serving loop under Poisson arrivals at a fixed rate: 4 resident scenes (61k-2.1M
+    triangles), requests with pixel counts log-uniform in [256^2, 1920x1080] and total spp
+    log-uniform in [4, 256] (iterations of up to 16 spp), depth 6. Latency runs from a request's
+    submission (its arrival) to its framebuffer in host memory; Mrays/s is every ray traced over
+    the wall time from the first arrival to the last delivery."""
+    import random
+    import threading
+
+    import swr
+    from swr import scenes
+
+    n_total = args.serve_warmup + args.serve_requests
+    t0 = time.time()
+    descs, reqs = scenes.serve_workload(requests=n_total)
+    gen_s = time.time() - t0
+    scs = [swr.Scene(ctx, d) for d in descs]
+    rds = [(s, swr.render_desc(r), it) for s, r, it in reqs]
+    as_batch = None
+    if args.serve_as_batch:
+        b = swr.Batch(ctx)
+        for s, rd, it in rds[:args.serve_as_batch]:
+            for i in range(it):
+                d = swr.RenderDesc.from_buffer_copy(rd)
+                d.iteration = i
+                b.add(scs[s], d)
+        b.run()
+        st = b.run()
+        as_batch = {"requests": args.serve_as_batch,
+                    "mrays_s": round((st["closest_rays"] + st["shadow_rays"]) / (st["ms_total"] * 1e-3) / 1e6, 2)}
+        b.close()
+    srv = swr.Server(ctx, scs, state_bytes=int(args.serve_state_gib * (1 << 30)), max_requests=256,
+                     backlog_waves=args.serve_backlog_waves, waves_per_check=args.serve_waves_per_check)
+    arrivals = []
+    rng = random.Random(0x5EED0006)
+    t = 0.0
+    for _ in range(n_total):
+        t += rng.expovariate(args.serve_rate)
+        arrivals.append(t)
+    ids = [None] * n_total
+    stats = [None] * n_total
+    ready = threading.Semaphore(0)
+
+    def waiter():
+        for k in range(n_total):
+            ready.acquire()
+            _, stats[k] = srv.wait(ids[k])
+
+    wt = threading.Thread(target=waiter)
+    wt.start()
+    start = time.perf_counter()
+    for k, (s, rd, it) in enumerate(rds):
+        delay = arrivals[k] - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        ids[k] = srv.submit(s, rd, it, keep=False)
+        ready.release()
+    wt.join()
+    wall = time.perf_counter() - start
+    tot = srv.totals()
+    srv.close()
+    for sc in scs:
+        sc.close()
+    lat = sorted(st["latency_ms"] for st in stats[args.serve_warmup:])
+    queue = sorted(st["queue_ms"] for st in stats[args.serve_warmup:])
+    pct = lambda v, q: v[min(len(v) - 1, int(q * len(v)))]
+    rays = tot["closest_rays"] + tot["shadow_rays"]
+    paths = sum(rd.width * rd.height * rd.base * rd.base * it for _, rd, it in rds)
+    return {
+        "workload": f"{args.serve_requests} requests (+{args.serve_warmup} warm-up) on 4 resident scenes "
+                    f"(61k-2.1M tris), 256^2-1920x1080 px, 4-256 spp, depth 6",
+        "rate_per_s": args.serve_rate,
+        "backlog_waves": args.serve_backlog_waves or 2,
+        "waves_per_check": args.serve_waves_per_check or 2,
+        "as_batch": as_batch,
+        "mrays_s": round(rays / wall / 1e6, 2),
+        "p50_ms": round(pct(lat, 0.50), 2),
+        "p99_ms": round(pct(lat, 0.99), 2),
+        "mean_ms": round(sum(lat) / len(lat), 2),
+        "queue_p50_ms": round(pct(queue, 0.50), 2),
+        "queue_p99_ms": round(pct(queue, 0.99), 2),
+        "completed": tot["completed"],
+        "rays": int(rays),
+        "paths": int(paths),
+        "wall_s": round(wall, 3),
+        "busy_s": round(tot["busy_s"], 3),
+        "achieved_rate_per_s": round(n_total / wall, 2),
+        "state_gib": round(tot["state_bytes"] / (1 << 30), 2),
+        "queue_gib": round(tot["queue_bytes"] / (1 << 30), 2),
+        "scene_gen_s": round(gen_s, 2),
     }
 
 

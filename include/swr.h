@@ -257,7 +257,9 @@ int swr_batch_autotune(swr_batch* batch, uint32_t wave_rays, swr_tuning* out);
  * pixels / iteration (main.cpp:104-124, 158-173). A server keeps scenes resident and runs one
  * persistent wavefront loop on its own thread; requests (a camera on a scene, M iterations)
  * are admitted as job slots and device memory free up, and their waves merge with everything
- * else in flight. A request's result is the mean of its M filtered iterations, accumulated in
+ * else in flight. Admission keeps a short backlog of ungenerated paths and fills it round
+ * robin, one iteration per admitted request per turn (a new request's first iteration at once),
+ * so short requests pass long ones and long ones are never starved. A request's result is the mean of its M filtered iterations, accumulated in
  * iteration order -- bit-identical to the same request served alone.
  * While a server exists its thread owns the context: use only swr_server_* and swr_cancel. */
 typedef struct swr_server swr_server;
@@ -265,8 +267,11 @@ typedef struct {
   uint64_t state_bytes;      /* device memory for request state (per-sample arenas, 24 B per sample of one
                                 iteration, and FP32 framebuffers, 12 B per pixel); 0 = 8 GiB */
   uint32_t wave_rays;        /* rays per wave (0 = 2^23) */
-  uint32_t max_requests;     /* requests on the device at once (0 = 256, at most 65536) */
+  uint32_t max_requests;     /* job slots: iterations on the device at once (0 = 256, at most 65536) */
   uint32_t waves_per_check;  /* graph-replayed waves between host checks (0 = 2) */
+  uint32_t iteration_window; /* iterations of one request in the loop at once (0 = 16); each holds its own
+                                per-sample arena range */
+  uint32_t backlog_waves;    /* admit while fewer paths than this many waves wait to be generated (0 = 2) */
 } swr_server_config;
 typedef struct {
   int32_t status;            /* SWR_OK, or why the request failed */

@@ -8,9 +8,11 @@
 //   admission  FIFO; a request waits for a free job slot and for ranges of the per-sample and
 //              framebuffer arenas (first fit, sized by the memory budget; no hipMalloc on the
 //              way), then its framebuffer is cleared and its first iteration joins the loop;
-//   iteration  when the loop reports the iteration's job finished, its samples are filtered and
-//              added to the framebuffer (iteration order, so the sum is the one a lone run
-//              makes), and the next iteration is admitted into the same slot and sample range;
+//   iterations up to `iteration_window` iterations of a request are in the loop at once, each a
+//              job with its own slot and sample range, topped up as resources free (newly
+//              arrived requests first); a finished iteration is filtered and added to the
+//              framebuffer strictly in iteration order (a later one that finishes first waits
+//              with its samples), so the sum is the one a lone run makes;
 //   delivery   after the last iteration the framebuffer is copied to pinned host memory; when
 //              that copy lands, the mean over iterations is written to the caller's buffer.
 // All device work is ordered on the loop's stream, so a slot or arena range can be handed to
@@ -36,6 +38,16 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// One iteration of a request in the loop: a job with its own slot and sample range.
+struct Iter {
+  int index = 0;
+  uint32_t slot = 0;
+  uint64_t sample_off = 0;
+  uint32_t seq = 0;  // admission number of its job
+  bool done = false;
+  FrameParams p{};
+};
+
 struct Request {
   uint64_t id = 0;
   int scene = 0;
@@ -46,11 +58,9 @@ struct Request {
   int status = SWR_OK;
   bool finished = false;
   // on the device (server thread only)
-  uint32_t slot = 0;
-  uint64_t sample_off = 0, sample_n = 0, pixel_off = 0, pixel_n = 0;
-  int admitted = 0;  // iterations admitted so far
-  FrameParams p{};
-  uint32_t seq = 0;  // admission number of the iteration in flight
+  uint64_t sample_n = 0, pixel_off = 0, pixel_n = 0;
+  int started = 0, accumulated = 0;
+  std::deque<Iter> iters;  // in flight or waiting to be accumulated, in iteration order
   float* pinned = nullptr;
   size_t pinned_bytes = 0;
   hipEvent_t ev = nullptr;
@@ -76,6 +86,10 @@ struct swr_server {
   std::unordered_map<uint64_t, std::unique_ptr<swr::Request>> reqs;
   std::deque<swr::Request*> pending;
   uint64_t next_id = 1;
+  uint64_t last_cursor = 0;  // paths generated, as of the latest snapshot
+  size_t rr = 0;             // round-robin position in `active`
+  bool busy = false;         // some request on the device since busy_since
+  std::chrono::steady_clock::time_point busy_since{};
   swr_server_totals stats{};
   // server thread only
   std::vector<uint32_t> free_slots;
@@ -113,48 +127,96 @@ hipEvent_t take_event(swr_server* s) {
   return e;
 }
 
-// Moves requests from the FIFO onto the device while a slot and arena ranges are free.
-void admit_pending(swr_server* s) {
-  while (true) {
-    Request* r = nullptr;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      if (s->pending.empty() || s->free_slots.empty()) return;
-      r = s->pending.front();
+// Starts iteration r->started of an admitted request if a slot and a sample range are free
+// (caller holds the lock for the allocation; the device calls run without it).
+bool start_iteration(swr_server* s, Request* r, std::unique_lock<std::mutex>& lk) {
+  if (s->free_slots.empty()) return false;
+  uint64_t off = 0;
+  if (!s->sample_space.alloc(r->sample_n, &off)) return false;
+  Iter it;
+  it.index = r->started;
+  it.slot = s->free_slots.back();
+  s->free_slots.pop_back();
+  it.sample_off = off;
+  lk.unlock();
+  it.p = s->loop->params((uint32_t)r->scene, r->desc, it.sample_off, r->pixel_off);
+  it.p.iteration = r->desc.iteration + (uint32_t)it.index;
+  it.seq = s->loop->admit(it.slot, it.p);
+  lk.lock();
+  r->started += 1;
+  r->iters.push_back(it);
+  return true;
+}
+
+// Admission. Paths are generated in admission order, so what is admitted is what runs next:
+// the server keeps the paths admitted but not yet generated (the backlog, as of the latest
+// snapshot) under `backlog_waves` waves and fills it round robin -- a newly arrived request
+// (first come, first served) gets its first iteration at once, then the admitted requests take
+// turns, one iteration each. Every admitted request advances one iteration per round, like
+// processor sharing at iteration granularity: short requests pass long ones, and long ones are
+// never starved by a stream of short ones.
+void admit(swr_server* s) {
+  const int window = (int)(s->cfg.iteration_window ? s->cfg.iteration_window : 16);
+  const uint64_t limit = (uint64_t)(s->cfg.backlog_waves ? s->cfg.backlog_waves : 2) * s->loop->capacity();
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (s->loop->paths_admitted() - s->last_cursor < limit && !s->free_slots.empty()) {
+    if (!s->pending.empty()) {
+      Request* r = s->pending.front();
       const uint64_t ns = WaveLoop::job_samples(r->desc), np = WaveLoop::job_pixels(r->desc);
-      if (!s->sample_space.fits(ns) || !s->pixel_space.fits(np)) return;  // first in, first served
-      s->pending.pop_front();
-      s->sample_space.alloc(ns, &r->sample_off);
-      s->pixel_space.alloc(np, &r->pixel_off);
-      r->sample_n = ns;
-      r->pixel_n = np;
-      r->slot = s->free_slots.back();
-      s->free_slots.pop_back();
-      r->t_admit = Clock::now();
+      if (s->sample_space.fits(ns) && s->pixel_space.fits(np)) {
+        s->pending.pop_front();
+        s->pixel_space.alloc(np, &r->pixel_off);
+        r->sample_n = ns;
+        r->pixel_n = np;
+        r->t_admit = Clock::now();
+        lk.unlock();
+        FrameParams p = s->loop->params((uint32_t)r->scene, r->desc, 0, r->pixel_off);
+        s->loop->clear_pixels(p);
+        lk.lock();
+        start_iteration(s, r, lk);  // fits: checked above under the same lock
+        s->active.push_back(r);
+        continue;
+      }
+      // The head waits for memory (first come, first served); admitted requests go on.
     }
-    r->p = s->loop->params((uint32_t)r->scene, r->desc, r->sample_off, r->pixel_off);
-    s->loop->clear_pixels(r->p);
-    r->seq = s->loop->admit(r->slot, r->p);
-    r->admitted = 1;
-    s->active.push_back(r);
+    bool started = false;
+    const size_t n = s->active.size();
+    for (size_t k = 0; k < n && !started; ++k) {
+      Request* r = s->active[(s->rr + k) % n];
+      if (r->started < r->iterations && (int)r->iters.size() < window && start_iteration(s, r, lk)) {
+        started = true;
+        s->rr = (s->rr + k + 1) % n;
+      }
+    }
+    if (!started) break;
   }
 }
 
-// Retires iterations finished as of a snapshot: filter-accumulates them, admits the next
-// iteration or starts the delivery copy.
+// Retires iterations finished as of a snapshot: frees their slots, filter-accumulates them in
+// iteration order (freeing their sample ranges), and starts the delivery copy of requests whose
+// every iteration is in the framebuffer.
 void retire(swr_server* s, const WaveState& w, uint32_t seq_limit) {
   const uint32_t done = std::min(w.done_seq, seq_limit);
   std::vector<Request*> still;
   for (Request* r : s->active) {
-    if (r->seq >= done) {
-      still.push_back(r);
-      continue;
+    for (Iter& it : r->iters) {
+      if (!it.done && it.seq < done) {
+        it.done = true;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->free_slots.push_back(it.slot);  // its job has finished; later jobs may take the slot
+      }
     }
-    s->loop->filter(r->p, true);
-    if (r->admitted < r->iterations) {
-      r->p.iteration = r->desc.iteration + (uint32_t)r->admitted;
-      r->seq = s->loop->admit(r->slot, r->p);
-      r->admitted += 1;
+    while (!r->iters.empty() && r->iters.front().done) {
+      s->loop->filter(r->iters.front().p, true);
+      {
+        // Stream order: a later job's raygen writes the range only after this filter read it.
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->sample_space.release(r->iters.front().sample_off, r->sample_n);
+      }
+      r->iters.pop_front();
+      r->accumulated += 1;
+    }
+    if (r->accumulated < r->iterations) {
       still.push_back(r);
       continue;
     }
@@ -164,11 +226,9 @@ void retire(swr_server* s, const WaveState& w, uint32_t seq_limit) {
                            s->loop->stream()));
     r->ev = take_event(s);
     SWR_HIP(hipEventRecord(r->ev, s->loop->stream()));
-    // Later work on the stream runs after the copy, so the slot and ranges are free now.
     {
+      // Later work on the stream runs after the copy, so the framebuffer range is free now.
       std::lock_guard<std::mutex> lk(s->mu);
-      s->free_slots.push_back(r->slot);
-      s->sample_space.release(r->sample_off, r->sample_n);
       s->pixel_space.release(r->pixel_off, r->pixel_n);
     }
     s->completing.push_back(r);
@@ -176,6 +236,7 @@ void retire(swr_server* s, const WaveState& w, uint32_t seq_limit) {
   s->active.swap(still);
   s->loop->retire_ranges(w.cursor);
   std::lock_guard<std::mutex> lk(s->mu);
+  s->last_cursor = w.cursor;
   s->stats.camera_rays = w.camera_rays;
   s->stats.closest_rays = w.closest_rays;
   s->stats.shadow_rays = w.shadow_rays + (w.counters >> 32);
@@ -239,8 +300,6 @@ void server_main(swr_server* s) {
     const int k = (int)(s->cfg.waves_per_check ? s->cfg.waves_per_check : 2);
     int ticket = -1;
     uint32_t ticket_limit = 0;
-    Clock::time_point busy_since{};
-    bool busy = false;
     while (true) {
       {
         std::unique_lock<std::mutex> lk(s->mu);
@@ -249,9 +308,13 @@ void server_main(swr_server* s) {
         });
         if (s->stop) break;
       }
-      admit_pending(s);
+      admit(s);
       const bool working = !s->active.empty();
-      if (working && !busy) busy_since = Clock::now(), busy = true;
+      if (working && !s->busy) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->busy_since = Clock::now();
+        s->busy = true;
+      }
       if (working) s->loop->launch_graph_waves(k);
       int t = -1;
       uint32_t limit = 0;
@@ -263,10 +326,10 @@ void server_main(swr_server* s) {
       ticket = t;
       ticket_limit = limit;
       deliver(s, s->active.empty() && ticket < 0);
-      if (busy && s->active.empty() && s->completing.empty()) {
+      if (s->busy && s->active.empty() && s->completing.empty()) {
         std::lock_guard<std::mutex> lk(s->mu);
-        s->stats.busy_s += std::chrono::duration<double>(Clock::now() - busy_since).count();
-        busy = false;
+        s->stats.busy_s += std::chrono::duration<double>(Clock::now() - s->busy_since).count();
+        s->busy = false;
       }
       if (!working && ticket >= 0) {
         // Nothing is tracing: settle the last snapshot now instead of spinning.
@@ -274,6 +337,11 @@ void server_main(swr_server* s) {
         ticket = -1;
         deliver(s, true);
       }
+    }
+    if (s->busy) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stats.busy_s += std::chrono::duration<double>(Clock::now() - s->busy_since).count();
+      s->busy = false;
     }
     SWR_HIP(hipStreamSynchronize(s->loop->stream()));
     fail_all(s, SWR_E_CANCELLED);
@@ -410,6 +478,7 @@ int swr_server_stats(swr_server* s, swr_server_totals* st) {
   if (!s || !st) return SWR_E_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);
   *st = s->stats;
+  if (s->busy) st->busy_s += std::chrono::duration<double>(swr::Clock::now() - s->busy_since).count();
   return SWR_OK;
 }
 

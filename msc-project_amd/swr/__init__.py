@@ -70,7 +70,7 @@ class Tuning(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("state_bytes", C.c_uint64), ("wave_rays", C.c_uint32), ("max_requests", C.c_uint32),
-                ("waves_per_check", C.c_uint32)]
+                ("waves_per_check", C.c_uint32), ("iteration_window", C.c_uint32), ("backlog_waves", C.c_uint32)]
 
 
 class RequestStats(C.Structure):
@@ -515,10 +515,10 @@ class Server:
     thread, wait for each; results are the mean of the request's filtered iterations."""
 
     def __init__(self, ctx: Context, scenes, state_bytes: int = 0, wave_rays: int = 0, max_requests: int = 0,
-                 waves_per_check: int = 0):
+                 waves_per_check: int = 0, iteration_window: int = 0, backlog_waves: int = 0):
         self.ctx, self.lib = ctx, ctx.lib
         arr = (_P * len(scenes))(*[sc.h for sc in scenes])
-        cfg = ServerConfig(state_bytes, wave_rays, max_requests, waves_per_check)
+        cfg = ServerConfig(state_bytes, wave_rays, max_requests, waves_per_check, iteration_window, backlog_waves)
         h = _P()
         ctx.check(self.lib.swr_server_create(ctx.h, arr, len(scenes), C.byref(cfg), C.byref(h)))
         self.h = h

@@ -267,6 +267,43 @@ def batch_workload(seed: int = 0x5EED0004, jobs: int = 256, max_depth: int = 6):
     return descs, out
 
 
+def serve_workload(seed: int = 0x5EED0005, requests: int = 1050, max_depth: int = 6):
+    """Serving workload: 4 resident scenes of 50k-2M triangles and a list of progressive
+    requests, each a seeded camera on a uniformly chosen scene with a pixel count log-uniform in
+    [256^2, 1920x1080] (aspect U[1, 16/9]) and a total spp log-uniform in [4, 256], rendered as
+    iterations of 16 spp (base 4; fewer when the total is below 16, base 2 or 3).
+    Returns (scene descriptions, [(scene index, render dict, iterations)])."""
+    descs = [
+        sphere_cluster(seed + 1, 3, 5),           # 61,452 triangles
+        terrain(seed + 2, grid=256, groups=16),   # 131,074
+        terrain(seed + 3, grid=512, groups=64),   # 524,290
+        terrain(seed + 4, grid=1024, groups=64),  # 2,097,154
+    ]
+    rng = np.random.default_rng(seed)
+    out = []
+    lo, hi = math.log(256.0 * 256.0), math.log(1920.0 * 1080.0)
+    for j in range(requests):
+        s = int(rng.integers(0, len(descs)))
+        px = math.exp(rng.uniform(lo, hi))
+        aspect = rng.uniform(1.0, 16.0 / 9.0)
+        w = max(1, int(round(math.sqrt(px * aspect))))
+        h = max(1, int(round(math.sqrt(px / aspect))))
+        spp = math.exp(rng.uniform(math.log(4.0), math.log(256.0)))
+        base = 4 if spp >= 16 else max(2, int(round(math.sqrt(spp))))
+        iterations = max(1, int(round(spp / (base * base))))
+        r = dict(descs[s]["render"])
+        if s == 0:
+            cam = camera_for([rng.uniform(-0.6, 0.6), rng.uniform(0.6, 1.4), rng.uniform(3.0, 3.4)],
+                             rng.uniform(80.0, 100.0), rng.uniform(-10.0, 5.0))
+        else:
+            cam = camera_for([rng.uniform(-0.5, 0.5), rng.uniform(0.6, 1.2), rng.uniform(1.5, 2.0)],
+                             rng.uniform(75.0, 105.0), rng.uniform(-35.0, -20.0))
+        r.update({"width": w, "height": h, "base": base, "spp": 0, "sampler": 0, "max_depth": max_depth,
+                  "min_depth": 2, "seed": seed * 1000 + j, "iteration": 0, "camera": cam})
+        out.append((s, r, iterations))
+    return descs, out
+
+
 def random_soup(seed: int, ntris: int, extent: float = 1.0, size: float = 0.1):
     """Random triangles in a cube (intersection tests)."""
     rng = np.random.default_rng(seed)
