@@ -450,7 +450,7 @@ def cpu_baseline(desc, rd, args):
     build_s = time.time() - t0
     od = swr.render_desc(desc["render"], orc.RenderDesc)
     total = rd.width * rd.height * (rd.spp or rd.base * rd.base)
-    n = args.cpu_samples or 2500000 * threads
+    n = args.cpu_samples or total  # the whole frame: a few seconds on 16 cores
     n = min(n, total)
     # a band of whole pixel columns in the middle of the frame (sample ids are column-major)
     start = (total // 2 // n) * n if n < total else 0
