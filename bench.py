@@ -35,7 +35,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=0, help="sample ids the CPU baseline traces (0 = auto)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--sort-bench", action="store_true", help="also time the 16M-key sort of the sort/traversal bench")
+    ap.add_argument("--no-kernels", action="store_true",
+                    help="skip the sort/traversal kernel bench (16M ray keys, 16M hit keys, 16M rays on the terrain BVH)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes instead of the autotuner: keys per thread, ray key bits, hit key bits, "
@@ -235,21 +236,8 @@ def main():
             out["roofline"]["traffic"] = round(per_ray * agg["closest_rays"] / agg["waves"], 1)
             out["roofline"]["traffic_per_ray"] = per_ray
 
-    if args.sort_bench:
-        n = 1 << 24
-        rng = np.random.default_rng(1)
-        keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
-        dk, dv = ctx.upload(keys), ctx.alloc(4 * n)
-        ms = []
-        for it in range(6):
-            dk.upload_from(keys)
-            ctx.sort_pairs(dk, dv, n, 32, iota=True)
-            ms.append(ctx.last_device_ms())
-        m = statistics.median(ms[1:])
-        out["sort16M"] = {"ms": round(m, 4), "GB/s": round(68.0 * n / (m * 1e-3) / 1e9, 1),
-                          "frac": round(68.0 * n / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        dk.free()
-        dv.free()
+    if not args.no_kernels:
+        out["kernels"] = kernels_bench(ctx, scene, desc, args)
 
     if not args.no_batched:
         scene.close()
@@ -261,6 +249,98 @@ def main():
         out["cpu_baseline"] = cpu_baseline(desc, rd, args)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def kernels_bench(ctx, scene, desc, args):
+    """Sort/traversal kernel bench. 2^24 rays with origins uniform in the unit cube and
+    directions uniform on the sphere, keyed by 3-bit octant + 29-bit Morton code and radix
+    sorted (key, index); then 2^24 hit records keyed ((mesh + 1) << 21 | face), mesh in [0, 64),
+    face in [0, 2^21), sorted by that key (28 bits). Both permutations are checked bit for bit
+    against a stable sort of the same keys. Then 2^24 rays placed at terrain scale (origins
+    uniform over x, z in [-1, 1], y in [0, 0.5]; directions uniform on the sphere) traced against
+    the terrain BVH for closest hits, and 2^24 shadow rays from the same origins to uniform points
+    of the sun quad. Device times from HIP events around each call; best of 5."""
+    import numpy as np
+
+    from swr import scenes
+
+    n = 1 << 24
+    res = {"n": n}
+    rays = scenes.sort_bench_rays(n)
+    d_rays = ctx.upload(rays)
+    d_keys = ctx.alloc(4 * n)
+    d_vals = ctx.alloc(4 * n)
+    ctx.ray_keys(d_rays, n, (0.0, 0.0, 0.0), (1024.0, 1024.0, 512.0), d_keys)
+    keys = d_keys.download(np.uint32, (n,))
+
+    def time_sort(host_keys, bits):
+        ms = []
+        for _ in range(6):
+            d_keys.upload_from(host_keys)
+            ctx.sort_pairs(d_keys, d_vals, n, bits, iota=True)
+            ms.append(ctx.last_device_ms())
+        perm = d_vals.download(np.uint32, (n,))
+        m = min(ms[1:])
+        gbs = 68.0 * n / (m * 1e-3) / 1e9
+        ok = bool(np.array_equal(perm, np.argsort(host_keys, kind="stable").astype(np.uint32)))
+        return {"ms": round(m, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "bits": bits, "bit_exact": ok}
+
+    res["ray_sort"] = time_sort(keys, 32)
+    rng = np.random.default_rng(0x5EED0007)
+    mesh = rng.integers(0, 64, n, dtype=np.uint32)
+    face = rng.integers(0, 1 << 21, n, dtype=np.uint32)
+    d_mesh, d_face = ctx.upload(mesh), ctx.upload(face)
+    ctx.mesh_face_keys(d_mesh, d_face, n, d_keys)
+    hkeys = d_keys.download(np.uint32, (n,))
+    res["hit_sort"] = time_sort(hkeys, 28)
+    res["hit_sort"]["keys_match"] = bool(np.array_equal(hkeys, ((mesh + 1) << 21) | face))
+    for a in (d_mesh, d_face, d_keys, d_vals):
+        a.free()
+
+    # traversal against the terrain BVH
+    t = np.zeros((n, 8), np.float32)
+    t[:, 0] = rays[:, 0] * 2.0 - 1.0
+    t[:, 1] = rays[:, 1] * 0.5
+    t[:, 2] = rays[:, 2] * 2.0 - 1.0
+    t[:, 3] = 0.001
+    t[:, 4:7] = rays[:, 4:7]
+    t[:, 7] = 1e5
+    d_rays.upload_from(t)
+    d_hits = ctx.alloc(16 * n)
+    ms = []
+    for _ in range(5):
+        scene.intersect_dev(d_rays, n, d_hits)
+        ms.append(ctx.last_device_ms())
+    hits = d_hits.download(np.uint32, (n, 4))
+    res["closest_hit"] = {"ms": round(min(ms), 4), "mrays_s": round(n / (min(ms) * 1e-3) / 1e6, 1),
+                          "hit_fraction": round(float(np.mean(hits[:, 3] != 0xFFFFFFFF)), 4)}
+    light = desc["lights"][0]
+    sx, sy = light["scale"]
+    u = rng.random((n, 2), dtype=np.float32) - 0.5
+    # points of the sun quad: translation + R (0, sx u, sy v); R = Rx Ry Rz of the light
+    ax, ay, az = [np.radians(v) for v in light["rotation"]]
+    rx = np.array([[1, 0, 0], [0, np.cos(ax), -np.sin(ax)], [0, np.sin(ax), np.cos(ax)]])
+    ry = np.array([[np.cos(ay), 0, np.sin(ay)], [0, 1, 0], [-np.sin(ay), 0, np.cos(ay)]])
+    rz = np.array([[np.cos(az), -np.sin(az), 0], [np.sin(az), np.cos(az), 0], [0, 0, 1]])
+    local = np.stack([np.zeros(n, np.float32), sx * u[:, 0], sy * u[:, 1]], 1)
+    pts = (local @ (rx @ ry @ rz).T.astype(np.float32)) + np.asarray(light["translation"], np.float32)
+    d = pts - t[:, 0:3]
+    dist = np.linalg.norm(d, axis=1)
+    t[:, 4:7] = d / dist[:, None]
+    t[:, 7] = dist
+    d_rays.upload_from(t)
+    d_flags = ctx.alloc(4 * n)
+    ms = []
+    for _ in range(5):
+        scene.occluded_dev(d_rays, n, d_flags)
+        ms.append(ctx.last_device_ms())
+    flags = d_flags.download(np.uint32, (n,))
+    res["shadow"] = {"ms": round(min(ms), 4), "mrays_s": round(n / (min(ms) * 1e-3) / 1e6, 1),
+                     "occluded_fraction": round(float(np.mean(flags != 0)), 4)}
+    for a in (d_rays, d_hits, d_flags):
+        a.free()
+    return res
 
 
 def batched_vs_plain(ctx, args, plain=True):

@@ -437,6 +437,13 @@ class Scene:
             d_r.free()
             d_h.free()
 
+    def intersect_dev(self, rays: "DeviceArray", n: int, hits: "DeviceArray", mask: int = MASK_ALL):
+        """Closest hits of n device rays into device hits (n x 4 words); no host copies."""
+        self.ctx.check(self.lib.swr_intersect(self.ctx.h, self.h, rays.ptr, n, mask, hits.ptr))
+
+    def occluded_dev(self, rays: "DeviceArray", n: int, flags: "DeviceArray", mask: int = MASK_SHADOW):
+        self.ctx.check(self.lib.swr_occluded(self.ctx.h, self.h, rays.ptr, n, mask, flags.ptr))
+
     def occluded(self, rays: np.ndarray, mask: int = MASK_SHADOW) -> np.ndarray:
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         n = rays.shape[0]
