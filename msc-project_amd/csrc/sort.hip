@@ -41,22 +41,25 @@ __global__ void sort_prepare_kernel(uint32_t* hist, uint32_t* counters, int pass
   if (t == 0) counters[kEpochWord] += (uint32_t)passes;  // pass p uses that - passes + 1 + p
 }
 
-// Lanes holding the same 8-bit digit (a 64-lane match-any from 8 ballots), among active lanes.
-__device__ inline uint64_t match_digit(uint32_t d, uint64_t active) {
-  uint64_t peers = active;
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bal = __ballot(bit);
-    peers &= bit ? bal : ~bal;
+// Adds key digits to this wave's LDS histogram: one LDS atomic per lane, or one per wave when
+// every active lane holds the same digit (skewed digits such as a few direction octants or a
+// mostly-zero top byte would otherwise serialise 64 atomics on one bank).
+__device__ inline void hist_add(uint32_t* h, uint32_t d, bool ok) {
+  const uint64_t active = __ballot(ok);
+  if (!active) return;
+  const int first = __ffsll((unsigned long long)active) - 1;
+  const uint32_t d0 = __shfl(d, first, 64);
+  if (__ballot(ok && d == d0) == active) {
+    if ((int)(threadIdx.x & 63) == first) atomicAdd(&h[d0], (uint32_t)__popcll(active));
+  } else if (ok) {
+    atomicAdd(&h[d], 1u);
   }
-  return peers;
 }
 
-// Histograms of all digit positions in one pass over the keys. Each wave owns a private copy
-// in LDS; per key and digit position only the lowest lane of each group of equal digits adds
-// the group's size, so skewed digits (e.g. a few direction octants) do not serialise on
-// atomics.
+// Histograms of all digit positions in one read of the keys: each thread reads 4 keys per
+// 16-byte load (grid-stride), each wave counts into its own LDS copy, and each workgroup adds its
+// counts to the global histogram once.
+template <bool kVec4>
 __global__ __launch_bounds__(256) void sort_histogram_kernel(const uint32_t* __restrict__ keys, size_t n_host,
                                                              const uint32_t* __restrict__ n_dev, int passes,
                                                              uint32_t* __restrict__ hist) {
@@ -64,18 +67,30 @@ __global__ __launch_bounds__(256) void sort_histogram_kernel(const uint32_t* __r
   const size_t n = n_dev ? (size_t)*n_dev : n_host;
   for (int i = threadIdx.x; i < kWaves * 4 * kRadix; i += blockDim.x) (&s_hist[0][0])[i] = 0;
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t* h = s_hist[threadIdx.x >> 6];
+  const size_t n4 = kVec4 ? n / 4 : 0;  // keys not 16-byte aligned: all go through the scalar loop
+  const uint4* k4 = reinterpret_cast<const uint4*>(keys);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t base = (size_t)blockIdx.x * blockDim.x + (size_t)w * 64; base < n; base += stride) {
-    const size_t i = base + lane;
+  // Whole waves iterate together (the bound is rounded up per wave) so the ballots see every lane.
+  const size_t first = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t wave_first = first - (threadIdx.x & 63);
+  for (size_t wi = wave_first, i = first; wi < n4; wi += stride, i += stride) {
+    const bool ok = i < n4;
+    const uint4 k = ok ? k4[i] : make_uint4(0u, 0u, 0u, 0u);
+    for (int p = 0; p < passes; ++p) {
+      const int sh = 8 * p;
+      uint32_t* hp = h + p * kRadix;
+      hist_add(hp, (k.x >> sh) & 255u, ok);
+      hist_add(hp, (k.y >> sh) & 255u, ok);
+      hist_add(hp, (k.z >> sh) & 255u, ok);
+      hist_add(hp, (k.w >> sh) & 255u, ok);
+    }
+  }
+  // The keys after the last full 16-byte group (all of them without vector loads).
+  for (size_t wi = 4 * n4 + first - (threadIdx.x & 63), i = 4 * n4 + first; wi < n; wi += stride, i += stride) {
     const bool ok = i < n;
     const uint32_t k = ok ? keys[i] : 0u;
-    const uint64_t active = __ballot(ok);
-    for (int p = 0; p < passes; ++p) {
-      const uint32_t d = (k >> (8 * p)) & 255u;
-      const uint64_t peers = match_digit(d, active);
-      if (ok && lane == __ffsll((unsigned long long)peers) - 1) s_hist[w][p * kRadix + d] += (uint32_t)__popcll(peers);
-    }
+    for (int p = 0; p < passes; ++p) hist_add(h + p * kRadix, (k >> (8 * p)) & 255u, ok);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x) {
@@ -314,8 +329,11 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
   uint32_t* counters = static_cast<uint32_t*>(ws.counters.ptr);
   uint64_t* status = static_cast<uint64_t*>(ws.status.ptr);
   sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters, passes);
-  const int hist_blocks = (int)std::min<size_t>((n + 4095) / 4096, (size_t)std::max(ctx->num_cus, 1) * 8);
-  sort_histogram_kernel<<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
+  const int hist_blocks = (int)std::max<size_t>(1, std::min<size_t>((n + 4095) / 4096, (size_t)std::max(ctx->num_cus, 1) * 4));
+  if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0)
+    sort_histogram_kernel<true><<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
+  else
+    sort_histogram_kernel<false><<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
   const int kpt = ctx->tuning.sort_kpt;
   const uint32_t tiles = (uint32_t)((n + (size_t)kSortBlock * kpt - 1) / ((size_t)kSortBlock * kpt));
   uint32_t *sk = keys, *sv = vals, *dk = alt_keys, *dv = alt_vals;

@@ -132,3 +132,27 @@ def test_sort_every_tile_size(ctx, kpt):
             assert np.array_equal(v, orc.stable_sort_perm(keys))
     finally:
         ctx.set_tuning(**before)
+
+
+@pytest.mark.parametrize("offset_words", [1, 2, 3])
+def test_sort_keys_not_16_byte_aligned(ctx, offset_words):
+    """Keys and values that start 4, 8 or 12 bytes into an allocation (the histogram then reads
+    keys one at a time instead of 16 bytes at a time)."""
+    import swr
+    n = 50001
+    rng = np.random.default_rng(offset_words)
+    keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    buf_k = ctx.alloc(4 * (n + 4))
+    buf_v = ctx.alloc(4 * (n + 4))
+    dk = swr.DeviceArray(ctx, buf_k.ptr + 4 * offset_words, 4 * n)
+    dv = swr.DeviceArray(ctx, buf_v.ptr + 4 * offset_words, 4 * n)
+    dk.upload_from(keys)
+    ctx.sort_pairs(dk, dv, n, 32, iota=True)
+    v = dv.download(np.uint32, (n,))
+    k = dk.download(np.uint32, (n,))
+    dk.ptr = dv.ptr = 0  # views: the allocations are freed below
+    buf_k.free()
+    buf_v.free()
+    perm = orc.stable_sort_perm(keys)
+    assert np.array_equal(v, perm)
+    assert np.array_equal(k, keys[perm])
