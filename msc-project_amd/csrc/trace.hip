@@ -14,17 +14,27 @@ __global__ __launch_bounds__(kTraceBlock) void trace_kernel(const Node4* __restr
                                                             uint32_t* __restrict__ keys, uint32_t* __restrict__ flags,
                                                             uint32_t* __restrict__ spill, uint32_t spill_stride) {
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
-  const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-  if (i >= n) return;
-  const RayState r = make_ray(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]);
-  const HitResult h = traverse<kTraceBlock, kShort, kAny>(nodes, leaf_tris, r, mask, s_ring + threadIdx.x,
-                                                          spill + (size_t)i * spill_stride);
-  if (kAny) {
-    flags[i] = h.id != kMiss ? 1u : 0u;
-  } else {
-    hits[i] = make_uint4(__float_as_uint(h.t), __float_as_uint(h.u), __float_as_uint(h.v), h.id);
-    if (keys) keys[i] = h.id + 1u;
-  }
+  const uint32_t waves = gridDim.x * (kTraceBlock / 64);
+  const uint32_t chunk = chunk_rays(n, waves);
+  const uint32_t wave = blockIdx.x * (kTraceBlock / 64) + threadIdx.x / 64;
+  const uint32_t begin = wave * chunk;
+  if (begin >= n) return;
+  const uint32_t end = min(n, begin + chunk);
+  trace_chunk<kTraceBlock, kShort, kAny>(
+      begin, end, mask, s_ring + threadIdx.x, spill + ((size_t)blockIdx.x * kTraceBlock + threadIdx.x) * spill_stride,
+      [&](uint32_t i, auto& wk) {
+        wk.start(make_ray(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]), nodes, leaf_tris);
+        return true;
+      },
+      [&](uint32_t i, auto& wk) {
+        if (kAny) {
+          flags[i] = wk.best.id != kMiss ? 1u : 0u;
+        } else {
+          hits[i] = make_uint4(__float_as_uint(wk.best.t), __float_as_uint(wk.best.u), __float_as_uint(wk.best.v),
+                               wk.best.id);
+          if (keys) keys[i] = wk.best.id + 1u;
+        }
+      });
 }
 
 template <bool kAny>
@@ -33,14 +43,16 @@ void launch_trace(swr_context* ctx, const DeviceScene& s, const float4* rays, ui
   if (!n) return;
   const int ring = choose_ring(ctx->tuning.trace_stack, s.max_depth);
   const uint32_t need = stack_need(s.max_depth);
+  const unsigned tb = trace_blocks(refill_waves(n));
   uint32_t* spill = nullptr;
   if ((uint32_t)ring < need) {
-    ensure(ctx, ctx->trace_spill, (size_t)n * need * sizeof(uint32_t));
+    // one spill column per traversal lane
+    ensure(ctx, ctx->trace_spill, (size_t)tb * kTraceBlock * need * sizeof(uint32_t));
     spill = static_cast<uint32_t*>(ctx->trace_spill.ptr);
   }
   with_ring(ring, [&](auto kShort) {
-    trace_kernel<kShort.value, kAny><<<(n + kTraceBlock - 1) / kTraceBlock, kTraceBlock, 0, stream>>>(
-        s.nodes, s.leaf_tris, rays, n, mask, hits, keys, flags, spill, need);
+    trace_kernel<kShort.value, kAny><<<tb, kTraceBlock, 0, stream>>>(s.nodes, s.leaf_tris, rays, n, mask, hits, keys,
+                                                                      flags, spill, need);
   });
   SWR_HIP(hipGetLastError());
 }
