@@ -166,8 +166,9 @@ def main():
     stage_ms = {k[3:]: round(agg[k] / args.steps, 3) for k in agg if k.startswith("ms_")}
     dominant = max(("trace", "shadow", "sort_rays", "sort_hits", "shade", "raygen", "accumulate", "filter"),
                    key=lambda k: stage_ms.get(k, 0.0))
-    # Ray sort: 68 B per element for a 4-pass u32 key + u32 index radix sort.
-    sort_gbs = 68.0 * agg["closest_rays"] / (agg["ms_sort_rays"] * 1e-3) / 1e9
+    # In-loop ray sort: key width as tuned, so 4 B (histogram read) + 16 B per 8-bit pass per element.
+    ray_passes = (tuning["ray_key_bits"] + 7) // 8
+    sort_gbs = (4.0 + 16.0 * ray_passes) * agg["closest_rays"] / (agg["ms_sort_rays"] * 1e-3) / 1e9
 
     out = {
         "metric": "Mrays/s on 1 MI355X (plain sorted-wavefront render; closest-hit + shadow rays)",
@@ -201,7 +202,8 @@ def main():
             "note": "algorithmic 48 B/ray (ray in + hit out); traversal is latency/L2-bound, node traffic excluded",
         },
         "sort_roofline": {
-            "kernel": "ray sort (onesweep, 4 passes)",
+            "kernel": f"in-loop ray sort ({ray_passes} passes of 8 bits, "
+                      f"{['onesweep', 'reduce-then-scan'][tuning['sort_algorithm']]})",
             "bound": "hbm",
             "achieved": round(sort_gbs, 2),
             "peak": HBM_PEAK_GBS,
@@ -286,15 +288,22 @@ def kernels_bench(ctx, scene, desc, args):
         return {"ms": round(m, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
                 "bits": bits, "bit_exact": ok}
 
-    res["ray_sort"] = time_sort(keys, 32)
-    rng = np.random.default_rng(0x5EED0007)
-    mesh = rng.integers(0, 64, n, dtype=np.uint32)
-    face = rng.integers(0, 1 << 21, n, dtype=np.uint32)
-    d_mesh, d_face = ctx.upload(mesh), ctx.upload(face)
-    ctx.mesh_face_keys(d_mesh, d_face, n, d_keys)
-    hkeys = d_keys.download(np.uint32, (n,))
+    # the sort's launch shapes tuned for this size: both pass structures at every tile size
+    saved = ctx.tuning()
+    sweep = {}
+    for alg in (0, 1):
+        for kpt in (8, 12, 16, 20, 24):
+            ctx.set_tuning(**dict(saved, sort_algorithm=alg, sort_keys_per_thread=kpt))
+            sweep[(alg, kpt)] = time_sort(keys, 32)
+    best = min(sweep, key=lambda c: sweep[c]["ms"])
+    ctx.set_tuning(**dict(saved, sort_algorithm=best[0], sort_keys_per_thread=best[1]))
+    res["ray_sort"] = dict(sweep[best], algorithm=["onesweep", "reduce-then-scan"][best[0]],
+                           keys_per_thread=best[1],
+                           sweep_ms={f"{['onesweep', 'scan'][a]}/{k}": v["ms"] for (a, k), v in sweep.items()})
+    del keys
     res["hit_sort"] = time_sort(hkeys, 28)
     res["hit_sort"]["keys_match"] = bool(np.array_equal(hkeys, ((mesh + 1) << 21) | face))
+    ctx.set_tuning(**saved)
     for a in (d_mesh, d_face, d_keys, d_vals):
         a.free()
 

@@ -108,6 +108,12 @@ typedef struct {
   double ms_raygen, ms_sort_rays, ms_trace, ms_sort_hits, ms_shade, ms_shadow, ms_accumulate, ms_filter;
 } swr_stats;
 
+/* Pass structures of the radix sort: onesweep (one kernel per digit, decoupled look-back over
+ * earlier tiles) or reduce-then-scan (per digit: tile counts, a scan over tiles, the scatter;
+ * no tile waits for another). */
+#define SWR_SORT_ONESWEEP 0
+#define SWR_SORT_REDUCE_SCAN 1
+
 /* Launch shapes of the loop. The reference exposes the equivalent knobs as settings
  * (bucket size, shading size, bin exponent; Settings.h:16-24); these are chosen per context by
  * swr_autotune or set directly. */
@@ -118,6 +124,7 @@ typedef struct {
   int32_t trace_stack_entries;   /* LDS ring of the traversal stack per ray: 8, 16, 32 or 64 entries, deeper
                                     entries spill to device memory (0 = per scene: the ring
                                     swr_batch_autotune chose for it, else sized from its BVH depth) */
+  int32_t sort_algorithm;        /* SWR_SORT_ONESWEEP or SWR_SORT_REDUCE_SCAN */
 } swr_tuning;
 
 /* ---- context (Pathtracer ctor/dtor, Pathtracer.cpp:369-392) ------------------------------ */

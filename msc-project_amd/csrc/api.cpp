@@ -149,6 +149,7 @@ int swr_get_tuning(const swr_context* ctx, swr_tuning* out) {
   out->ray_key_bits = ctx->tuning.ray_key_bits;
   out->hit_key_bits = ctx->tuning.hit_key_bits;
   out->trace_stack_entries = ctx->tuning.trace_stack;
+  out->sort_algorithm = ctx->tuning.sort_algorithm;
   return SWR_OK;
 }
 
@@ -158,16 +159,18 @@ int swr_set_tuning(swr_context* ctx, const swr_tuning* t) {
   if (!(k == 8 || k == 12 || k == 16 || k == 20 || k == 24) || t->ray_key_bits < 3 || t->ray_key_bits > 32 ||
       t->hit_key_bits < 0 || t->hit_key_bits > 32 ||
       !(t->trace_stack_entries == 0 || t->trace_stack_entries == 8 || t->trace_stack_entries == 16 ||
-        t->trace_stack_entries == 32 || t->trace_stack_entries == 64)) {
+        t->trace_stack_entries == 32 || t->trace_stack_entries == 64) ||
+      !(t->sort_algorithm == SWR_SORT_ONESWEEP || t->sort_algorithm == SWR_SORT_REDUCE_SCAN)) {
     ctx->err =
         "set_tuning: keys per thread must be 8/12/16/20/24, ray key bits 3..32, hit key bits 0..32, "
-        "trace stack entries 0/8/16/32/64";
+        "trace stack entries 0/8/16/32/64, sort algorithm 0/1";
     return SWR_E_INVALID;
   }
   ctx->tuning.sort_kpt = k;
   ctx->tuning.ray_key_bits = t->ray_key_bits;
   ctx->tuning.hit_key_bits = t->hit_key_bits;
   ctx->tuning.trace_stack = t->trace_stack_entries;
+  ctx->tuning.sort_algorithm = t->sort_algorithm;
   return SWR_OK;
 }
 

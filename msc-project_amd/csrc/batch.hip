@@ -240,11 +240,13 @@ int swr_batch_autotune(swr_batch* b, uint32_t wave_rays, swr_tuning* out) {
         if (ms < best_ms) best_ms = ms, best = t;
         ctx->tuning = best;
       };
-      for (int kpt : {8, 12, 16, 20, 24}) {
-        swr::Tuning t = best;
-        t.sort_kpt = kpt;
-        trial(t);
-      }
+      for (int alg : {SWR_SORT_ONESWEEP, SWR_SORT_REDUCE_SCAN})
+        for (int kpt : {8, 12, 16, 20, 24}) {
+          swr::Tuning t = best;
+          t.sort_algorithm = alg;
+          t.sort_kpt = kpt;
+          trial(t);
+        }
       for (int rb : {32, 24, 20, 16}) {
         swr::Tuning t = best;
         t.ray_key_bits = rb;

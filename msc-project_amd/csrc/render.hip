@@ -1192,12 +1192,14 @@ int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc
     swr::Tuning best = ctx->tuning;
     swr::render_frame(ctx, scene, d, nullptr, nullptr, nullptr);  // allocations outside the timings
     double best_ms = time_it(best);
-    for (int kpt : {8, 12, 16, 20, 24}) {
-      swr::Tuning t = best;
-      t.sort_kpt = kpt;
-      const double ms = time_it(t);
-      if (ms < best_ms) best_ms = ms, best = t;
-    }
+    for (int alg : {SWR_SORT_ONESWEEP, SWR_SORT_REDUCE_SCAN})
+      for (int kpt : {8, 12, 16, 20, 24}) {
+        swr::Tuning t = best;
+        t.sort_algorithm = alg;
+        t.sort_kpt = kpt;
+        const double ms = time_it(t);
+        if (ms < best_ms) best_ms = ms, best = t;
+      }
     for (int rb : {32, 24, 20, 16}) {
       swr::Tuning t = best;
       t.ray_key_bits = rb;

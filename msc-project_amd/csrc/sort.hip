@@ -120,7 +120,64 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp) {
   return pre + x - v;
 }
 
-template <bool kIota, int kKpt>
+// Upsweep of a scanned pass: every tile's digit counts, digit-major (counts[d * stride + tile]),
+// for the same tiles the pass kernel will take.
+template <int kKpt>
+__global__ __launch_bounds__(kSortBlock) void upsweep_kernel(const uint32_t* __restrict__ keys, uint32_t n_host,
+                                                             const uint32_t* __restrict__ n_dev, int shift,
+                                                             uint32_t* __restrict__ counts, uint32_t stride) {
+  constexpr int kTile = kSortBlock * kKpt;
+  __shared__ uint32_t s_h[kWaves][kRadix];
+  const uint32_t n = n_dev ? *n_dev : n_host;
+  const uint32_t tile = blockIdx.x, base = tile * (uint32_t)kTile;
+  if (base >= n) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (int i = t; i < kWaves * kRadix; i += kSortBlock) (&s_h[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t wbase = base + (uint32_t)w * 64u * kKpt;
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    const bool ok = p < n;
+    const uint32_t k = ok ? keys[p] : 0u;
+    hist_add(s_h[w], (k >> shift) & 255u, ok);
+  }
+  __syncthreads();
+  uint32_t c = 0;
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) c += s_h[ww][t];
+  counts[(size_t)t * stride + tile] = c;
+}
+
+// Exclusive prefix over the tiles of one digit's counts (in place; workgroup = digit) and the
+// digit's total.
+__global__ __launch_bounds__(kSortBlock) void tile_scan_kernel(uint32_t* __restrict__ counts, uint32_t stride,
+                                                               uint32_t n_host, const uint32_t* __restrict__ n_dev,
+                                                               uint32_t tile_keys, uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_tmp[kWaves];
+  const uint32_t n = n_dev ? *n_dev : n_host;
+  const uint32_t tiles = (n + tile_keys - 1) / tile_keys;
+  uint32_t* row = counts + (size_t)blockIdx.x * stride;
+  const uint32_t per = (tiles + kSortBlock - 1) / kSortBlock;
+  const uint32_t b = threadIdx.x * per, e = min(tiles, b + per);
+  uint32_t sum = 0;
+  for (uint32_t i = b; i < e; ++i) sum += row[i];
+  const uint32_t pre = block_exclusive_scan(sum, s_tmp);
+  uint32_t run = pre;
+  for (uint32_t i = b; i < e; ++i) {
+    const uint32_t c = row[i];
+    row[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == kSortBlock - 1) totals[blockIdx.x] = pre + sum;
+}
+
+// One digit pass over a tile of 256 * kKpt keys: stable rank inside the tile, then scatter
+// keys and values to their global positions. The tile's global offset per digit comes either from
+// a decoupled look-back over earlier tiles (onesweep: tiles ticketed in launch order, statuses
+// tagged by epoch) or, with kScanned, from the per-tile digit prefixes of an upsweep + scan
+// (tile = workgroup index, no cross-tile waiting).
+template <bool kIota, int kKpt, bool kScanned>
 __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __restrict__ keys_in,
                                                               const uint32_t* __restrict__ vals_in,
                                                               uint32_t* __restrict__ keys_out,
@@ -128,7 +185,9 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
                                                               const uint32_t* __restrict__ n_dev, int shift,
                                                               const uint32_t* __restrict__ hist,
                                                               uint32_t* __restrict__ counters, int pass, int passes,
-                                                              uint64_t* __restrict__ status) {
+                                                              uint64_t* __restrict__ status,
+                                                              const uint32_t* __restrict__ tile_prefix,
+                                                              uint32_t tile_stride) {
   constexpr int kTile = kSortBlock * kKpt;
   __shared__ uint32_t s_buf[kTile];
   __shared__ uint32_t s_wave[kWaves][kRadix];
@@ -139,8 +198,13 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (t == 0) {
-    s_tile = atomicAdd(counters + pass, 1u);
-    s_epoch = counters[kEpochWord] - (uint32_t)passes + 1u + (uint32_t)pass;
+    if (kScanned) {
+      s_tile = blockIdx.x;
+      s_epoch = 0;
+    } else {
+      s_tile = atomicAdd(counters + pass, 1u);
+      s_epoch = counters[kEpochWord] - (uint32_t)passes + 1u + (uint32_t)pass;
+    }
   }
   for (int i = t; i < kWaves * kRadix; i += kSortBlock) (&s_wave[0][0])[i] = 0;
   const uint32_t n = n_dev ? *n_dev : n_host;
@@ -206,14 +270,17 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
   }
   const uint64_t tag = (uint64_t)epoch << 32;
   uint64_t* my_status = status + (size_t)tile * kRadix + t;
-  __hip_atomic_store(my_status, tag | (uint64_t)((tile == 0 ? kFlagPre : kFlagAgg) | total), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  if (!kScanned)
+    __hip_atomic_store(my_status, tag | (uint64_t)((tile == 0 ? kFlagPre : kFlagAgg) | total), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t start = block_exclusive_scan(total, s_tmp);
   s_start[t] = start;
 
-  // Decoupled look-back over earlier tiles for digit t.
   uint32_t excl = 0;
-  if (tile > 0) {
+  if (kScanned) {
+    excl = tile_prefix[(size_t)t * tile_stride + tile];
+  } else if (tile > 0) {
+    // Decoupled look-back over earlier tiles for digit t.
     int pred = (int)tile - 1;
     for (;;) {
       const uint64_t v = __hip_atomic_load(status + (size_t)pred * kRadix + t, __ATOMIC_RELAXED,
@@ -291,6 +358,12 @@ void sort_reserve(swr_context* ctx, size_t n) {
     sort_reset_epochs(ctx, ctx->stream);
     SWR_HIP(hipStreamSynchronize(ctx->stream));
   }
+  // Per-tile digit counts of the scanned passes: 256 rows of the tile count at the smallest tile.
+  const size_t tiles = (n + (size_t)kSortBlock * kSortMinKeysPerThread - 1) / ((size_t)kSortBlock * kSortMinKeysPerThread);
+  if (tiles > ws.tile_stride) {
+    ensure(ctx, ws.tile_counts, (size_t)kRadix * tiles * sizeof(uint32_t));
+    ws.tile_stride = (uint32_t)tiles;
+  }
 }
 
 void sort_reset_epochs(swr_context* ctx, hipStream_t stream) {
@@ -328,33 +401,53 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
   uint32_t* hist = static_cast<uint32_t*>(ws.hist.ptr);
   uint32_t* counters = static_cast<uint32_t*>(ws.counters.ptr);
   uint64_t* status = static_cast<uint64_t*>(ws.status.ptr);
-  sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters, passes);
-  const int hist_blocks = (int)std::max<size_t>(1, std::min<size_t>((n + 4095) / 4096, (size_t)std::max(ctx->num_cus, 1) * 4));
-  if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0)
-    sort_histogram_kernel<true><<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
-  else
-    sort_histogram_kernel<false><<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
+  uint32_t* tile_counts = static_cast<uint32_t*>(ws.tile_counts.ptr);
+  const bool scanned = ctx->tuning.sort_algorithm == SWR_SORT_REDUCE_SCAN;
   const int kpt = ctx->tuning.sort_kpt;
-  const uint32_t tiles = (uint32_t)((n + (size_t)kSortBlock * kpt - 1) / ((size_t)kSortBlock * kpt));
+  const uint32_t tile_keys = (uint32_t)(kSortBlock * kpt);
+  const uint32_t tiles = (uint32_t)((n + tile_keys - 1) / tile_keys);
+  if (!scanned) {
+    sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters, passes);
+    const int hist_blocks =
+        (int)std::max<size_t>(1, std::min<size_t>((n + 4095) / 4096, (size_t)std::max(ctx->num_cus, 1) * 4));
+    if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0)
+      sort_histogram_kernel<true><<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
+    else
+      sort_histogram_kernel<false><<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
+  }
   uint32_t *sk = keys, *sv = vals, *dk = alt_keys, *dv = alt_vals;
   for (int p = 0; p < passes; ++p) {
     const bool first_iota = iota && p == 0;
-#define SWR_ONESWEEP(KPT)                                                                                      \
-  if (first_iota)                                                                                              \
-    onesweep_kernel<true, KPT><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev, 8 * p,    \
-                                                                 hist + p * kRadix, counters, p, passes, status); \
-  else                                                                                                         \
-    onesweep_kernel<false, KPT><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev, 8 * p,   \
-                                                                  hist + p * kRadix, counters, p, passes, status);
+    // Onesweep: the pass kernel alone. Reduce-then-scan: upsweep, per-digit scan (the digit
+    // totals land in hist), then the pass kernel with the scanned per-tile offsets.
+#define SWR_PASS(KPT)                                                                                           \
+  if (scanned) {                                                                                                \
+    upsweep_kernel<KPT><<<tiles, kSortBlock, 0, stream>>>(sk, (uint32_t)n, n_dev, 8 * p, tile_counts,           \
+                                                          ws.tile_stride);                                      \
+    tile_scan_kernel<<<kRadix, kSortBlock, 0, stream>>>(tile_counts, ws.tile_stride, (uint32_t)n, n_dev,          \
+                                                        tile_keys, hist + p * kRadix);                          \
+    if (first_iota)                                                                                             \
+      onesweep_kernel<true, KPT, true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,     \
+          8 * p, hist + p * kRadix, counters, p, passes, status, tile_counts, ws.tile_stride);                   \
+    else                                                                                                        \
+      onesweep_kernel<false, KPT, true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,    \
+          8 * p, hist + p * kRadix, counters, p, passes, status, tile_counts, ws.tile_stride);                   \
+  } else if (first_iota) {                                                                                      \
+    onesweep_kernel<true, KPT, false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,      \
+        8 * p, hist + p * kRadix, counters, p, passes, status, tile_counts, ws.tile_stride);                     \
+  } else {                                                                                                      \
+    onesweep_kernel<false, KPT, false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,     \
+        8 * p, hist + p * kRadix, counters, p, passes, status, tile_counts, ws.tile_stride);                     \
+  }
     switch (kpt) {
-      case 8: SWR_ONESWEEP(8) break;
-      case 12: SWR_ONESWEEP(12) break;
-      case 16: SWR_ONESWEEP(16) break;
-      case 20: SWR_ONESWEEP(20) break;
-      case 24: SWR_ONESWEEP(24) break;
+      case 8: SWR_PASS(8) break;
+      case 12: SWR_PASS(12) break;
+      case 16: SWR_PASS(16) break;
+      case 20: SWR_PASS(20) break;
+      case 24: SWR_PASS(24) break;
       default: fail(SWR_E_INVALID, "sort: keys per thread must be 8, 12, 16, 20 or 24");
     }
-#undef SWR_ONESWEEP
+#undef SWR_PASS
     std::swap(sk, dk);
     std::swap(sv, dv);
   }

@@ -43,6 +43,8 @@ struct SortWorkspace {
   DeviceBuffer hist;                 // 4 passes x 256 digits
   DeviceBuffer counters;             // 4 tile tickets and the device epoch counter
   DeviceBuffer status;               // look-back words, tiles x 256 x u64
+  DeviceBuffer tile_counts;          // per-tile digit counts of the reduce-then-scan passes, 256 x tile_stride
+  uint32_t tile_stride = 0;
   uint64_t epoch = 0;                // host mirror of the device epoch counter (tags issued so far)
 };
 
@@ -59,6 +61,7 @@ struct Tuning {
   int ray_key_bits = 32;   // top bits of the octant + Morton ray key that the ray sort orders by
   int hit_key_bits = 0;    // top bits of the hit key sorted (0 = all: full (geomID, primID) order)
   int trace_stack = 0;     // LDS ring entries of the traversal stack (0 = from the BVH depth)
+  int sort_algorithm = SWR_SORT_ONESWEEP;  // pass structure of the radix sort
 };
 }  // namespace swr
 

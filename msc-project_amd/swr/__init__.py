@@ -25,6 +25,7 @@ SWR_E_CANCELLED = -4
 SWR_E_IO = -5
 SWR_E_PARSE = -6
 SWR_E_BUSY = -7
+SORT_ONESWEEP, SORT_REDUCE_SCAN = 0, 1
 
 MASK_ALL = 0xFFFFFFFF
 MASK_SHADOW = 0x0FFFFFFF
@@ -62,7 +63,7 @@ class QuadLightDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sort_keys_per_thread", C.c_int32), ("ray_key_bits", C.c_int32), ("hit_key_bits", C.c_int32),
-                ("trace_stack_entries", C.c_int32)]
+                ("trace_stack_entries", C.c_int32), ("sort_algorithm", C.c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -256,8 +257,9 @@ class Context:
         self.check(self.lib.swr_get_tuning(self.h, C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, sort_keys_per_thread=16, ray_key_bits=32, hit_key_bits=0, trace_stack_entries=0):
-        t = Tuning(sort_keys_per_thread, ray_key_bits, hit_key_bits, trace_stack_entries)
+    def set_tuning(self, sort_keys_per_thread=16, ray_key_bits=32, hit_key_bits=0, trace_stack_entries=0,
+                   sort_algorithm=0):
+        t = Tuning(sort_keys_per_thread, ray_key_bits, hit_key_bits, trace_stack_entries, sort_algorithm)
         self.check(self.lib.swr_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, scene: "Scene", desc: "RenderDesc") -> dict:

@@ -162,8 +162,8 @@ def test_tuning_changes_speed_not_pixels(ctx):
     before = ctx.tuning()
     ref = sc.render(rd)[0]
     try:
-        for kpt, rb, hb, ring in [(8, 32, 0, 8), (12, 24, 0, 16), (20, 16, 8, 64), (24, 3, 4, 32)]:
-            ctx.set_tuning(kpt, rb, hb, ring)
+        for kpt, rb, hb, ring, alg in [(8, 32, 0, 8, 0), (12, 24, 0, 16, 1), (20, 16, 8, 64, 1), (24, 3, 4, 32, 0)]:
+            ctx.set_tuning(kpt, rb, hb, ring, alg)
             np.testing.assert_array_equal(sc.render(rd)[0], ref)
         with pytest.raises(swr.SwrError):
             ctx.set_tuning(10, 32, 0)

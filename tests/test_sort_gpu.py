@@ -120,16 +120,22 @@ def test_gather(ctx, words):
     assert np.array_equal(do.download(np.uint32, (n, words)), src[perm])
 
 
+@pytest.mark.parametrize("alg", [0, 1])
 @pytest.mark.parametrize("kpt", [8, 12, 20, 24])
-def test_sort_every_tile_size(ctx, kpt):
+def test_sort_every_tile_size(ctx, kpt, alg):
+    """Both pass structures (onesweep, reduce-then-scan) at every tile size, on sizes that leave
+    partial tiles and on skewed keys."""
     before = ctx.tuning()
     try:
-        ctx.set_tuning(kpt, 32, 0)
+        ctx.set_tuning(kpt, 32, 0, 0, alg)
         rng = np.random.default_rng(kpt)
         for n in (1, 5000, 300001):
             keys = rng.integers(0, 1 << 12, n).astype(np.uint32)
             k, v = _sort(ctx, keys, 12)
             assert np.array_equal(v, orc.stable_sort_perm(keys))
+        keys = (rng.integers(0, 3, 200000) << 24 | rng.integers(0, 1 << 24, 200000)).astype(np.uint32)
+        k, v = _sort(ctx, keys, 32)
+        assert np.array_equal(v, orc.stable_sort_perm(keys))
     finally:
         ctx.set_tuning(**before)
 
