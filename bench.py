@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
     ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched workload")
     ap.add_argument("--no-plain", action="store_true", help="batched workload: skip the job-by-job plain pass")
+    ap.add_argument("--batch-wave-rays", type=int, default=0,
+                    help="rays per wave of the batched loop (0 = the batch autotuner's choice)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serve-requests", type=int, default=1000, help="measured requests of the serving run")
     ap.add_argument("--serve-warmup", type=int, default=50, help="requests before the measured ones")
@@ -396,10 +398,17 @@ def batched_vs_plain(ctx, args, plain=True):
             plain_pix[j] = pix
     plain_wall = time.perf_counter() - wall
 
-    paced = b.run(paced=True)  # the same batch launched wave by wave from the host: per-stage times
-    b.run()  # warm-up of the graph path
+    wave_rays = args.batch_wave_rays
+    if not wave_rays:  # the batched loop's own wave size: the fastest of 2^23 .. 2^26 rays
+        trial = {}
+        for wr in (1 << 23, 1 << 24, 1 << 25, 1 << 26):
+            b.run(wr)
+            trial[wr] = b.run(wr)["ms_total"]
+        wave_rays = min(trial, key=trial.get)
+    paced = b.run(wave_rays, paced=True)  # the same batch launched wave by wave from the host: per-stage times
+    b.run(wave_rays)  # warm-up of the graph path
     wall = time.perf_counter()
-    st = b.run()
+    st = b.run(wave_rays)
     batch_wall = time.perf_counter() - wall
     batch_rays = st["closest_rays"] + st["shadow_rays"]
     same = all(np.array_equal(b.result(j)[0], p) for j, p in plain_pix.items())
@@ -422,6 +431,7 @@ def batched_vs_plain(ctx, args, plain=True):
         "plain_wall_s": round(plain_wall, 3),
         "batched_wall_s": round(batch_wall, 3),
         "waves": {"batched": st["waves"]},
+        "batched_wave_rays": wave_rays,
         "paced_batched_mrays_s": round((paced["closest_rays"] + paced["shadow_rays"]) / (paced["ms_total"] * 1e-3) / 1e6,
                                        2),
         "paced_stage_ms": {k[3:]: round(paced[k], 2) for k in paced if k.startswith("ms_")},

@@ -246,7 +246,7 @@ int swr_batch_create(swr_context* ctx, swr_batch** out);
 void swr_batch_destroy(swr_batch* batch);
 /* Adds a job (at most 65536; scenes: at most 256 distinct). *job receives its index. */
 int swr_batch_add(swr_batch* batch, const swr_scene* scene, const swr_render_desc* desc, int32_t* job);
-/* Traces every job to completion with waves of wave_rays rays (0 = 2^23). stats: the whole
+/* Traces every job to completion with waves of wave_rays rays (0 = 2^23, at most 2^26). stats: the whole
  * batch (ms_total = first raygen to the last resolved framebuffer). flags: 0, or
  * SWR_BATCH_PACED to launch each wave from the host with sizes read back after the previous one
  * (no graphs; fills the per-stage times; for profiling and comparison). */
@@ -273,7 +273,7 @@ typedef struct swr_server swr_server;
 typedef struct {
   uint64_t state_bytes;      /* device memory for request state (per-sample arenas, 24 B per sample of one
                                 iteration, and FP32 framebuffers, 12 B per pixel); 0 = 8 GiB */
-  uint32_t wave_rays;        /* rays per wave (0 = 2^23) */
+  uint32_t wave_rays;        /* rays per wave (0 = 2^23, at most 2^26) */
   uint32_t max_requests;     /* job slots: iterations on the device at once (0 = 256, at most 65536) */
   uint32_t waves_per_check;  /* graph-replayed waves between host checks (0 = 2) */
   uint32_t iteration_window; /* iterations of one request in the loop at once (0 = 16); each holds its own

@@ -383,7 +383,7 @@ int swr_server_create(swr_context* ctx, const swr_scene* const* scenes, int32_t 
     const uint32_t max_req = s->cfg.max_requests ? s->cfg.max_requests : 256;
     if (max_req > 65536) swr::fail(SWR_E_INVALID, "server: at most 65536 requests on the device");
     uint32_t cap = s->cfg.wave_rays ? s->cfg.wave_rays : (1u << 23);
-    cap = std::max<uint32_t>(64, std::min<uint32_t>(cap, 1u << 24));
+    cap = std::max<uint32_t>(64, std::min<uint32_t>(cap, 1u << 26));
     const uint64_t before = ctx->in_use;
     s->loop = std::make_unique<swr::WaveLoop>(ctx, s->bufs, s->scenes, max_req, cap);
     s->stats.queue_bytes = ctx->in_use - before;
