@@ -713,7 +713,7 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
     : ctx_(ctx), b_(bufs), stream_(ctx->stream), scenes_(scenes), max_jobs_(max_jobs), cap_(capacity) {
   if (scenes.empty() || scenes.size() > 256) fail(SWR_E_INVALID, "wavefront: 1 to 256 scenes");
   if (max_jobs == 0 || max_jobs > 65536) fail(SWR_E_INVALID, "wavefront: 1 to 65536 job slots");
-  if (capacity < 64 || capacity > (1u << 24)) fail(SWR_E_INVALID, "wavefront: wave capacity must be 64 .. 2^24");
+  if (capacity < 64 || capacity > (1u << 26)) fail(SWR_E_INVALID, "wavefront: wave capacity must be 64 .. 2^26");
   uint32_t max_tris = 1, max_depth = 0;
   for (const swr_scene* s : scenes) {
     if (!s || s->ctx != ctx) fail(SWR_E_INVALID, "wavefront: scene belongs to another (or a destroyed) context");
@@ -1078,7 +1078,7 @@ void destroy_loop_buffers(swr_context* ctx) {
 namespace {
 
 uint32_t wave_capacity(const swr_render_desc& desc, uint64_t total) {
-  const int exp = std::max(10, std::min(desc.bin_exponent > 0 ? desc.bin_exponent : 20, 24));
+  const int exp = std::max(10, std::min(desc.bin_exponent > 0 ? desc.bin_exponent : 20, 26));
   return (uint32_t)std::min<uint64_t>(uint64_t(1) << exp, std::max<uint64_t>(total, 64));
 }
 
