@@ -303,6 +303,12 @@ def kernels_bench(ctx, scene, desc, args):
                            keys_per_thread=best[1],
                            sweep_ms={f"{['onesweep', 'scan'][a]}/{k}": v["ms"] for (a, k), v in sweep.items()})
     del keys
+    rng = np.random.default_rng(0x5EED0007)
+    mesh = rng.integers(0, 64, n, dtype=np.uint32)
+    face = rng.integers(0, 1 << 21, n, dtype=np.uint32)
+    d_mesh, d_face = ctx.upload(mesh), ctx.upload(face)
+    ctx.mesh_face_keys(d_mesh, d_face, n, d_keys)
+    hkeys = d_keys.download(np.uint32, (n,))
     res["hit_sort"] = time_sort(hkeys, 28)
     res["hit_sort"]["keys_match"] = bool(np.array_equal(hkeys, ((mesh + 1) << 21) | face))
     ctx.set_tuning(**saved)

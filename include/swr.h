@@ -118,7 +118,7 @@ typedef struct {
  * (bucket size, shading size, bin exponent; Settings.h:16-24); these are chosen per context by
  * swr_autotune or set directly. */
 typedef struct {
-  int32_t sort_keys_per_thread;  /* onesweep tile = 256 * this: 8, 12, 16, 20 or 24 */
+  int32_t sort_keys_per_thread;  /* sort tile = 512 * this: 8, 12, 16, 20 or 24 */
   int32_t ray_key_bits;          /* top bits of the octant + Morton ray key the ray sort orders by (3..32) */
   int32_t hit_key_bits;          /* top bits of the hit key the hit sort orders by (0 = all bits) */
   int32_t trace_stack_entries;   /* LDS ring of the traversal stack per ray: 8, 16, 32 or 64 entries, deeper

@@ -8,16 +8,21 @@
 //
 // Design (onesweep, Adinets & Merrill 2022, restated for CDNA4 wave64):
 //   1. one read of the keys builds the 256-bin histograms of every 8-bit digit position;
-//   2. one kernel per digit: each workgroup takes a 4096-key tile (ticketed in launch order),
-//      ranks keys stably inside each wave with 8 ballots per key (a 64-lane match-any),
-//      publishes its per-digit counts, resolves its global offsets by decoupled look-back over
-//      earlier tiles, stages the tile in LDS in digit order and writes runs of equal digits
-//      contiguously.
+//   2. one kernel per digit: each 512-lane workgroup takes a tile of 512 * kpt keys (ticketed in
+//      launch order), ranks keys stably inside each wave, publishes its per-digit counts,
+//      resolves its global offsets by decoupled look-back over earlier tiles, stages the tile in
+//      LDS in digit order and writes runs of equal digits contiguously.
+// A key's peers (the lanes of its wave holding the same digit) come from LDS: each lane ORs its
+// lane bit into its wave's 64-bit word for its digit, reads the word back and clears it. That is
+// about a quarter of the VALU work of matching the 8 digit bits by ballots, which made the pass
+// ALU-bound. Values are loaded after the ranking, so their registers are not live across it.
 // Look-back status words are 64-bit: [63:32] epoch tag of the pass, [31:30] flag
 // (1 aggregate, 2 inclusive prefix), [29:0] count. Words from older passes carry an older
 // epoch and read as "not ready", so the status buffer never needs clearing between passes.
 // The epoch counter and the element count can live in device memory, so that a captured
 // graph of the wavefront loop replays the sort with fresh tags and the current wave size.
+// The reduce-then-scan alternative replaces the look-back by an upsweep of per-tile digit counts
+// and a per-digit scan over tiles.
 #include "swr_internal.h"
 
 namespace swr {
@@ -25,6 +30,11 @@ namespace {
 
 constexpr int kRadix = 256;
 constexpr int kWaves = kSortBlock / 64;
+constexpr int kHistBlock = 1024;
+// The histogram kernel adds into kHistCopies copies of the 4 x 256 counts (workgroup i into copy
+// i % kHistCopies, i.e. one copy per XCD under round-robin dispatch): adds from every CU to one
+// word serialise, and spreading them over copies cuts that chain eightfold.
+constexpr int kHistCopies = 8;
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPre = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1;
@@ -36,54 +46,58 @@ constexpr int kEpochWord = 64;
 __global__ void sort_prepare_kernel(uint32_t* hist, uint32_t* counters, int passes) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int p = 0; p < 4; ++p) hist[p * kRadix + t] = 0;
+  for (int c = 0; c < kHistCopies * 4; ++c) hist[c * kRadix + t] = 0;
   if (t < 4) counters[t] = 0;
   if (t == 0) counters[kEpochWord] += (uint32_t)passes;  // pass p uses that - passes + 1 + p
 }
 
-// Adds key digits to this wave's LDS histogram: one LDS atomic per lane, or one per wave when
-// every active lane holds the same digit (skewed digits such as a few direction octants or a
-// mostly-zero top byte would otherwise serialise 64 atomics on one bank).
+// Adds key digits to an LDS histogram: one LDS atomic per lane, or one per wave when every
+// active lane holds the same digit (skewed digits such as a few direction octants or a mostly-
+// zero top byte would otherwise serialise 64 atomics on one bank). Called by whole waves.
 __device__ inline void hist_add(uint32_t* h, uint32_t d, bool ok) {
   const uint64_t active = __ballot(ok);
   if (!active) return;
-  const int first = __ffsll((unsigned long long)active) - 1;
-  const uint32_t d0 = __shfl(d, first, 64);
-  if (__ballot(ok && d == d0) == active) {
-    if ((int)(threadIdx.x & 63) == first) atomicAdd(&h[d0], (uint32_t)__popcll(active));
+  const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);  // lane 0 (the whole wave is active)
+  if ((active & 1ull) && __ballot(ok && d == d0) == active) {
+    if ((threadIdx.x & 63) == 0) atomicAdd(&h[d0], (uint32_t)__popcll(active));
   } else if (ok) {
     atomicAdd(&h[d], 1u);
   }
 }
 
-// Histograms of all digit positions in one read of the keys: each thread reads 4 keys per
-// 16-byte load (grid-stride), each wave counts into its own LDS copy, and each workgroup adds its
-// counts to the global histogram once.
+// Histograms of all digit positions in one read of the keys: one 1024-lane workgroup per CU,
+// grid-stride over 16-byte loads, four LDS copies (one per four waves); each workgroup adds its
+// counts to its global copy once.
 template <bool kVec4>
-__global__ __launch_bounds__(256) void sort_histogram_kernel(const uint32_t* __restrict__ keys, size_t n_host,
-                                                             const uint32_t* __restrict__ n_dev, int passes,
-                                                             uint32_t* __restrict__ hist) {
-  __shared__ uint32_t s_hist[kWaves][4 * kRadix];
+__global__ __launch_bounds__(kHistBlock) void sort_histogram_kernel(const uint32_t* __restrict__ keys, size_t n_host,
+                                                                    const uint32_t* __restrict__ n_dev, int passes,
+                                                                    uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_hist[4][4 * kRadix];
   const size_t n = n_dev ? (size_t)*n_dev : n_host;
-  for (int i = threadIdx.x; i < kWaves * 4 * kRadix; i += blockDim.x) (&s_hist[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < 4 * 4 * kRadix; i += kHistBlock) (&s_hist[0][0])[i] = 0;
   __syncthreads();
-  uint32_t* h = s_hist[threadIdx.x >> 6];
+  uint32_t* h = s_hist[threadIdx.x >> 8];
   const size_t n4 = kVec4 ? n / 4 : 0;  // keys not 16-byte aligned: all go through the scalar loop
   const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t stride = (size_t)gridDim.x * kHistBlock;
   // Whole waves iterate together (the bound is rounded up per wave) so the ballots see every lane.
-  const size_t first = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t first = (size_t)blockIdx.x * kHistBlock + threadIdx.x;
   const size_t wave_first = first - (threadIdx.x & 63);
-  for (size_t wi = wave_first, i = first; wi < n4; wi += stride, i += stride) {
-    const bool ok = i < n4;
-    const uint4 k = ok ? k4[i] : make_uint4(0u, 0u, 0u, 0u);
+  for (size_t wi = wave_first, i = first; wi < n4; wi += 2 * stride, i += 2 * stride) {
+    const bool ok0 = i < n4, ok1 = i + stride < n4;
+    const uint4 a = ok0 ? k4[i] : make_uint4(0u, 0u, 0u, 0u);
+    const uint4 b = ok1 ? k4[i + stride] : make_uint4(0u, 0u, 0u, 0u);
     for (int p = 0; p < passes; ++p) {
       const int sh = 8 * p;
       uint32_t* hp = h + p * kRadix;
-      hist_add(hp, (k.x >> sh) & 255u, ok);
-      hist_add(hp, (k.y >> sh) & 255u, ok);
-      hist_add(hp, (k.z >> sh) & 255u, ok);
-      hist_add(hp, (k.w >> sh) & 255u, ok);
+      hist_add(hp, (a.x >> sh) & 255u, ok0);
+      hist_add(hp, (a.y >> sh) & 255u, ok0);
+      hist_add(hp, (a.z >> sh) & 255u, ok0);
+      hist_add(hp, (a.w >> sh) & 255u, ok0);
+      hist_add(hp, (b.x >> sh) & 255u, ok1);
+      hist_add(hp, (b.y >> sh) & 255u, ok1);
+      hist_add(hp, (b.z >> sh) & 255u, ok1);
+      hist_add(hp, (b.w >> sh) & 255u, ok1);
     }
   }
   // The keys after the last full 16-byte group (all of them without vector loads).
@@ -93,16 +107,17 @@ __global__ __launch_bounds__(256) void sort_histogram_kernel(const uint32_t* __r
     for (int p = 0; p < passes; ++p) hist_add(h + p * kRadix, (k >> (8 * p)) & 255u, ok);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x) {
-    uint32_t v = 0;
-#pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) v += s_hist[ww][i];
-    if (v) atomicAdd(&hist[i], v);
+  uint32_t* copy = hist + (size_t)(blockIdx.x % kHistCopies) * 4 * kRadix;
+  for (int i = threadIdx.x; i < passes * kRadix; i += kHistBlock) {
+    const uint32_t v = s_hist[0][i] + s_hist[1][i] + s_hist[2][i] + s_hist[3][i];
+    if (v) atomicAdd(&copy[i], v);
   }
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
+// Exclusive scan of one value per thread over a kBlock-thread workgroup.
+template <int kBlock>
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp) {
+  constexpr int kW = kBlock / 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t x = v;
 #pragma unroll
@@ -114,65 +129,73 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp) {
   __syncthreads();
   uint32_t pre = 0;
 #pragma unroll
-  for (int i = 0; i < kWaves; ++i)
+  for (int i = 0; i < kW; ++i)
     if (i < w) pre += s_tmp[i];
   __syncthreads();
   return pre + x - v;
 }
 
-// Upsweep of a scanned pass: every tile's digit counts, digit-major (counts[d * stride + tile]),
-// for the same tiles the pass kernel will take.
+// Upsweep of a scanned pass: digit counts of kUpTiles consecutive tiles per workgroup (the same
+// tiles the pass kernel will take), digit-major (counts[d * stride + tile]): each digit row gets a
+// run of kUpTiles consecutive words from a workgroup instead of single scattered words.
+constexpr int kUpTiles = 4;
 template <int kKpt>
 __global__ __launch_bounds__(kSortBlock) void upsweep_kernel(const uint32_t* __restrict__ keys, uint32_t n_host,
                                                              const uint32_t* __restrict__ n_dev, int shift,
                                                              uint32_t* __restrict__ counts, uint32_t stride) {
   constexpr int kTile = kSortBlock * kKpt;
-  __shared__ uint32_t s_h[kWaves][kRadix];
+  __shared__ uint32_t s_h[kUpTiles][kRadix];
   const uint32_t n = n_dev ? *n_dev : n_host;
-  const uint32_t tile = blockIdx.x, base = tile * (uint32_t)kTile;
-  if (base >= n) return;
+  const uint32_t tile0 = blockIdx.x * (uint32_t)kUpTiles;
+  if (tile0 * (uint32_t)kTile >= n) return;
+  const uint32_t tiles = (n + kTile - 1) / kTile;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  for (int i = t; i < kWaves * kRadix; i += kSortBlock) (&s_h[0][0])[i] = 0;
+  for (int i = t; i < kUpTiles * kRadix; i += kSortBlock) (&s_h[0][0])[i] = 0;
   __syncthreads();
-  const uint32_t wbase = base + (uint32_t)w * 64u * kKpt;
+#pragma unroll 1
+  for (int g = 0; g < kUpTiles; ++g) {
+    if (tile0 + g >= tiles) break;
+    const uint32_t wbase = (tile0 + g) * (uint32_t)kTile + (uint32_t)w * 64u * kKpt;
+    uint32_t k[kKpt];
 #pragma unroll
-  for (int i = 0; i < kKpt; ++i) {
-    const uint32_t p = wbase + i * 64 + lane;
-    const bool ok = p < n;
-    const uint32_t k = ok ? keys[p] : 0u;
-    hist_add(s_h[w], (k >> shift) & 255u, ok);
+    for (int i = 0; i < kKpt; ++i) {
+      const uint32_t p = wbase + i * 64 + lane;
+      k[i] = p < n ? keys[p] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < kKpt; ++i) hist_add(s_h[g], (k[i] >> shift) & 255u, wbase + i * 64 + lane < n);
   }
   __syncthreads();
-  uint32_t c = 0;
-#pragma unroll
-  for (int ww = 0; ww < kWaves; ++ww) c += s_h[ww][t];
-  counts[(size_t)t * stride + tile] = c;
+  for (int i = t; i < kUpTiles * kRadix; i += kSortBlock) {
+    const int d = i / kUpTiles, g = i % kUpTiles;
+    if (tile0 + g < tiles) counts[(size_t)d * stride + tile0 + g] = s_h[g][d];
+  }
 }
 
-// Exclusive prefix over the tiles of one digit's counts (in place; workgroup = digit) and the
-// digit's total.
-__global__ __launch_bounds__(kSortBlock) void tile_scan_kernel(uint32_t* __restrict__ counts, uint32_t stride,
-                                                               uint32_t n_host, const uint32_t* __restrict__ n_dev,
-                                                               uint32_t tile_keys, uint32_t* __restrict__ totals) {
-  __shared__ uint32_t s_tmp[kWaves];
+// Exclusive prefix over the tiles of one digit's counts (in place; workgroup = digit, each
+// thread a run of consecutive tiles) and the digit's total.
+__global__ __launch_bounds__(256) void tile_scan_kernel(uint32_t* __restrict__ counts, uint32_t stride,
+                                                        uint32_t n_host, const uint32_t* __restrict__ n_dev,
+                                                        uint32_t tile_keys, uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_tmp[4];
   const uint32_t n = n_dev ? *n_dev : n_host;
   const uint32_t tiles = (n + tile_keys - 1) / tile_keys;
   uint32_t* row = counts + (size_t)blockIdx.x * stride;
-  const uint32_t per = (tiles + kSortBlock - 1) / kSortBlock;
+  const uint32_t per = (tiles + 255) / 256;
   const uint32_t b = threadIdx.x * per, e = min(tiles, b + per);
   uint32_t sum = 0;
   for (uint32_t i = b; i < e; ++i) sum += row[i];
-  const uint32_t pre = block_exclusive_scan(sum, s_tmp);
+  const uint32_t pre = block_exclusive_scan<256>(sum, s_tmp);
   uint32_t run = pre;
   for (uint32_t i = b; i < e; ++i) {
     const uint32_t c = row[i];
     row[i] = run;
     run += c;
   }
-  if (threadIdx.x == kSortBlock - 1) totals[blockIdx.x] = pre + sum;
+  if (threadIdx.x == 255) totals[blockIdx.x] = pre + sum;
 }
 
-// One digit pass over a tile of 256 * kKpt keys: stable rank inside the tile, then scatter
+// One digit pass over a tile of 512 * kKpt keys: stable rank inside the tile, then scatter
 // keys and values to their global positions. The tile's global offset per digit comes either from
 // a decoupled look-back over earlier tiles (onesweep: tiles ticketed in launch order, statuses
 // tagged by epoch) or, with kScanned, from the per-tile digit prefixes of an upsweep + scan
@@ -183,13 +206,14 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
                                                               uint32_t* __restrict__ keys_out,
                                                               uint32_t* __restrict__ vals_out, uint32_t n_host,
                                                               const uint32_t* __restrict__ n_dev, int shift,
-                                                              const uint32_t* __restrict__ hist,
+                                                              const uint32_t* __restrict__ hist, int hist_copies,
                                                               uint32_t* __restrict__ counters, int pass, int passes,
                                                               uint64_t* __restrict__ status,
                                                               const uint32_t* __restrict__ tile_prefix,
                                                               uint32_t tile_stride) {
   constexpr int kTile = kSortBlock * kKpt;
   __shared__ uint32_t s_buf[kTile];
+  __shared__ uint64_t s_peer[kWaves][kRadix];
   __shared__ uint32_t s_wave[kWaves][kRadix];
   __shared__ uint32_t s_start[kRadix];
   __shared__ uint32_t s_global[kRadix];
@@ -206,7 +230,10 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
       s_epoch = counters[kEpochWord] - (uint32_t)passes + 1u + (uint32_t)pass;
     }
   }
-  for (int i = t; i < kWaves * kRadix; i += kSortBlock) (&s_wave[0][0])[i] = 0;
+  for (int i = t; i < kWaves * kRadix; i += kSortBlock) {
+    (&s_wave[0][0])[i] = 0;
+    (&s_peer[0][0])[i] = 0;
+  }
   const uint32_t n = n_dev ? *n_dev : n_host;
   __syncthreads();
   const uint32_t tile = s_tile;
@@ -215,17 +242,46 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
   // Launches are sized for the largest wave; tickets past the current count have no keys, and
   // no earlier tile waits on them.
   if (tile_base >= n) return;
-  // Global digit bases of this pass: exclusive scan of the histogram.
-  const uint32_t digit_base = block_exclusive_scan(hist[t], s_tmp);
   const uint32_t valid = min((uint32_t)kTile, n - tile_base);
   const uint32_t wbase = (uint32_t)w * 64u * kKpt;
+  // Thread t < 256 owns digit t from here to the end.
+  uint32_t digit_total = 0;
+  if (t < kRadix)
+    for (int c = 0; c < hist_copies; ++c) digit_total += hist[(size_t)c * 4 * kRadix + t];
+  const uint32_t tile_pre = kScanned && t < kRadix ? tile_prefix[(size_t)t * tile_stride + tile] : 0u;
 
-  uint32_t key[kKpt], val[kKpt], rank[kKpt];
+  uint32_t key[kKpt], rank[kKpt];
 #pragma unroll
   for (int i = 0; i < kKpt; ++i) {
     const uint32_t p = wbase + i * 64 + lane;
     key[i] = p < valid ? keys_in[tile_base + p] : 0xFFFFFFFFu;
   }
+
+  // Stable rank of every key among equal digits of its wave, in original order: the running
+  // count of the digit in this wave (an LDS add by the lowest peer) plus the peers below the lane.
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t my_bit = 1ull << lane;
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const bool ok = wbase + i * 64 + lane < valid;
+    const uint32_t d = (key[i] >> shift) & 255u;
+    const uint64_t active = __ballot(ok);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    uint64_t peers;
+    if ((active & 1ull) && __ballot(ok && d == d0) == active) {
+      peers = active;
+    } else {
+      uint64_t* word = &s_peer[w][d];
+      if (ok) __hip_atomic_fetch_or(word, my_bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      peers = ok ? __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
+      if (ok) __hip_atomic_store(word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+    const int leader = peers ? (__ffsll((unsigned long long)peers) - 1) : 0;
+    uint32_t old = 0;
+    if (ok && lane == leader) old = atomicAdd(&s_wave[w][d], (uint32_t)__popcll(peers));
+    rank[i] = (uint32_t)__shfl((int)old, leader, 64) + (uint32_t)__popcll(peers & lt_mask);
+  }
+  uint32_t val[kKpt];
 #pragma unroll
   for (int i = 0; i < kKpt; ++i) {
     const uint32_t p = wbase + i * 64 + lane;
@@ -234,71 +290,51 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
     else
       val[i] = p < valid ? vals_in[tile_base + p] : 0u;
   }
-
-  // Stable rank of every key among equal digits of its wave, in original order.
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-  for (int i = 0; i < kKpt; ++i) {
-    const uint32_t p = wbase + i * 64 + lane;
-    const bool ok = p < valid;
-    const uint32_t d = (key[i] >> shift) & 255u;
-    uint64_t peers = __ballot(ok);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
-    }
-    const int leader = peers ? (__ffsll((unsigned long long)peers) - 1) : 0;
-    uint32_t old = 0;
-    if (ok && lane == leader) {
-      old = s_wave[w][d];
-      s_wave[w][d] = old + (uint32_t)__popcll(peers);
-    }
-    old = __shfl(old, leader, 64);
-    rank[i] = old + (uint32_t)__popcll(peers & lt_mask);
-  }
   __syncthreads();
 
-  // Per-digit wave prefixes and tile totals (thread t owns digit t).
+  // Per-digit wave prefixes and tile totals.
   uint32_t total = 0;
+  if (t < kRadix) {
 #pragma unroll
-  for (int ww = 0; ww < kWaves; ++ww) {
-    const uint32_t c = s_wave[ww][t];
-    s_wave[ww][t] = total;
-    total += c;
+    for (int ww = 0; ww < kWaves; ++ww) {
+      const uint32_t c = s_wave[ww][t];
+      s_wave[ww][t] = total;
+      total += c;
+    }
   }
   const uint64_t tag = (uint64_t)epoch << 32;
   uint64_t* my_status = status + (size_t)tile * kRadix + t;
-  if (!kScanned)
+  if (!kScanned && t < kRadix)
     __hip_atomic_store(my_status, tag | (uint64_t)((tile == 0 ? kFlagPre : kFlagAgg) | total), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t start = block_exclusive_scan(total, s_tmp);
-  s_start[t] = start;
+  // Global digit bases of this pass: exclusive scan of the histogram; the tile's digit starts.
+  const uint32_t digit_base = block_exclusive_scan<kSortBlock>(digit_total, s_tmp);
+  const uint32_t start = block_exclusive_scan<kSortBlock>(total, s_tmp);
 
-  uint32_t excl = 0;
-  if (kScanned) {
-    excl = tile_prefix[(size_t)t * tile_stride + tile];
-  } else if (tile > 0) {
-    // Decoupled look-back over earlier tiles for digit t.
-    int pred = (int)tile - 1;
-    for (;;) {
-      const uint64_t v = __hip_atomic_load(status + (size_t)pred * kRadix + t, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t lo = (uint32_t)v;
-      const uint32_t flag = lo & ~kCountMask;
-      if ((uint32_t)(v >> 32) != epoch || flag == 0) {
-        __builtin_amdgcn_s_sleep(1);
-        continue;
+  if (t < kRadix) {
+    uint32_t excl = tile_pre;
+    if (!kScanned && tile > 0) {
+      // Decoupled look-back over earlier tiles for digit t.
+      int pred = (int)tile - 1;
+      for (;;) {
+        const uint64_t v = __hip_atomic_load(status + (size_t)pred * kRadix + t, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t lo = (uint32_t)v;
+        const uint32_t flag = lo & ~kCountMask;
+        if ((uint32_t)(v >> 32) != epoch || flag == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += lo & kCountMask;
+        if (flag == kFlagPre) break;
+        --pred;
       }
-      excl += lo & kCountMask;
-      if (flag == kFlagPre) break;
-      --pred;
+      __hip_atomic_store(my_status, tag | (uint64_t)(kFlagPre | (excl + total)), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
-    __hip_atomic_store(my_status, tag | (uint64_t)(kFlagPre | (excl + total)), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    s_start[t] = start;
+    s_global[t] = digit_base + excl - start;  // wraps; start <= tile position of the digit's keys
   }
-  s_global[t] = digit_base + excl - start;  // wraps; start <= tile position of the digit's keys
   __syncthreads();
 
   // Stage keys in digit order.
@@ -347,7 +383,7 @@ size_t sort_status_bytes(size_t n) {
 
 void sort_reserve(swr_context* ctx, size_t n) {
   SortWorkspace& ws = ctx->sort;
-  ensure(ctx, ws.hist, 4 * kRadix * sizeof(uint32_t));
+  ensure(ctx, ws.hist, kHistCopies * 4 * kRadix * sizeof(uint32_t));
   if (!ws.counters.ptr) {
     ensure(ctx, ws.counters, 2 * kEpochWord * sizeof(uint32_t));
     SWR_HIP(hipMemset(ws.counters.ptr, 0, ws.counters.bytes));
@@ -358,8 +394,10 @@ void sort_reserve(swr_context* ctx, size_t n) {
     sort_reset_epochs(ctx, ctx->stream);
     SWR_HIP(hipStreamSynchronize(ctx->stream));
   }
-  // Per-tile digit counts of the scanned passes: 256 rows of the tile count at the smallest tile.
-  const size_t tiles = (n + (size_t)kSortBlock * kSortMinKeysPerThread - 1) / ((size_t)kSortBlock * kSortMinKeysPerThread);
+  // Per-tile digit counts of the scanned passes: 256 rows of the tile count at the smallest tile,
+  // rounded up to whole upsweep groups.
+  size_t tiles = (n + (size_t)kSortBlock * kSortMinKeysPerThread - 1) / ((size_t)kSortBlock * kSortMinKeysPerThread);
+  tiles = (tiles + kUpTiles - 1) / kUpTiles * kUpTiles;
   if (tiles > ws.tile_stride) {
     ensure(ctx, ws.tile_counts, (size_t)kRadix * tiles * sizeof(uint32_t));
     ws.tile_stride = (uint32_t)tiles;
@@ -408,12 +446,12 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
   const uint32_t tiles = (uint32_t)((n + tile_keys - 1) / tile_keys);
   if (!scanned) {
     sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters, passes);
-    const int hist_blocks =
-        (int)std::max<size_t>(1, std::min<size_t>((n + 4095) / 4096, (size_t)std::max(ctx->num_cus, 1) * 4));
+    const int hist_blocks = (int)std::max<size_t>(
+        1, std::min<size_t>((n + 8 * kHistBlock - 1) / (8 * kHistBlock), (size_t)std::max(ctx->num_cus, 1)));
     if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0)
-      sort_histogram_kernel<true><<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
+      sort_histogram_kernel<true><<<hist_blocks, kHistBlock, 0, stream>>>(keys, n, n_dev, passes, hist);
     else
-      sort_histogram_kernel<false><<<hist_blocks, 256, 0, stream>>>(keys, n, n_dev, passes, hist);
+      sort_histogram_kernel<false><<<hist_blocks, kHistBlock, 0, stream>>>(keys, n, n_dev, passes, hist);
   }
   uint32_t *sk = keys, *sv = vals, *dk = alt_keys, *dv = alt_vals;
   for (int p = 0; p < passes; ++p) {
@@ -422,22 +460,22 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
     // totals land in hist), then the pass kernel with the scanned per-tile offsets.
 #define SWR_PASS(KPT)                                                                                           \
   if (scanned) {                                                                                                \
-    upsweep_kernel<KPT><<<tiles, kSortBlock, 0, stream>>>(sk, (uint32_t)n, n_dev, 8 * p, tile_counts,           \
-                                                          ws.tile_stride);                                      \
-    tile_scan_kernel<<<kRadix, kSortBlock, 0, stream>>>(tile_counts, ws.tile_stride, (uint32_t)n, n_dev,          \
-                                                        tile_keys, hist + p * kRadix);                          \
+    upsweep_kernel<KPT><<<(tiles + kUpTiles - 1) / kUpTiles, kSortBlock, 0, stream>>>(sk, (uint32_t)n, n_dev,    \
+        8 * p, tile_counts, ws.tile_stride);                                                                    \
+    tile_scan_kernel<<<kRadix, 256, 0, stream>>>(tile_counts, ws.tile_stride, (uint32_t)n, n_dev, tile_keys,     \
+                                                 hist + p * kRadix);                                            \
     if (first_iota)                                                                                             \
       onesweep_kernel<true, KPT, true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,     \
-          8 * p, hist + p * kRadix, counters, p, passes, status, tile_counts, ws.tile_stride);                   \
+          8 * p, hist + p * kRadix, scanned ? 1 : kHistCopies, counters, p, passes, status, tile_counts, ws.tile_stride);                                   \
     else                                                                                                        \
       onesweep_kernel<false, KPT, true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,    \
-          8 * p, hist + p * kRadix, counters, p, passes, status, tile_counts, ws.tile_stride);                   \
+          8 * p, hist + p * kRadix, scanned ? 1 : kHistCopies, counters, p, passes, status, tile_counts, ws.tile_stride);                                   \
   } else if (first_iota) {                                                                                      \
     onesweep_kernel<true, KPT, false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,      \
-        8 * p, hist + p * kRadix, counters, p, passes, status, tile_counts, ws.tile_stride);                     \
+        8 * p, hist + p * kRadix, scanned ? 1 : kHistCopies, counters, p, passes, status, tile_counts, ws.tile_stride);                                     \
   } else {                                                                                                      \
     onesweep_kernel<false, KPT, false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,     \
-        8 * p, hist + p * kRadix, counters, p, passes, status, tile_counts, ws.tile_stride);                     \
+        8 * p, hist + p * kRadix, scanned ? 1 : kHistCopies, counters, p, passes, status, tile_counts, ws.tile_stride);                                     \
   }
     switch (kpt) {
       case 8: SWR_PASS(8) break;

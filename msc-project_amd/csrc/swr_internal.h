@@ -40,7 +40,7 @@ struct DeviceBuffer {
 // Workspace of the onesweep radix sort.
 struct SortWorkspace {
   DeviceBuffer alt_keys, alt_vals;   // ping-pong partners of the caller's buffers
-  DeviceBuffer hist;                 // 4 passes x 256 digits
+  DeviceBuffer hist;                 // 8 copies x 4 passes x 256 digits (summed by the pass kernel)
   DeviceBuffer counters;             // 4 tile tickets and the device epoch counter
   DeviceBuffer status;               // look-back words, tiles x 256 x u64
   DeviceBuffer tile_counts;          // per-tile digit counts of the reduce-then-scan passes, 256 x tile_stride
@@ -57,7 +57,7 @@ struct LoopBuffers;
 void destroy_loop_buffers(swr_context* ctx);
 // Launch shapes chosen per context (by swr_autotune or swr_set_tuning).
 struct Tuning {
-  int sort_kpt = 16;       // keys per thread of a onesweep tile (tile = 256 * kpt)
+  int sort_kpt = 16;       // keys per thread of a onesweep tile (tile = 512 * kpt)
   int ray_key_bits = 32;   // top bits of the octant + Morton ray key that the ray sort orders by
   int hit_key_bits = 0;    // top bits of the hit key sorted (0 = all: full (geomID, primID) order)
   int trace_stack = 0;     // LDS ring entries of the traversal stack (0 = from the BVH depth)
@@ -125,7 +125,7 @@ void sort_reset_epochs(swr_context* ctx, hipStream_t stream);
 // Scratch sizes the sort needs for n elements.
 size_t sort_status_bytes(size_t n);
 void sort_reserve(swr_context* ctx, size_t n);
-constexpr int kSortBlock = 256;
+constexpr int kSortBlock = 512;
 constexpr int kSortMinKeysPerThread = 8;
 
 // ---- keys.hip ----
