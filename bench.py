@@ -388,10 +388,12 @@ def batched_vs_plain(ctx, args, plain=True):
         ctx.set_tuning(*[int(v) for v in args.batch_tuning.split(",")])
         tuning = ctx.tuning()
     else:
-        tuning = b.autotune()  # per-scene stack rings + sort tile and key widths, used by both passes
+        # per-scene stack rings + sort tile and key widths at the plain operating point (the
+        # default wave size); the batched loop is tuned again at the wave size it runs
+        tuning = b.autotune()
     tune_s = time.time() - t0
 
-    # plain: one render per job through the host-paced loop, same launch shapes
+    # plain: one render per job through the host-paced loop, with the tuning above
     plain_ms, plain_rays, plain_pix = 0.0, 0, {}
     wall = time.perf_counter()
     for j, ((s, _), rd) in enumerate(zip(jobs, rds)):
@@ -405,12 +407,17 @@ def batched_vs_plain(ctx, args, plain=True):
     plain_wall = time.perf_counter() - wall
 
     wave_rays = args.batch_wave_rays
-    if not wave_rays:  # the batched loop's own wave size: the fastest of 2^23 .. 2^26 rays
+    if not wave_rays:  # the batched loop's own wave size: the fastest of 2^24 .. 2^28 rays
         trial = {}
-        for wr in (1 << 23, 1 << 24, 1 << 25, 1 << 26):
+        for wr in (1 << 24, 1 << 25, 1 << 26, 1 << 27, 1 << 28):
             b.run(wr)
             trial[wr] = b.run(wr)["ms_total"]
         wave_rays = min(trial, key=trial.get)
+    batch_tuning = tuning
+    if not args.batch_tuning:
+        t0 = time.time()
+        batch_tuning = b.autotune(wave_rays)
+        tune_s += time.time() - t0
     paced = b.run(wave_rays, paced=True)  # the same batch launched wave by wave from the host: per-stage times
     b.run(wave_rays)  # warm-up of the graph path
     wall = time.perf_counter()
@@ -443,7 +450,8 @@ def batched_vs_plain(ctx, args, plain=True):
         "paced_stage_ms": {k[3:]: round(paced[k], 2) for k in paced if k.startswith("ms_")},
         "images_equal_sampled": bool(same and plain_rays == batch_rays) if plain else None,
         "scene_rings": rings,
-        "batch_tuning": tuning,
+        "plain_tuning": tuning,
+        "batch_tuning": batch_tuning,
         "autotune_s": round(tune_s, 2),
         "scene_gen_s": round(gen_s, 2),
         "scene_upload_bvh_s": round(build_s, 2),

@@ -246,7 +246,7 @@ int swr_batch_create(swr_context* ctx, swr_batch** out);
 void swr_batch_destroy(swr_batch* batch);
 /* Adds a job (at most 65536; scenes: at most 256 distinct). *job receives its index. */
 int swr_batch_add(swr_batch* batch, const swr_scene* scene, const swr_render_desc* desc, int32_t* job);
-/* Traces every job to completion with waves of wave_rays rays (0 = 2^23, at most 2^26). stats: the whole
+/* Traces every job to completion with waves of wave_rays rays (0 = 2^23, at most 2^28). stats: the whole
  * batch (ms_total = first raygen to the last resolved framebuffer). flags: 0, or
  * SWR_BATCH_PACED to launch each wave from the host with sizes read back after the previous one
  * (no graphs; fills the per-stage times; for profiling and comparison). */

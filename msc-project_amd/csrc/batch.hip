@@ -32,7 +32,7 @@ namespace {
 
 uint32_t wave_capacity(uint32_t wave_rays, uint64_t total) {
   uint64_t cap = wave_rays ? wave_rays : (1u << 23);
-  cap = std::min<uint64_t>(cap, 1u << 26);
+  cap = std::min<uint64_t>(cap, 1u << 28);
   cap = std::min<uint64_t>(cap, std::max<uint64_t>(total, 64));
   return (uint32_t)std::max<uint64_t>(cap, 64);
 }

@@ -199,7 +199,11 @@ struct ShadeOut {
   uint32_t* block_min;   // per workgroup: smallest admission number among its continuations
 };
 
+// Workgroup sizes of the shading kernel: one job (62 VGPRs) fits two 1024-lane workgroups per CU.
+// The multi-job variant holds its lanes' job constants in vector registers; it is held to 80 VGPRs
+// (6 waves per SIMD), so that three 512-lane workgroups share a CU.
 constexpr int kShadeBlock = 1024;
+constexpr int kShadeBlockMulti = 512;
 
 // Integrator::operator() for every queued ray, in hit-key order (Integrator.cpp:12-255).
 // Radiance is added in place: a path has one live ray, so no two lanes of a launch touch the
@@ -211,13 +215,13 @@ constexpr int kShadeBlock = 1024;
 //
 // kOne: the loop holds one job on one scene, whose constants are then wave-uniform (scalar
 // loads). Otherwise each lane reads its job's constants and scene pointers.
-template <bool kOne>
-__global__ __launch_bounds__(kShadeBlock) void shade_kernel(const DeviceScene* __restrict__ scenes,
+template <bool kOne, int kBlock>
+__global__ __launch_bounds__(kBlock, kOne ? 1 : 6) void shade_kernel(const DeviceScene* __restrict__ scenes,
                                                             const FrameParams* __restrict__ jobs,
                                                             const uint32_t* __restrict__ perm, WaveState* w,
                                                             const RayRec* __restrict__ q, ShadeOut out) {
-  __shared__ uint32_t s_append[2 * (kShadeBlock / 64) + 1];
-  __shared__ uint32_t s_min[kShadeBlock / 64];
+  __shared__ uint32_t s_append[2 * (kBlock / 64) + 1];
+  __shared__ uint32_t s_min[kBlock / 64];
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = m < w->n;
   bool has_c = false, has_s = false, has_n = false;
@@ -324,10 +328,10 @@ __global__ __launch_bounds__(kShadeBlock) void shade_kernel(const DeviceScene* _
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) live = min(live, (uint32_t)__shfl_xor((int)live, off, 64));
   if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = live;
-  const uint32_t cslot = block_append<kShadeBlock>(&w->counters, has_n, has_s, s_append);
+  const uint32_t cslot = block_append<kBlock>(&w->counters, has_n, has_s, s_append);
   if (threadIdx.x == 0) {
     uint32_t b = 0xFFFFFFFFu;
-    for (int i = 0; i < kShadeBlock / 64; ++i) b = min(b, s_min[i]);
+    for (int i = 0; i < kBlock / 64; ++i) b = min(b, s_min[i]);
     out.block_min[blockIdx.x] = b;
   }
   if (has_n) {
@@ -713,7 +717,7 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
     : ctx_(ctx), b_(bufs), stream_(ctx->stream), scenes_(scenes), max_jobs_(max_jobs), cap_(capacity) {
   if (scenes.empty() || scenes.size() > 256) fail(SWR_E_INVALID, "wavefront: 1 to 256 scenes");
   if (max_jobs == 0 || max_jobs > 65536) fail(SWR_E_INVALID, "wavefront: 1 to 65536 job slots");
-  if (capacity < 64 || capacity > (1u << 26)) fail(SWR_E_INVALID, "wavefront: wave capacity must be 64 .. 2^26");
+  if (capacity < 64 || capacity > (1u << 28)) fail(SWR_E_INVALID, "wavefront: wave capacity must be 64 .. 2^28");
   uint32_t max_tris = 1, max_depth = 0;
   for (const swr_scene* s : scenes) {
     if (!s || s->ctx != ctx) fail(SWR_E_INVALID, "wavefront: scene belongs to another (or a destroyed) context");
@@ -759,7 +763,7 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
   ensure(ctx, b_.scene_tab, sizeof(DeviceScene) * scenes.size());
   ensure(ctx, b_.scene_ring, scenes.size());
   ensure(ctx, b_.wave, sizeof(WaveState));
-  ensure(ctx, b_.block_min, 4ull * blocks(cap_, kShadeBlock));
+  ensure(ctx, b_.block_min, 4ull * blocks(cap_, std::min(kShadeBlock, kShadeBlockMulti)));
   sort_reserve(ctx, cap_);
   if (!b_.host_state) {
     SWR_HIP(hipHostMalloc((void**)&b_.host_state, 8 * sizeof(WaveState), hipHostMallocDefault));
@@ -870,7 +874,7 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
       raygen_kernel<false><<<blocks(kb), 256, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
   }
   mark(1);
-  const uint32_t shade_blocks = blocks(nb, kShadeBlock);
+  const uint32_t shade_blocks = blocks(nb, one_job ? kShadeBlock : kShadeBlockMulti);
   if (nb) {
     uint32_t *rk = nullptr, *rperm = nullptr;
     sort_pairs(ctx_, keys[cur], perm, alt_k, alt_v, nb, ray_bits_, true, &rk, &rperm, st, n_dev);
@@ -905,9 +909,10 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
     so.radiance = radiance_;
     so.block_min = static_cast<uint32_t*>(b_.block_min.ptr);
     if (one_job)
-      shade_kernel<true><<<shade_blocks, kShadeBlock, 0, st>>>(stab, jobs, hperm, w, queue[cur], so);
+      shade_kernel<true, kShadeBlock><<<shade_blocks, kShadeBlock, 0, st>>>(stab, jobs, hperm, w, queue[cur], so);
     else
-      shade_kernel<false><<<shade_blocks, kShadeBlock, 0, st>>>(stab, jobs, hperm, w, queue[cur], so);
+      shade_kernel<false, kShadeBlockMulti><<<shade_blocks, kShadeBlockMulti, 0, st>>>(stab, jobs, hperm, w,
+                                                                                       queue[cur], so);
     mark(5);
     for (int ring : ring_set_) {
       with_ring(ring, [&](auto kShort) {

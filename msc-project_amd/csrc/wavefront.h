@@ -65,7 +65,7 @@ struct LoopBuffers {
 class WaveLoop {
  public:
   // scenes: the resident scenes (at most 256); max_jobs: job slots (at most 65536);
-  // capacity: rays per wave (64 .. 2^26).
+  // capacity: rays per wave (64 .. 2^28).
   WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const swr_scene*>& scenes, uint32_t max_jobs,
            uint32_t capacity);
   ~WaveLoop();
