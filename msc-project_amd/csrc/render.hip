@@ -161,6 +161,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const DeviceSc
                                                                   uint32_t hit_tri_bits, uint32_t hit_shift,
                                                                   uint32_t* __restrict__ spill, uint32_t spill_stride) {
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
+  __shared__ uint16_t s_dist[kShort * kTraceBlock];
   const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
   const uint32_t n = w->n;
   if (blockIdx.x * kTraceBlock >= n) return;
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const DeviceSc
     const RayState r = make_ray(o, d);
     const HitResult h = traverse<kTraceBlock, kShort, false>(scenes[s0].nodes, scenes[s0].leaf_tris, r,
                                                              SWR_MASK_ALL, s_ring + threadIdx.x,
-                                                             spill + (size_t)j * spill_stride);
+                                                             spill + (size_t)j * spill_stride, s_dist + threadIdx.x);
     reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
     hit_keys[i] = ((s0 << hit_tri_bits) | (h.id + 1u)) >> hit_shift;
   };

@@ -80,6 +80,43 @@ __device__ inline bool tri_test(const RayState& r, float4 v0, float4 e1, float4 
   return true;
 }
 
+// Node and triangle pointers come out of the scene table in memory, so the compiler cannot tell
+// their address space and would issue flat loads; flat loads also count against the LDS wait
+// counter, so every stack access would wait for the node fetches in flight. These casts make
+// them global loads.
+// (Loads go through clang's native 4-vectors: HIP's float4 class cannot be copied from a
+// global-address-space reference.)
+#define SWR_GLOBAL __attribute__((address_space(1)))
+typedef float gf4 __attribute__((ext_vector_type(4)));
+typedef uint32_t gu4 __attribute__((ext_vector_type(4)));
+__device__ inline float4 gload(const SWR_GLOBAL gf4* p) {
+  const gf4 v = *p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline const SWR_GLOBAL gf4* global_f4(const float4* p) { return (const SWR_GLOBAL gf4*)p; }
+__device__ inline const SWR_GLOBAL gf4* global_nodes(const Node4* p) { return (const SWR_GLOBAL gf4*)p; }
+
+// Entry distances of the four children of a node; +inf for a missed box or an unused slot.
+// All four box tests run unconditionally and unused slots are masked afterwards, so the node's
+// 128 bytes are fetched in one round of loads rather than behind per-slot branches.
+// nd: the node as 8 16-byte rows (lox, hix, loy, hiy, loz, hiz, child, unused).
+__device__ inline void node_entries(const RayState& r, const SWR_GLOBAL gf4* nd, float tfar, uint4* ch,
+                                    float* t0, float* t1, float* t2, float* t3) {
+  const float4 lx = gload(nd + 0), hx = gload(nd + 1), ly = gload(nd + 2), hy = gload(nd + 3), lz = gload(nd + 4),
+               hz = gload(nd + 5);
+  const gu4 c = *(const SWR_GLOBAL gu4*)(nd + 6);
+  *ch = make_uint4(c.x, c.y, c.z, c.w);
+  const float inf = __int_as_float(0x7F800000);
+  const float s0 = slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tfar);
+  const float s1 = slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tfar);
+  const float s2 = slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tfar);
+  const float s3 = slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tfar);
+  *t0 = ch->x == kNoChild ? inf : s0;
+  *t1 = ch->y == kNoChild ? inf : s1;
+  *t2 = ch->z == kNoChild ? inf : s2;
+  *t3 = ch->w == kNoChild ? inf : s3;
+}
+
 __device__ inline void cswap(uint32_t& a, uint32_t& b) {
   const uint32_t lo = min(a, b), hi = max(a, b);
   a = lo;
@@ -96,19 +133,24 @@ struct HitResult {
 // lds[(k % kShort) * kBlock], so a wave's accesses hit 64 consecutive words); entries pushed
 // deeper than that spill, oldest first, to the lane's column in HBM and come back on pop.
 // A small ring keeps occupancy high; the spill makes any BVH depth correct.
-template <int kBlock, int kShort>
+// kDist: ring entries also keep the entry distance of their box, as the top 16 bits of the
+// float (a truncation, so never above the distance); spilled entries come back with 0.
+template <int kBlock, int kShort, bool kDist = false>
 struct ShortStack {
   static_assert((kShort & (kShort - 1)) == 0, "ring size must be a power of two");
   uint32_t* lds;    // this lane's ring column (stride kBlock)
+  uint16_t* dist;   // this lane's distance ring column (stride kBlock), with kDist
   uint32_t* spill;  // this lane's spill column (entries [0, base))
   int sp, base;
-  __device__ ShortStack(uint32_t* l, uint32_t* s) : lds(l), spill(s), sp(0), base(0) {}
-  __device__ void push(uint32_t v) {
+  __device__ ShortStack(uint32_t* l, uint32_t* s, uint16_t* dl = nullptr)
+      : lds(l), dist(dl), spill(s), sp(0), base(0) {}
+  __device__ void push(uint32_t v, uint32_t tbits = 0) {
     if (sp - base == kShort) {
       spill[base] = lds[(base & (kShort - 1)) * kBlock];
       ++base;
     }
     lds[(sp & (kShort - 1)) * kBlock] = v;
+    if (kDist) dist[(sp & (kShort - 1)) * kBlock] = (uint16_t)(tbits >> 16);
     ++sp;
   }
   __device__ bool empty() const { return sp == 0; }
@@ -120,32 +162,46 @@ struct ShortStack {
     }
     return lds[(sp & (kShort - 1)) * kBlock];
   }
+  // Pops an entry with a lower bound of its entry distance (0 for an entry that was spilled).
+  __device__ uint32_t pop(float* entry) {
+    --sp;
+    if (sp < base) {
+      base = sp;
+      *entry = 0.f;
+      return spill[sp];
+    }
+    *entry = __uint_as_float((uint32_t)dist[(sp & (kShort - 1)) * kBlock] << 16);
+    return lds[(sp & (kShort - 1)) * kBlock];
+  }
 };
 
 // Traverses one ray. kAny: returns on the first hit (id of that triangle), else the closest.
 // lds_col: this lane's LDS ring column (kShort entries, stride kBlock); spill_col: this lane's
 // HBM spill column (stack_need(depth) entries; unused when the ring is deep enough).
+// dist_col (closest hit only): this lane's ring of entry distances. A popped node whose entry
+// distance exceeds the closest hit so far by more than the box slack is skipped unfetched:
+// that is the parent's box test run again with the current hit distance, so the result is the
+// one without the skip.
 template <int kBlock, int kShort, bool kAny>
 __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const float4* __restrict__ leaf_tris,
-                                     const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col) {
+                                     const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col,
+                                     uint16_t* dist_col = nullptr) {
   HitResult best;
   best.t = r.tmax;
   best.u = best.v = __int_as_float(0x7FC00000);
   best.id = kMiss;
   best.leaf = kMiss;
   const float inf = __int_as_float(0x7F800000);
-  ShortStack<kBlock, kShort> st(lds_col, spill_col);
+  ShortStack<kBlock, kShort, !kAny> st(lds_col, spill_col, dist_col);
+  const SWR_GLOBAL gf4* gnodes = global_nodes(nodes);
+  const SWR_GLOBAL gf4* gtris = global_f4(leaf_tris);
   uint32_t node = 0;  // root
   for (;;) {
     if (!(node & kLeafBit)) {
-      const Node4* nd = nodes + node;
-      const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
-      const uint4 ch = nd->child;
       // Unused slots (kNoChild) are never entered, whatever their box.
-      const float t0 = ch.x == kNoChild ? inf : slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, best.t);
-      const float t1 = ch.y == kNoChild ? inf : slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, best.t);
-      const float t2 = ch.z == kNoChild ? inf : slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, best.t);
-      const float t3 = ch.w == kNoChild ? inf : slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, best.t);
+      uint4 ch;
+      float t0, t1, t2, t3;
+      node_entries(r, gnodes + 8 * (size_t)node, best.t, &ch, &t0, &t1, &t2, &t3);
       if (kAny) {
         if (t0 != inf) st.push(ch.x);
         if (t1 != inf) st.push(ch.y);
@@ -167,9 +223,9 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
         const uint32_t cs[4] = {ch.x, ch.y, ch.z, ch.w};
         // Push hit children far to near except the nearest (keys ascend, so a hit k3 implies
         // hits k0..k2), then descend into the nearest.
-        if ((k3 & ~3u) < miss_key) st.push(cs[k3 & 3u]);
-        if ((k2 & ~3u) < miss_key) st.push(cs[k2 & 3u]);
-        if ((k1 & ~3u) < miss_key) st.push(cs[k1 & 3u]);
+        if ((k3 & ~3u) < miss_key) st.push(cs[k3 & 3u], k3 & ~3u);
+        if ((k2 & ~3u) < miss_key) st.push(cs[k2 & 3u], k2 & ~3u);
+        if ((k1 & ~3u) < miss_key) st.push(cs[k1 & 3u], k1 & ~3u);
         if ((k0 & ~3u) < miss_key) {
           node = cs[k0 & 3u];
           continue;
@@ -178,10 +234,10 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
     } else {
       const uint32_t first = leaf_first(node), count = leaf_count(node);
       for (uint32_t k = 0; k < count; ++k) {
-        const float4* tp = leaf_tris + 3 * (size_t)(first + k);
-        const float4 e1 = tp[1];
+        const SWR_GLOBAL gf4* tp = gtris + 3 * (size_t)(first + k);
+        const float4 e1 = gload(tp + 1);
         if ((__float_as_uint(e1.w) & mask) == 0u) continue;
-        const float4 v0 = tp[0], e2 = tp[2];
+        const float4 v0 = gload(tp), e2 = gload(tp + 2);
         float t, u, v;
         if (tri_test(r, v0, e1, e2, &t, &u, &v)) {
           const uint32_t id = __float_as_uint(v0.w);
@@ -203,8 +259,21 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
         }
       }
     }
-    if (st.empty()) break;
-    node = st.pop();
+    if (kAny) {
+      if (st.empty()) break;
+      node = st.pop();
+    } else {
+      bool more = false;
+      while (!st.empty()) {
+        float entry;
+        node = st.pop(&entry);
+        if (!(entry > best.t * kBoxSlack)) {
+          more = true;
+          break;
+        }
+      }
+      if (!more) break;
+    }
   }
   return best;
 }
@@ -239,14 +308,10 @@ struct Walker {
   __device__ bool step(uint32_t mask) {
     const float inf = __int_as_float(0x7F800000);
     if (!(node & kLeafBit)) {
-      const Node4* nd = nodes + node;
-      const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
-      const uint4 ch = nd->child;
       // Unused slots (kNoChild) are never entered, whatever their box.
-      const float t0 = ch.x == kNoChild ? inf : slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, best.t);
-      const float t1 = ch.y == kNoChild ? inf : slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, best.t);
-      const float t2 = ch.z == kNoChild ? inf : slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, best.t);
-      const float t3 = ch.w == kNoChild ? inf : slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, best.t);
+      uint4 ch;
+      float t0, t1, t2, t3;
+      node_entries(r, global_nodes(nodes) + 8 * (size_t)node, best.t, &ch, &t0, &t1, &t2, &t3);
       if (kAny) {
         if (t0 != inf) st.push(ch.x);
         if (t1 != inf) st.push(ch.y);
@@ -268,9 +333,9 @@ struct Walker {
         const uint32_t cs[4] = {ch.x, ch.y, ch.z, ch.w};
         // Push hit children far to near except the nearest (keys ascend, so a hit k3 implies
         // hits k0..k2), then descend into the nearest.
-        if ((k3 & ~3u) < miss_key) st.push(cs[k3 & 3u]);
-        if ((k2 & ~3u) < miss_key) st.push(cs[k2 & 3u]);
-        if ((k1 & ~3u) < miss_key) st.push(cs[k1 & 3u]);
+        if ((k3 & ~3u) < miss_key) st.push(cs[k3 & 3u], k3 & ~3u);
+        if ((k2 & ~3u) < miss_key) st.push(cs[k2 & 3u], k2 & ~3u);
+        if ((k1 & ~3u) < miss_key) st.push(cs[k1 & 3u], k1 & ~3u);
         if ((k0 & ~3u) < miss_key) {
           node = cs[k0 & 3u];
           return false;
@@ -279,10 +344,10 @@ struct Walker {
     } else {
       const uint32_t first = leaf_first(node), count = leaf_count(node);
       for (uint32_t k = 0; k < count; ++k) {
-        const float4* tp = leaf_tris + 3 * (size_t)(first + k);
-        const float4 e1 = tp[1];
+        const SWR_GLOBAL gf4* tp = global_f4(leaf_tris) + 3 * (size_t)(first + k);
+        const float4 e1 = gload(tp + 1);
         if ((__float_as_uint(e1.w) & mask) == 0u) continue;
-        const float4 v0 = tp[0], e2 = tp[2];
+        const float4 v0 = gload(tp), e2 = gload(tp + 2);
         float t, u, v;
         if (tri_test(r, v0, e1, e2, &t, &u, &v)) {
           const uint32_t id = __float_as_uint(v0.w);
