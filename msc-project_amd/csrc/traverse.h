@@ -235,11 +235,10 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
       const uint32_t first = leaf_first(node), count = leaf_count(node);
       for (uint32_t k = 0; k < count; ++k) {
         const SWR_GLOBAL gf4* tp = gtris + 3 * (size_t)(first + k);
-        const float4 e1 = gload(tp + 1);
-        if ((__float_as_uint(e1.w) & mask) == 0u) continue;
-        const float4 v0 = gload(tp), e2 = gload(tp + 2);
+        // The mask is checked after the test, so that the triangle's three rows load together.
+        const float4 v0 = gload(tp), e1 = gload(tp + 1), e2 = gload(tp + 2);
         float t, u, v;
-        if (tri_test(r, v0, e1, e2, &t, &u, &v)) {
+        if (tri_test(r, v0, e1, e2, &t, &u, &v) && (__float_as_uint(e1.w) & mask) != 0u) {
           const uint32_t id = __float_as_uint(v0.w);
           if (kAny) {
             best.t = t;
@@ -345,11 +344,10 @@ struct Walker {
       const uint32_t first = leaf_first(node), count = leaf_count(node);
       for (uint32_t k = 0; k < count; ++k) {
         const SWR_GLOBAL gf4* tp = global_f4(leaf_tris) + 3 * (size_t)(first + k);
-        const float4 e1 = gload(tp + 1);
-        if ((__float_as_uint(e1.w) & mask) == 0u) continue;
-        const float4 v0 = gload(tp), e2 = gload(tp + 2);
+        // The mask is checked after the test, so that the triangle's three rows load together.
+        const float4 v0 = gload(tp), e1 = gload(tp + 1), e2 = gload(tp + 2);
         float t, u, v;
-        if (tri_test(r, v0, e1, e2, &t, &u, &v)) {
+        if (tri_test(r, v0, e1, e2, &t, &u, &v) && (__float_as_uint(e1.w) & mask) != 0u) {
           const uint32_t id = __float_as_uint(v0.w);
           if (kAny) {
             best.t = t;
