@@ -153,6 +153,19 @@ struct ShortStack {
     if (kDist) dist[(sp & (kShort - 1)) * kBlock] = (uint16_t)(tbits >> 16);
     ++sp;
   }
+  // Makes room for n unchecked pushes: spills the oldest ring entries while the ring would
+  // overflow, exactly the spills n checked pushes would make.
+  __device__ void reserve(int n) {
+    while (sp - base + n > kShort) {
+      spill[base] = lds[(base & (kShort - 1)) * kBlock];
+      ++base;
+    }
+  }
+  __device__ void push_reserved(uint32_t v, uint32_t tbits = 0) {
+    lds[(sp & (kShort - 1)) * kBlock] = v;
+    if (kDist) dist[(sp & (kShort - 1)) * kBlock] = (uint16_t)(tbits >> 16);
+    ++sp;
+  }
   __device__ bool empty() const { return sp == 0; }
   __device__ uint32_t pop() {
     --sp;
@@ -178,11 +191,13 @@ struct ShortStack {
 // Traverses one ray. kAny: returns on the first hit (id of that triangle), else the closest.
 // lds_col: this lane's LDS ring column (kShort entries, stride kBlock); spill_col: this lane's
 // HBM spill column (stack_need(depth) entries; unused when the ring is deep enough).
+// kPairTris: leaf triangles are tested two per step with their loads together (fewer memory
+// round trips, more registers: worth it where the kernel has registers to spare).
 // dist_col (closest hit only): this lane's ring of entry distances. A popped node whose entry
 // distance exceeds the closest hit so far by more than the box slack is skipped unfetched:
 // that is the parent's box test run again with the current hit distance, so the result is the
 // one without the skip.
-template <int kBlock, int kShort, bool kAny>
+template <int kBlock, int kShort, bool kAny, bool kPairTris = false>
 __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const float4* __restrict__ leaf_tris,
                                      const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col,
                                      uint16_t* dist_col = nullptr) {
@@ -223,37 +238,52 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
         const uint32_t cs[4] = {ch.x, ch.y, ch.z, ch.w};
         // Push hit children far to near except the nearest (keys ascend, so a hit k3 implies
         // hits k0..k2), then descend into the nearest.
-        if ((k3 & ~3u) < miss_key) st.push(cs[k3 & 3u], k3 & ~3u);
-        if ((k2 & ~3u) < miss_key) st.push(cs[k2 & 3u], k2 & ~3u);
-        if ((k1 & ~3u) < miss_key) st.push(cs[k1 & 3u], k1 & ~3u);
+        const bool h1 = (k1 & ~3u) < miss_key, h2 = (k2 & ~3u) < miss_key, h3 = (k3 & ~3u) < miss_key;
+        st.reserve((int)h1 + (int)h2 + (int)h3);
+        if (h3) st.push_reserved(cs[k3 & 3u], k3 & ~3u);
+        if (h2) st.push_reserved(cs[k2 & 3u], k2 & ~3u);
+        if (h1) st.push_reserved(cs[k1 & 3u], k1 & ~3u);
         if ((k0 & ~3u) < miss_key) {
           node = cs[k0 & 3u];
           continue;
         }
       }
     } else {
+      // With kPairTris, two triangles per step, loaded together; with an odd count the second load
+      // repeats the first triangle's address and its test is skipped. Tests run in leaf order, so
+      // the first hit of an any-hit query and the tie order of a closest-hit query are those of one
+      // at a time.
       const uint32_t first = leaf_first(node), count = leaf_count(node);
-      for (uint32_t k = 0; k < count; ++k) {
-        const SWR_GLOBAL gf4* tp = gtris + 3 * (size_t)(first + k);
-        // The mask is checked after the test, so that the triangle's three rows load together.
-        const float4 v0 = gload(tp), e1 = gload(tp + 1), e2 = gload(tp + 2);
-        float t, u, v;
-        if (tri_test(r, v0, e1, e2, &t, &u, &v) && (__float_as_uint(e1.w) & mask) != 0u) {
-          const uint32_t id = __float_as_uint(v0.w);
-          if (kAny) {
-            best.t = t;
-            best.u = u;
-            best.v = v;
-            best.id = id;
-            best.leaf = first + k;
-            return best;
-          }
-          if (best.id == kMiss || t < best.t || (t == best.t && id < best.id)) {
-            best.t = t;
-            best.u = u;
-            best.v = v;
-            best.id = id;
-            best.leaf = first + k;
+      for (uint32_t k = 0; k < count; k += kPairTris ? 2 : 1) {
+        const bool two = kPairTris && k + 1 < count;
+        const SWR_GLOBAL gf4* ta = gtris + 3 * (size_t)(first + k);
+        const SWR_GLOBAL gf4* tb = two ? ta + 3 : ta;
+        const float4 va = gload(ta), ea = gload(ta + 1), fa = gload(ta + 2);
+        float4 vb = va, eb = ea, fb = fa;
+        if (kPairTris) vb = gload(tb), eb = gload(tb + 1), fb = gload(tb + 2);
+#pragma unroll
+        for (int s = 0; s < (kPairTris ? 2 : 1); ++s) {
+          if (s == 1 && !two) break;
+          const float4 v0 = s ? vb : va, e1 = s ? eb : ea, e2 = s ? fb : fa;
+          float t, u, v;
+          // The mask is checked after the test, so that the triangle's rows load together.
+          if (tri_test(r, v0, e1, e2, &t, &u, &v) && (__float_as_uint(e1.w) & mask) != 0u) {
+            const uint32_t id = __float_as_uint(v0.w);
+            if (kAny) {
+              best.t = t;
+              best.u = u;
+              best.v = v;
+              best.id = id;
+              best.leaf = first + k + s;
+              return best;
+            }
+            if (best.id == kMiss || t < best.t || (t == best.t && id < best.id)) {
+              best.t = t;
+              best.u = u;
+              best.v = v;
+              best.id = id;
+              best.leaf = first + k + s;
+            }
           }
         }
       }
