@@ -255,12 +255,12 @@ __global__ __launch_bounds__(kBlock, kOne ? 1 : 6) void shade_kernel(const Devic
       }
     } else {
       const float t = __uint_as_float(rec.id.z);
-      const float4* lt = s.leaf_tris + 3 * (size_t)leaf;
-      const float4 e1 = lt[1], e2 = lt[2];
-      const GeomRec g = s.geoms[__float_as_uint(e2.w)];
+      const SWR_GLOBAL gf4* lt = global_f4(s.leaf_tris) + 3 * (size_t)leaf;
+      const float4 e1 = gload(lt + 1), e2 = gload(lt + 2);
+      const GeomRec g = gload_rec(s.geoms + __float_as_uint(e2.w));
       const V3 dn = normalize(v3(rec.d.x, rec.d.y, rec.d.z));
       if (g.light >= 0) {  // light hit (Integrator.cpp:37-75)
-        const LightRec& L = s.lights[g.light];
+        const LightRec L = gload_rec(s.lights + g.light);
         const float c = dot(v3(L.n[0], L.n[1], L.n[2]), dn * -1.f);  // QuadLight::radiance
         const V3 le = c < 0.f ? v3(0.f, 0.f, 0.f) : v3(L.intensity, L.intensity, L.intensity);
         const float pdfa = 1.f / (L.sx * L.sy);
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kBlock, kOne ? 1 : 6) void shade_kernel(const Devic
           contrib = v3(le.x * mis * w.x, le.y * mis * w.y, le.z * mis * w.z);
         }
       } else {
-        const ShaderRec sh = s.shaders[g.shader];
+        const ShaderRec sh = gload_rec(s.shaders + g.shader);
         // Ng = (v0 - v1) x (v2 - v0), the leaf's e1 x e2; N = -normalize(Ng) (Integrator.cpp:121-125)
         const V3 nrm = normalize(cross(v3(e1.x, e1.y, e1.z), v3(e2.x, e2.y, e2.z))) * -1.f;
         P = v3(rec.o.x, rec.o.y, rec.o.z) + dn * t;  // Integrator.cpp:126
@@ -285,7 +285,8 @@ __global__ __launch_bounds__(kBlock, kOne ? 1 : 6) void shade_kernel(const Devic
           const uint32_t l = min(p.num_lights - 1u, (uint32_t)((float)p.num_lights * r.r[0]));
           V3 in, le;
           float dist, pdfw;
-          light_illuminate(s.lights[l], r.r[1], r.r[2], P, &in, &dist, &le, &pdfw);
+          const LightRec Ll = gload_rec(s.lights + l);
+          light_illuminate(Ll, r.r[1], r.r[2], P, &in, &dist, &le, &pdfw);
           if (maxc(le) > kEps && isfinite(pdfw)) {
             const Eval e = bsdf_eval(sh, in, out_dir, nrm);
             if (maxc(e.f) > kEps) {

@@ -94,6 +94,19 @@ __device__ inline float4 gload(const SWR_GLOBAL gf4* p) {
   return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ inline const SWR_GLOBAL gf4* global_f4(const float4* p) { return (const SWR_GLOBAL gf4*)p; }
+// A scene record (a multiple of 16 bytes, 16-byte aligned in its array) as global loads.
+template <typename T>
+__device__ inline T gload_rec(const T* p) {
+  static_assert(sizeof(T) % 16 == 0, "record size must be a multiple of 16 bytes");
+  T v;
+  const SWR_GLOBAL gu4* src = (const SWR_GLOBAL gu4*)p;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 16); ++i) {
+    const gu4 c = src[i];
+    __builtin_memcpy(reinterpret_cast<char*>(&v) + 16 * i, &c, 16);
+  }
+  return v;
+}
 __device__ inline const SWR_GLOBAL gf4* global_nodes(const Node4* p) { return (const SWR_GLOBAL gf4*)p; }
 
 // Entry distances of the four children of a node; +inf for a missed box or an unused slot.
