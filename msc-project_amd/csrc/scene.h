@@ -116,7 +116,7 @@ struct swr_scene {
   std::vector<swr_quad_light_desc> light_descs;
   float env[3] = {0.f, 0.f, 0.f};
   bool committed = false;
-  int leaf_max = 4;
+  int leaf_max = 2;
   int tuned_ring = 0;  // traversal stack ring chosen for this scene by the batch autotuner (0 = none)
 
   // committed
