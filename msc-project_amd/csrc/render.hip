@@ -406,11 +406,7 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const DeviceScene* 
       }
     }
   };
-  if (kOne) {
-    if (pending) body(0u);
-  } else {
-    SWR_BY_UNIFORM(pending, scene, s0, { body(s0); })
-  }
+  if (pending) body(kOne ? 0u : scene);
 }
 
 // After a wave: every job admitted before the oldest job that still has a live path (a
