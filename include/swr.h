@@ -230,8 +230,8 @@ int swr_raygen(swr_context* ctx, const swr_render_desc* desc, uint32_t sid0, siz
                float* xy_dev);
 /* One iteration: every sample traced through the sorted wavefront loop, reduced per sample and
  * filtered. pixels: host W*H*3 floats, the filtered iteration (row-major y*W+x as Pixel in
- * Image.h:16-20); samples: optional host W*H*spp*5 floats (x, y, r, g, b) in reference sampleID
- * order (Camera.cpp:28). stats may be NULL. */
+ * Image.h:16-20), or NULL to leave it in device memory; samples: optional host W*H*spp*5 floats
+ * (x, y, r, g, b) in reference sampleID order (Camera.cpp:28). stats may be NULL. */
 int swr_render(swr_context* ctx, const swr_scene* scene, const swr_render_desc* desc, float* pixels, float* samples,
                swr_stats* stats);
 

@@ -699,8 +699,9 @@ void LoopBuffers::release(swr_context* ctx) {
   host_state = nullptr;
   for (auto& e : snap_ev)
     if (e) (void)hipEventDestroy(e), e = nullptr;
-  for (auto& e : stage_ev)
-    if (e) (void)hipEventDestroy(e), e = nullptr;
+  for (auto& row : stage_ev)
+    for (auto& e : row)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
 }
 
 uint64_t WaveLoop::job_samples(const swr_render_desc& d) {
@@ -766,7 +767,8 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
   if (!b_.host_state) {
     SWR_HIP(hipHostMalloc((void**)&b_.host_state, 8 * sizeof(WaveState), hipHostMallocDefault));
     for (auto& e : b_.snap_ev) SWR_HIP(hipEventCreate(&e));
-    for (auto& e : b_.stage_ev) SWR_HIP(hipEventCreate(&e));
+    for (auto& row : b_.stage_ev)
+      for (auto& e : row) SWR_HIP(hipEventCreate(&e));
   }
   std::vector<DeviceScene> tab;
   for (const swr_scene* s : scenes) tab.push_back(s->dev);
@@ -839,7 +841,7 @@ void WaveLoop::retire_ranges(uint64_t cursor) {
   }
 }
 
-void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times) {
+void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times, int ev_set) {
   hipStream_t st = stream_;
   const uint32_t nb = exact ? n : cap_, kb = exact ? k : cap_;
   WaveState* w = static_cast<WaveState*>(b_.wave.ptr);
@@ -858,7 +860,7 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
   const uint32_t mask = range_cap_ - 1;
   const bool filter = ring_set_.size() > 1;
   auto mark = [&](int i) {
-    if (times) SWR_HIP(hipEventRecord(b_.stage_ev[i], st));
+    if (times) SWR_HIP(hipEventRecord(b_.stage_ev[ev_set][i], st));
   };
 
   wave_begin_kernel<<<1, 1, 0, st>>>(w);
@@ -946,23 +948,55 @@ void WaveLoop::run_paced(StageTimes* times) {
     enqueue_wave(cur_, true, n, k, times);
     SWR_HIP(hipMemcpyAsync(hs, b_.wave.ptr, sizeof(WaveState), hipMemcpyDeviceToHost, stream_));
     SWR_HIP(hipStreamSynchronize(stream_));
-    if (times) {
-      auto ms = [&](int a, int b) {
-        float v = 0.f;
-        SWR_HIP(hipEventElapsedTime(&v, b_.stage_ev[a], b_.stage_ev[b]));
-        return (double)v;
-      };
-      times->raygen += ms(0, 1);
-      times->sort_rays += ms(1, 2);
-      times->trace += ms(2, 3);
-      times->sort_hits += ms(3, 4);
-      times->shade += ms(4, 5);
-      times->shadow += ms(5, 6);
-    }
+    add_stage_times(times, 0);
     n_cont = (uint32_t)hs->counters;
     cursor += k;
     cur_ ^= 1;
   }
+}
+
+void WaveLoop::add_stage_times(StageTimes* times, int ev_set) {
+  if (!times) return;
+  auto ms = [&](int a, int b) {
+    float v = 0.f;
+    SWR_HIP(hipEventElapsedTime(&v, b_.stage_ev[ev_set][a], b_.stage_ev[ev_set][b]));
+    return (double)v;
+  };
+  times->raygen += ms(0, 1);
+  times->sort_rays += ms(1, 2);
+  times->trace += ms(2, 3);
+  times->sort_hits += ms(3, 4);
+  times->shade += ms(4, 5);
+  times->shadow += ms(5, 6);
+}
+
+void WaveLoop::run_ahead(StageTimes* times) {
+  int pending = -1, pending_set = 0;
+  for (uint64_t wave = 0;; ++wave) {
+    if (ctx_->cancel.load()) {
+      SWR_HIP(hipStreamSynchronize(stream_));
+      fail(SWR_E_CANCELLED, "render cancelled");
+    }
+    // Sorts sized on the device skip their own epoch check.
+    sort_epoch_guard(ctx_, 2ull * (uint64_t)passes_per_wave_, stream_);
+    const int set = (int)(wave % 3);
+    enqueue_wave(cur_, false, cap_, cap_, times, set);
+    cur_ ^= 1;
+    const int t = snapshot();
+    if (pending >= 0) {
+      const WaveState& s = wait_snapshot(pending);  // the previous wave has finished
+      add_stage_times(times, pending_set);
+      if (s.cursor == s.total && s.total == total_ && (uint32_t)s.counters == 0) {
+        pending = t;
+        pending_set = set;
+        break;
+      }
+    }
+    pending = t;
+    pending_set = set;
+  }
+  SWR_HIP(hipStreamSynchronize(stream_));
+  add_stage_times(times, pending_set);  // the empty wave after the last
 }
 
 void WaveLoop::capture() {
@@ -1110,7 +1144,7 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   StageTimes times;
   SWR_HIP(hipEventRecord(ev[0], st));
   loop.admit(0, p);
-  loop.run_paced(&times);
+  loop.run_ahead(&times);
   SWR_HIP(hipEventRecord(ev[1], st));
   loop.filter(p);
   SWR_HIP(hipEventRecord(ev[2], st));

@@ -58,7 +58,7 @@ struct LoopBuffers {
   DeviceBuffer radiance, sample_xy, pixels, staging;  // arenas, when the driver keeps them here
   WaveState* host_state = nullptr;                     // pinned snapshot slots
   hipEvent_t snap_ev[8] = {};
-  hipEvent_t stage_ev[8] = {};
+  hipEvent_t stage_ev[3][8] = {};  // per-stage events of up to three waves in flight
   void release(swr_context* ctx);
 };
 
@@ -94,10 +94,17 @@ class WaveLoop {
   // One wave: raygen refill, ray sort, trace, hit sort, shade, shadow. With `exact` the wave
   // size is known on the host and launches are sized to it; otherwise they are sized for the
   // capacity and read the size on the device (the form captured into graphs).
-  void enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times);
+  // With `times`, per-stage events are recorded into event set `ev_set` (0..2).
+  void enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times, int ev_set = 0);
+  // Adds the stage times of a finished wave recorded into event set `ev_set`.
+  void add_stage_times(StageTimes* times, int ev_set);
 
   // Host-paced loop until every admitted path has finished; per-stage times if asked.
   void run_paced(StageTimes* times);
+  // Run-ahead loop: device-sized waves enqueued without host round trips, each checked through
+  // a snapshot one wave behind, so the device always has the next wave queued; per-stage times
+  // (read once a wave is known finished) if asked. One empty wave runs after the last.
+  void run_ahead(StageTimes* times);
 
   // Persistent loop: captures the wave for both queue parities once, then replays graphs until
   // every admitted path has finished (checked every `check_every` waves, two checks in flight).

@@ -458,14 +458,16 @@ class Scene:
             d_r.free()
             d_f.free()
 
-    def render(self, desc: RenderDesc, samples: bool = False):
+    def render(self, desc: RenderDesc, samples: bool = False, pixels: bool = True):
         """One iteration (Pathtracer::process): filtered pixels (H, W, 3) and optionally the
-        per-sample (x, y, r, g, b) records; returns (pixels, samples or None, stats dict)."""
+        per-sample (x, y, r, g, b) records; returns (pixels or None, samples or None, stats dict).
+        pixels=False leaves the image in device memory (no copy to the host)."""
         spp = desc.spp if desc.spp > 0 else desc.base * desc.base
-        pix = np.zeros((desc.height, desc.width, 3), np.float32)
+        pix = np.zeros((desc.height, desc.width, 3), np.float32) if pixels else None
         smp = np.zeros((desc.width * desc.height * spp, 5), np.float32) if samples else None
         st = Stats()
-        self.ctx.check(self.lib.swr_render(self.ctx.h, self.h, C.byref(desc), pix.ctypes.data,
+        self.ctx.check(self.lib.swr_render(self.ctx.h, self.h, C.byref(desc),
+                                           pix.ctypes.data if pix is not None else None,
                                            smp.ctypes.data if smp is not None else None, C.byref(st)))
         return pix, smp, st.as_dict()
 
