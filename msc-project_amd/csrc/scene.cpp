@@ -172,6 +172,9 @@ void commit_scene(swr_scene* s) {
   SWR_HIP(hipStreamSynchronize(ctx->stream));
   if (stack_need(s->bvh.max_depth) > 3 * max_supported_depth() + 4)
     fail(SWR_E_INVALID, "scene: BVH depth " + std::to_string(s->bvh.max_depth) + " exceeds the traversal stack");
+  // Traversal addresses nodes by 32-bit byte offsets (128-byte nodes).
+  if (s->bvh.num_nodes >= (1u << 25))
+    fail(SWR_E_INVALID, "scene: " + std::to_string(s->bvh.num_nodes) + " BVH nodes; at most 2^25 - 1");
 
   DeviceScene& d = s->dev;
   d.pos = static_cast<const float4*>(s->d_pos.ptr);

@@ -109,21 +109,60 @@ __device__ inline T gload_rec(const T* p) {
 }
 __device__ inline const SWR_GLOBAL gf4* global_nodes(const Node4* p) { return (const SWR_GLOBAL gf4*)p; }
 
+// Byte offsets, within a node, of the near and far plane rows of each axis for one ray: the lo
+// row when the ray's inverse direction on that axis is >= 0, else the hi row. (lo - o) * inv and
+// (hi - o) * inv are ordered like lo and hi (or reversed for a negative inverse), since both steps
+// are monotone, so the near/far distances equal the min/max of the two: the same box test with
+// the per-axis min and max replaced by a per-ray choice of row.
+struct PlaneRows {
+  uint32_t nx, fx, ny, fy, nz, fz;
+};
+__device__ inline PlaneRows plane_rows(const RayState& r) {
+  const uint32_t sx = r.ix < 0.f ? 16u : 0u, sy = r.iy < 0.f ? 16u : 0u, sz = r.iz < 0.f ? 16u : 0u;
+  return PlaneRows{sx, 16u - sx, 32u + sy, 48u - sy, 64u + sz, 80u - sz};
+}
+
+// Slab entry distance of one child box from its near and far planes, or +inf when missed.
+__device__ inline float slab_nf(const RayState& r, float nx, float fx, float ny, float fy, float nz, float fz,
+                                float tfar) {
+  const float tnx = (nx - r.ox) * r.ix, tfx = (fx - r.ox) * r.ix;
+  const float tny = (ny - r.oy) * r.iy, tfy = (fy - r.oy) * r.iy;
+  const float tnz = (nz - r.oz) * r.iz, tfz = (fz - r.oz) * r.iz;
+  const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
+  const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tfar));
+  return tn <= tf * kBoxSlack ? tn : __int_as_float(0x7F800000);
+}
+
 // Entry distances of the four children of a node; +inf for a missed box or an unused slot.
 // All four box tests run unconditionally and unused slots are masked afterwards, so the node's
-// 128 bytes are fetched in one round of loads rather than behind per-slot branches.
-// nd: the node as 8 16-byte rows (lox, hix, loy, hiy, loz, hiz, child, unused).
-__device__ inline void node_entries(const RayState& r, const SWR_GLOBAL gf4* nd, float tfar, uint4* ch,
-                                    float* t0, float* t1, float* t2, float* t3) {
-  const float4 lx = gload(nd + 0), hx = gload(nd + 1), ly = gload(nd + 2), hy = gload(nd + 3), lz = gload(nd + 4),
-               hz = gload(nd + 5);
-  const gu4 c = *(const SWR_GLOBAL gu4*)(nd + 6);
+// rows are fetched in one round of loads rather than behind per-slot branches. The node is the
+// 128 bytes at base + node_off (rows lox, hix, loy, hiy, loz, hiz, child, unused); the offsets
+// are 32-bit, so the loads take the form with a scalar base and a per-lane offset.
+// kNearFar: each lane loads its ray's near and far rows (six per-lane offsets, fewer VALU
+// operations, more registers); otherwise all six rows at fixed offsets and per-axis min/max.
+template <bool kNearFar>
+__device__ inline void node_entries(const RayState& r, const PlaneRows& pr, const SWR_GLOBAL char* base,
+                                    uint32_t node_off, float tfar, uint4* ch, float* t0, float* t1, float* t2,
+                                    float* t3) {
+  auto row = [&](uint32_t off) { return gload((const SWR_GLOBAL gf4*)(base + (node_off + off))); };
+  const gu4 c = *(const SWR_GLOBAL gu4*)(base + (node_off + 96u));
   *ch = make_uint4(c.x, c.y, c.z, c.w);
   const float inf = __int_as_float(0x7F800000);
-  const float s0 = slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tfar);
-  const float s1 = slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tfar);
-  const float s2 = slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tfar);
-  const float s3 = slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tfar);
+  float s0, s1, s2, s3;
+  if (kNearFar) {
+    const float4 nx = row(pr.nx), fx = row(pr.fx), ny = row(pr.ny), fy = row(pr.fy), nz = row(pr.nz),
+                 fz = row(pr.fz);
+    s0 = slab_nf(r, nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, tfar);
+    s1 = slab_nf(r, nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, tfar);
+    s2 = slab_nf(r, nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, tfar);
+    s3 = slab_nf(r, nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, tfar);
+  } else {
+    const float4 lx = row(0), hx = row(16), ly = row(32), hy = row(48), lz = row(64), hz = row(80);
+    s0 = slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tfar);
+    s1 = slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tfar);
+    s2 = slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tfar);
+    s3 = slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tfar);
+  }
   *t0 = ch->x == kNoChild ? inf : s0;
   *t1 = ch->y == kNoChild ? inf : s1;
   *t2 = ch->z == kNoChild ? inf : s2;
@@ -205,12 +244,14 @@ struct ShortStack {
 // lds_col: this lane's LDS ring column (kShort entries, stride kBlock); spill_col: this lane's
 // HBM spill column (stack_need(depth) entries; unused when the ring is deep enough).
 // kPairTris: leaf triangles are tested two per step with their loads together (fewer memory
-// round trips, more registers: worth it where the kernel has registers to spare).
+// round trips, more registers: worth it where the kernel has registers to spare). kNearFar: box
+// tests from the ray's near and far planes (see node_entries; same results, fewer operations,
+// more registers).
 // dist_col (closest hit only): this lane's ring of entry distances. A popped node whose entry
 // distance exceeds the closest hit so far by more than the box slack is skipped unfetched:
 // that is the parent's box test run again with the current hit distance, so the result is the
 // one without the skip.
-template <int kBlock, int kShort, bool kAny, bool kPairTris = false>
+template <int kBlock, int kShort, bool kAny, bool kPairTris = false, bool kNearFar = false>
 __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const float4* __restrict__ leaf_tris,
                                      const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col,
                                      uint16_t* dist_col = nullptr) {
@@ -221,15 +262,16 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
   best.leaf = kMiss;
   const float inf = __int_as_float(0x7F800000);
   ShortStack<kBlock, kShort, !kAny> st(lds_col, spill_col, dist_col);
-  const SWR_GLOBAL gf4* gnodes = global_nodes(nodes);
+  const SWR_GLOBAL char* gnodes = (const SWR_GLOBAL char*)global_nodes(nodes);
   const SWR_GLOBAL gf4* gtris = global_f4(leaf_tris);
+  const PlaneRows pr = plane_rows(r);
   uint32_t node = 0;  // root
   for (;;) {
     if (!(node & kLeafBit)) {
       // Unused slots (kNoChild) are never entered, whatever their box.
       uint4 ch;
       float t0, t1, t2, t3;
-      node_entries(r, gnodes + 8 * (size_t)node, best.t, &ch, &t0, &t1, &t2, &t3);
+      node_entries<kNearFar>(r, pr, gnodes, node * 128u, best.t, &ch, &t0, &t1, &t2, &t3);
       if (kAny) {
         if (t0 != inf) st.push(ch.x);
         if (t1 != inf) st.push(ch.y);
@@ -353,7 +395,8 @@ struct Walker {
       // Unused slots (kNoChild) are never entered, whatever their box.
       uint4 ch;
       float t0, t1, t2, t3;
-      node_entries(r, global_nodes(nodes) + 8 * (size_t)node, best.t, &ch, &t0, &t1, &t2, &t3);
+      node_entries<true>(r, plane_rows(r), (const SWR_GLOBAL char*)global_nodes(nodes), node * 128u, best.t, &ch,
+                         &t0, &t1, &t2, &t3);
       if (kAny) {
         if (t0 != inf) st.push(ch.x);
         if (t1 != inf) st.push(ch.y);
