@@ -247,7 +247,7 @@ int swr_batch_autotune(swr_batch* b, uint32_t wave_rays, swr_tuning* out) {
           t.sort_kpt = kpt;
           trial(t);
         }
-      for (int rb : {32, 24, 20, 16}) {
+      for (int rb : {32, 24, 20, 16, 12, 8}) {
         swr::Tuning t = best;
         t.ray_key_bits = rb;
         trial(t);

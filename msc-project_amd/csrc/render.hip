@@ -1237,7 +1237,7 @@ int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc
         const double ms = time_it(t);
         if (ms < best_ms) best_ms = ms, best = t;
       }
-    for (int rb : {32, 24, 20, 16}) {
+    for (int rb : {32, 24, 20, 16, 12, 8}) {
       swr::Tuning t = best;
       t.ray_key_bits = rb;
       const double ms = time_it(t);

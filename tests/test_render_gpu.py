@@ -147,6 +147,22 @@ def test_converged_images_4096spp(ctx, which):
     assert np.max(z) < 6.0
 
 
+def test_render_without_host_pixels(ctx):
+    """pixels=None leaves the image on the device: same work (ray counts) as a render that copies
+    it, and a render with a copy right after still gives the oracle's image."""
+    desc = scenes.box_room(width=48, height=32, base=3, sphere_level=2)
+    sc, oc = swr.Scene(ctx, desc), orc.Scene(desc)
+    rd, od = _pair(desc, iteration=2)
+    pix0, _, st0 = sc.render(rd, pixels=False)
+    assert pix0 is None and st0["camera_rays"] == 48 * 32 * 9
+    pix, smp, st = sc.render(rd, samples=True)
+    for k in ("camera_rays", "closest_rays", "shadow_rays", "waves"):
+        assert st0[k] == st[k], k
+    _, osmp, _ = oc.render(od, samples=True)
+    _compare_samples(smp, osmp, 0.02)
+    assert np.isfinite(pix).all()
+
+
 def test_render_errors_and_cancel(ctx):
     desc = scenes.box_room(width=16, height=16, base=2, sphere_level=1)
     sc = swr.Scene(ctx, desc)
