@@ -252,7 +252,7 @@ int swr_batch_autotune(swr_batch* b, uint32_t wave_rays, swr_tuning* out) {
         t.ray_key_bits = rb;
         trial(t);
       }
-      for (int hb : {0, 24, 20, 16}) {
+      for (int hb : {0, 24, 20, 16, 12, 8}) {
         swr::Tuning t = best;
         t.hit_key_bits = hb;
         trial(t);

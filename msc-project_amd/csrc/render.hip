@@ -1244,8 +1244,8 @@ int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc
       if (ms < best_ms) best_ms = ms, best = t;
     }
     const int full = swr::bits_for((uint64_t)scene->dev.num_tris);
-    for (int hb : {0, full - 6, full - 12}) {
-      if (hb < 0 || (hb > 0 && hb < 4)) continue;
+    for (int hb : {0, full - 6, full - 12, 8}) {
+      if (hb < 0 || (hb > 0 && hb < 4) || (hb == 8 && full - 12 <= 8)) continue;
       swr::Tuning t = best;
       t.hit_key_bits = hb;
       const double ms = time_it(t);
