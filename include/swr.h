@@ -88,6 +88,24 @@ typedef struct {
   float roughness;       /* Glossy only: GGX alpha */
 } swr_shader_desc;
 
+/* Texture nodes: the reference's TextureInterface implementations. CONSTANT is ConstantTexture
+ * (ConstantTexture.h convert<>); IMAGE is StandardTexture (StandardTexture.cpp:11-43), an image
+ * looked up at the hit's interpolated texture coordinates with periodic wrap in s and t and zero
+ * filter width, as OpenImageIO's texture() does there: texel centres at (i + 0.5) / size and
+ * uniform cubic B-spline weights over the 4 x 4 texels around (s, t); LAYERED is LayeredTexture
+ * (LayeredTexture.cpp:14-32): upper * mask + lower * (1 - mask). At most 8 colours may be
+ * pending while one shader's texture graph is evaluated (each LAYERED node holds its upper and
+ * lower colours while its mask is evaluated); deeper graphs fail at commit. */
+#define SWR_TEXTURE_CONSTANT 0
+#define SWR_TEXTURE_IMAGE 1
+#define SWR_TEXTURE_LAYERED 2
+typedef struct {
+  int32_t type;
+  float colour[3];              /* CONSTANT */
+  int32_t image;                /* IMAGE: index returned by swr_scene_add_image */
+  int32_t upper, lower, mask;   /* LAYERED: indices of earlier texture nodes */
+} swr_texture_desc;
+
 /* Quad light record (QuadLight.h convert<>). */
 typedef struct {
   float translation[3];
@@ -184,6 +202,15 @@ int swr_scene_create(swr_context* ctx, swr_scene** out);
 void swr_scene_destroy(swr_scene* scene);
 /* Shaders in document order; returns the shader index used by swr_scene_add_mesh. */
 int swr_scene_add_shader(swr_scene* scene, const swr_shader_desc* desc);
+/* An RGB float image for IMAGE textures (the OpenImageIO texture file of StandardTexture, decoded):
+ * width x height texels, rows from t = 0 down, 3 floats each. Returns the image index. */
+int swr_scene_add_image(swr_scene* scene, int32_t width, int32_t height, const float* rgb);
+/* A texture node (LAYERED nodes name earlier nodes); returns the node index. */
+int swr_scene_add_texture(swr_scene* scene, const swr_texture_desc* desc);
+/* Gives a shader a texture (LambertShader's texture, Integrator.cpp:80-99): its colour at a hit is
+ * the texture at (u, v) = (1 - s - t) uv0 + s uv1 + t uv2 of the hit triangle's vertex uvs and hit
+ * barycentrics (PolygonObject.cpp:45-63), instead of the shader's constant colour. */
+int swr_scene_set_shader_texture(swr_scene* scene, int32_t shader, int32_t texture);
 /* rtcNewTriangleMesh + rtcSetBuffer (Pathtracer.cpp:178-201): float4 positions (w ignored),
  * u32x3 indices, optional float2 uv per vertex. Returns the object index (= geomID). */
 int swr_scene_add_mesh(swr_scene* scene, const float* pos4, uint32_t nverts, const uint32_t* idx3, uint32_t ntris,

@@ -235,6 +235,68 @@ __device__ inline void camera_ray(const FrameParams& p, uint32_t sid, V3* org, V
   *dir = normalize(focal - *org);
 }
 
+// ---- textures --------------------------------------------------------------------------------
+// StandardTexture::initialize (StandardTexture.cpp:11-41) looks each hit's texture coordinate up
+// with OpenImageIO's texture(): periodic wrap in s and t, zero derivatives. At zero filter width
+// that samples the finest level with the bicubic filter: texel centres at (i + 0.5) / size and
+// uniform cubic B-spline weights over the 4 x 4 texels around the point. t = 0 is the first row.
+struct TexAxis {
+  uint32_t i[4];  // texel indices, wrapped
+  float w[4];     // B-spline weights
+};
+__device__ inline TexAxis tex_axis(float s, uint32_t n) {
+  if (!isfinite(s)) s = 0.f;
+  const float x = (s - floorf(s)) * (float)n - 0.5f;  // in [-0.5, n - 0.5]
+  const float fx = floorf(x);
+  const float f = x - fx;
+  const int i0 = (int)fx - 1 + 2 * (int)n;            // >= 0 for n >= 1
+  const float g = 1.f - f, f2 = f * f, f3 = f2 * f;
+  TexAxis a;
+  a.w[0] = g * g * g * (1.f / 6.f);
+  a.w[1] = (3.f * f3 - 6.f * f2 + 4.f) * (1.f / 6.f);
+  a.w[2] = (-3.f * f3 + 3.f * f2 + 3.f * f + 1.f) * (1.f / 6.f);
+  a.w[3] = f3 * (1.f / 6.f);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) a.i[k] = (uint32_t)(i0 + k) % n;
+  return a;
+}
+__device__ inline V3 tex_lookup(const DeviceScene& sc, int32_t image, float s, float t) {
+  const TexImage im = sc.tex_images[image];
+  const TexAxis ax = tex_axis(s, im.width), ay = tex_axis(t, im.height);
+  V3 acc = v3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float4* row = sc.texels + im.offset + (uint64_t)ay.i[j] * im.width;
+    V3 r = v3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 c = row[ax.i[i]];
+      r = r + v3(c.x, c.y, c.z) * ax.w[i];
+    }
+    acc = acc + r * ay.w[j];
+  }
+  return acc;
+}
+// A shader's texture colour at (s, t): its post-order program run on a stack of colours.
+// LayeredTexture::initialize (LayeredTexture.cpp:14-32): upper * mask + lower * (1 - mask).
+__device__ inline V3 texture_colour(const DeviceScene& sc, const ShaderRec& sh, float s, float t) {
+  V3 st[kTexStack];
+  int sp = 0;
+  for (int k = 0; k < sh.tex_len; ++k) {
+    const TexOp op = sc.tex_ops[sh.tex_ofs + k];
+    if (op.op == SWR_TEXTURE_LAYERED) {
+      const V3 m = st[sp - 1], lo = st[sp - 2], up = st[sp - 3];
+      sp -= 3;
+      st[sp++] = v3(up.x * m.x + lo.x * (1.f - m.x), up.y * m.y + lo.y * (1.f - m.y), up.z * m.z + lo.z * (1.f - m.z));
+    } else if (op.op == SWR_TEXTURE_IMAGE) {
+      st[sp++] = tex_lookup(sc, op.image, s, t);
+    } else {
+      st[sp++] = v3(op.colour[0], op.colour[1], op.colour[2]);
+    }
+  }
+  return st[0];
+}
+
 // ---- queue records ----------------------------------------------------------------------------
 // One queued ray is one 64-byte line: origin + tmin, direction + tmax, throughput + lastPdf, and
 // id = (sample id within its job, job << 16 | depth, scene index until traced then hit t, hit

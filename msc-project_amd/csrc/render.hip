@@ -216,7 +216,9 @@ constexpr int kShadeBlockMulti = 512;
 //
 // kOne: the loop holds one job on one scene, whose constants are then wave-uniform (scalar
 // loads). Otherwise each lane reads its job's constants and scene pointers.
-template <bool kOne, int kBlock>
+// kTex: some scene of the loop has textured shaders (a separate instance, so that untextured
+// loops do not carry the texture code's registers and stack).
+template <bool kOne, int kBlock, bool kTex>
 __global__ __launch_bounds__(kBlock, kOne ? 1 : 6) void shade_kernel(const DeviceScene* __restrict__ scenes,
                                                             const FrameParams* __restrict__ jobs,
                                                             const uint32_t* __restrict__ perm, WaveState* w,
@@ -274,7 +276,26 @@ __global__ __launch_bounds__(kBlock, kOne ? 1 : 6) void shade_kernel(const Devic
           contrib = v3(le.x * mis * w.x, le.y * mis * w.y, le.z * mis * w.z);
         }
       } else {
-        const ShaderRec sh = gload_rec(s.shaders + g.shader);
+        ShaderRec sh = gload_rec(s.shaders + g.shader);
+        if (kTex && sh.tex_len > 0) {
+          // The texture coordinate of the hit (PolygonObject::texture, PolygonObject.cpp:45-63)
+          // from the triangle's barycentrics, recomputed as traversal computed them.
+          const float4 v0 = gload(lt);
+          float bu, bv;
+          tri_uv(rec.o, rec.d, v0, e1, e2, &bu, &bv);
+          float tu = 0.f, tv = 0.f;
+          if (s.uv) {
+            const uint4 tr = s.tri[__float_as_uint(v0.w)];
+            const float2 a = s.uv[tr.x], b = s.uv[tr.y], c = s.uv[tr.z];
+            const float w0 = 1.f - bu - bv;
+            tu = w0 * a.x + bu * b.x + bv * c.x;
+            tv = w0 * a.y + bu * b.y + bv * c.y;
+          }
+          const V3 col = texture_colour(s, sh, tu, tv);
+          sh.colour[0] = col.x;
+          sh.colour[1] = col.y;
+          sh.colour[2] = col.z;
+        }
         // Ng = (v0 - v1) x (v2 - v0), the leaf's e1 x e2; N = -normalize(Ng) (Integrator.cpp:121-125)
         const V3 nrm = normalize(cross(v3(e1.x, e1.y, e1.z), v3(e2.x, e2.y, e2.z))) * -1.f;
         P = v3(rec.o.x, rec.o.y, rec.o.z) + dn * t;  // Integrator.cpp:126
@@ -740,6 +761,7 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
                                         : choose_ring(scenes[i]->tuned_ring, scenes[i]->dev.max_depth);
     rings_.push_back(ring);
     ring_tab[i] = (uint8_t)ring;
+    textured_ |= scenes[i]->textured;
     if (std::find(ring_set_.begin(), ring_set_.end(), ring) == ring_set_.end()) ring_set_.push_back(ring);
     if ((uint32_t)ring < stack_need(scenes[i]->dev.max_depth)) spill_ = true;
   }
@@ -908,11 +930,16 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
     so.shadow = static_cast<float4*>(b_.shadow.ptr);
     so.radiance = radiance_;
     so.block_min = static_cast<uint32_t*>(b_.block_min.ptr);
-    if (one_job)
-      shade_kernel<true, kShadeBlock><<<shade_blocks, kShadeBlock, 0, st>>>(stab, jobs, hperm, w, queue[cur], so);
+    if (one_job && textured_)
+      shade_kernel<true, kShadeBlock, true><<<shade_blocks, kShadeBlock, 0, st>>>(stab, jobs, hperm, w, queue[cur], so);
+    else if (one_job)
+      shade_kernel<true, kShadeBlock, false><<<shade_blocks, kShadeBlock, 0, st>>>(stab, jobs, hperm, w, queue[cur], so);
+    else if (textured_)
+      shade_kernel<false, kShadeBlockMulti, true><<<shade_blocks, kShadeBlockMulti, 0, st>>>(stab, jobs, hperm, w,
+                                                                                             queue[cur], so);
     else
-      shade_kernel<false, kShadeBlockMulti><<<shade_blocks, kShadeBlockMulti, 0, st>>>(stab, jobs, hperm, w,
-                                                                                       queue[cur], so);
+      shade_kernel<false, kShadeBlockMulti, false><<<shade_blocks, kShadeBlockMulti, 0, st>>>(stab, jobs, hperm, w,
+                                                                                              queue[cur], so);
     mark(5);
     for (int ring : ring_set_) {
       with_ring(ring, [&](auto kShort) {

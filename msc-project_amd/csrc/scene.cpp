@@ -90,13 +90,57 @@ void commit_scene(swr_scene* s) {
   s->geoms.clear();
   s->all_shaders.clear();
   s->lights.clear();
-  for (const auto& d : s->shaders) {
+  // Texture programs: each textured shader's graph in post order (children before parents).
+  std::vector<TexOp> ops;
+  s->textured = false;
+  for (size_t i = 0; i < s->shaders.size(); ++i) {
+    const swr_shader_desc& d = s->shaders[i];
     ShaderRec r{};
     r.type = d.type;
     std::memcpy(r.colour, d.colour, sizeof r.colour);
     r.reflectance = d.reflectance;
     r.roughness = d.roughness;
+    const int32_t root = i < s->shader_texture.size() ? s->shader_texture[i] : -1;
+    if (root >= 0) {
+      r.tex_ofs = (int32_t)ops.size();
+      int depth = 0, max_depth = 0;
+      // Post order by an explicit stack of (node, children visited).
+      std::vector<std::pair<int32_t, bool>> todo{{root, false}};
+      while (!todo.empty()) {
+        auto [node, expanded] = todo.back();
+        todo.pop_back();
+        const swr_texture_desc& t = s->textures[node];
+        if (t.type == SWR_TEXTURE_LAYERED && !expanded) {
+          todo.push_back({node, true});
+          todo.push_back({t.mask, false});
+          todo.push_back({t.lower, false});
+          todo.push_back({t.upper, false});
+          continue;
+        }
+        TexOp op{};
+        op.op = t.type;
+        std::memcpy(op.colour, t.colour, sizeof op.colour);
+        op.image = t.image;
+        ops.push_back(op);
+        if (ops.size() - (size_t)r.tex_ofs > kTexOpsMax)
+          fail(SWR_E_INVALID, "scene: texture of shader " + std::to_string(i) + " expands to more than " +
+                                  std::to_string(kTexOpsMax) + " nodes");
+        depth += t.type == SWR_TEXTURE_LAYERED ? -2 : 1;
+        max_depth = std::max(max_depth, depth);
+      }
+      if (max_depth > kTexStack)
+        fail(SWR_E_INVALID, "scene: texture of shader " + std::to_string(i) + " nests layers too deeply");
+      r.tex_len = (int32_t)ops.size() - r.tex_ofs;
+      s->textured = true;
+    }
     s->all_shaders.push_back(r);
+  }
+  std::vector<TexImage> images;
+  std::vector<float4> texels;
+  for (const auto& im : s->images) {
+    images.push_back(TexImage{(uint32_t)im.width, (uint32_t)im.height, (uint64_t)texels.size()});
+    for (size_t k = 0; k < (size_t)im.width * im.height; ++k)
+      texels.push_back(make_float4(im.rgb[3 * k], im.rgb[3 * k + 1], im.rgb[3 * k + 2], 0.f));
   }
   std::vector<float4> pos;
   std::vector<uint4> tri;
@@ -167,6 +211,9 @@ void commit_scene(swr_scene* s) {
   upload(ctx, s->d_geoms, s->geoms);
   upload(ctx, s->d_shaders, s->all_shaders);
   upload(ctx, s->d_lights, s->lights);
+  if (!ops.empty()) upload(ctx, s->d_tex_ops, ops);
+  if (!images.empty()) upload(ctx, s->d_tex_images, images);
+  if (!texels.empty()) upload(ctx, s->d_texels, texels);
   build_bvh(ctx, static_cast<const float4*>(s->d_pos.ptr), static_cast<const uint4*>(s->d_tri.ptr),
             static_cast<const GeomRec*>(s->d_geoms.ptr), (uint32_t)tri.size(), clo, chi, s->leaf_max, &s->bvh);
   SWR_HIP(hipStreamSynchronize(ctx->stream));
@@ -185,6 +232,9 @@ void commit_scene(swr_scene* s) {
   d.lights = static_cast<const LightRec*>(s->d_lights.ptr);
   d.nodes = static_cast<const Node4*>(s->bvh.nodes.ptr);
   d.leaf_tris = static_cast<const float4*>(s->bvh.leaf_tris.ptr);
+  d.tex_ops = ops.empty() ? nullptr : static_cast<const TexOp*>(s->d_tex_ops.ptr);
+  d.tex_images = images.empty() ? nullptr : static_cast<const TexImage*>(s->d_tex_images.ptr);
+  d.texels = texels.empty() ? nullptr : static_cast<const float4*>(s->d_texels.ptr);
   d.num_tris = (uint32_t)tri.size();
   d.num_geoms = (uint32_t)s->geoms.size();
   d.num_lights = (uint32_t)s->lights.size();
@@ -200,7 +250,7 @@ void commit_scene(swr_scene* s) {
 
 void swr::detach_scene(swr_scene* s) {
   for (DeviceBuffer* b : {&s->d_pos, &s->d_tri, &s->d_uv, &s->d_geoms, &s->d_shaders, &s->d_lights, &s->bvh.nodes,
-                          &s->bvh.leaf_tris})
+                          &s->bvh.leaf_tris, &s->d_tex_ops, &s->d_tex_images, &s->d_texels})
     *b = DeviceBuffer{};
   s->dev = DeviceScene{};
   s->committed = false;
@@ -240,7 +290,7 @@ void swr_scene_destroy(swr_scene* s) {
   if (swr_context* ctx = s->ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (swr::DeviceBuffer* b : {&s->d_pos, &s->d_tri, &s->d_uv, &s->d_geoms, &s->d_shaders, &s->d_lights,
-                                 &s->bvh.nodes, &s->bvh.leaf_tris}) {
+                                 &s->bvh.nodes, &s->bvh.leaf_tris, &s->d_tex_ops, &s->d_tex_images, &s->d_texels}) {
       try {
         swr::release(ctx, *b);
       } catch (const Error&) {
@@ -280,6 +330,53 @@ int swr_scene_add_mesh(swr_scene* s, const float* pos4, uint32_t nverts, const u
   m.shader = shader;
   s->objects.push_back(std::move(m));
   return (int)s->objects.size() - 1;
+}
+
+int swr_scene_add_image(swr_scene* s, int32_t width, int32_t height, const float* rgb) {
+  if (!s || !s->ctx) return SWR_E_INVALID;
+  if (s->committed || width <= 0 || height <= 0 || !rgb || (int64_t)width * height > (int64_t(1) << 28)) {
+    s->ctx->err = "scene_add_image: scene committed, empty image, null texels or more than 2^28 texels";
+    return SWR_E_INVALID;
+  }
+  swr_scene::Image im;
+  im.width = width;
+  im.height = height;
+  im.rgb.assign(rgb, rgb + 3 * (size_t)width * height);
+  for (float v : im.rgb)
+    if (!std::isfinite(v)) {
+      s->ctx->err = "scene_add_image: texels must be finite";
+      return SWR_E_INVALID;
+    }
+  s->images.push_back(std::move(im));
+  return (int)s->images.size() - 1;
+}
+
+int swr_scene_add_texture(swr_scene* s, const swr_texture_desc* d) {
+  if (!s || !d || !s->ctx) return SWR_E_INVALID;
+  const int32_t n = (int32_t)s->textures.size();
+  const bool ok = !s->committed &&
+                  ((d->type == SWR_TEXTURE_CONSTANT) ||
+                   (d->type == SWR_TEXTURE_IMAGE && d->image >= 0 && d->image < (int32_t)s->images.size()) ||
+                   (d->type == SWR_TEXTURE_LAYERED && d->upper >= 0 && d->upper < n && d->lower >= 0 && d->lower < n &&
+                    d->mask >= 0 && d->mask < n));
+  if (!ok) {
+    s->ctx->err = "scene_add_texture: scene committed, unknown type, or an image / layer index out of range";
+    return SWR_E_INVALID;
+  }
+  s->textures.push_back(*d);
+  return n;
+}
+
+int swr_scene_set_shader_texture(swr_scene* s, int32_t shader, int32_t texture) {
+  if (!s || !s->ctx) return SWR_E_INVALID;
+  if (s->committed || shader < 0 || shader >= (int32_t)s->shaders.size() || texture < -1 ||
+      texture >= (int32_t)s->textures.size()) {
+    s->ctx->err = "scene_set_shader_texture: scene committed or shader / texture index out of range";
+    return SWR_E_INVALID;
+  }
+  if (s->shader_texture.size() < s->shaders.size()) s->shader_texture.resize(s->shaders.size(), -1);
+  s->shader_texture[shader] = texture;
+  return SWR_OK;
 }
 
 int swr_scene_add_quad_light(swr_scene* s, const swr_quad_light_desc* d) {

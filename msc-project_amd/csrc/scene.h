@@ -51,8 +51,26 @@ struct ShaderRec {
   float colour[3];
   float reflectance;
   float roughness;
-  float pad[2];
+  int32_t tex_ofs;  // texture program: tex_len ops from tex_ops[tex_ofs] (tex_len 0: constant colour)
+  int32_t tex_len;
 };
+
+// A shader's texture graph compiled to a post-order stack program: CONSTANT and IMAGE push a
+// colour, LAYERED pops mask, lower, upper and pushes upper * mask + lower * (1 - mask).
+constexpr int kTexStack = 8;        // deepest stack a program may use
+constexpr size_t kTexOpsMax = 4096;  // longest program (shared subgraphs are expanded)
+struct TexOp {
+  int32_t op;       // SWR_TEXTURE_*
+  float colour[3];  // CONSTANT
+  int32_t image;    // IMAGE
+  int32_t pad[3];
+};
+struct TexImage {
+  uint32_t width, height;
+  uint64_t offset;  // first texel in the texel array (float4 per texel, rgb + unused)
+};
+static_assert(sizeof(ShaderRec) == 32 && sizeof(TexOp) == 32 && sizeof(TexImage) == 16,
+              "scene records are read as 16-byte rows");
 
 // QuadLight after construct() (QuadLight.cpp:5-39): transform T*R*S(1,sx,sy) and normal.
 struct LightRec {
@@ -73,6 +91,9 @@ struct DeviceScene {
   const LightRec* lights = nullptr;
   const Node4* nodes = nullptr;
   const float4* leaf_tris = nullptr;  // 3 float4 per triangle in BVH order
+  const TexOp* tex_ops = nullptr;
+  const TexImage* tex_images = nullptr;
+  const float4* texels = nullptr;
   uint32_t num_tris = 0, num_geoms = 0, num_lights = 0, num_nodes = 0;
   uint32_t max_depth = 0;             // BVH4 depth (root = 0)
   float env[3] = {0.f, 0.f, 0.f};
@@ -113,6 +134,14 @@ struct swr_scene {
   };
   std::vector<Mesh> objects;
   std::vector<swr_shader_desc> shaders;
+  std::vector<int32_t> shader_texture;  // per shader: texture node or -1
+  std::vector<swr_texture_desc> textures;
+  struct Image {
+    int32_t width = 0, height = 0;
+    std::vector<float> rgb;
+  };
+  std::vector<Image> images;
+  bool textured = false;  // committed: some shader has a texture
   std::vector<swr_quad_light_desc> light_descs;
   float env[3] = {0.f, 0.f, 0.f};
   bool committed = false;
@@ -123,7 +152,7 @@ struct swr_scene {
   std::vector<swr::GeomRec> geoms;
   std::vector<swr::ShaderRec> all_shaders;
   std::vector<swr::LightRec> lights;
-  swr::DeviceBuffer d_pos, d_tri, d_uv, d_geoms, d_shaders, d_lights;
+  swr::DeviceBuffer d_pos, d_tri, d_uv, d_geoms, d_shaders, d_lights, d_tex_ops, d_tex_images, d_texels;
   swr::BvhBuild bvh;
   swr::DeviceScene dev;
 };

@@ -80,6 +80,23 @@ __device__ inline bool tri_test(const RayState& r, float4 v0, float4 e1, float4 
   return true;
 }
 
+// The (u, v) that tri_test returns for a ray known to hit the triangle (same arithmetic, so the
+// same bits): u weights vertex 1, v vertex 2.
+__device__ inline void tri_uv(float4 o, float4 d, float4 v0, float4 e1, float4 e2, float* u, float* v) {
+  const float ngx = e1.y * e2.z - e1.z * e2.y;
+  const float ngy = e1.z * e2.x - e1.x * e2.z;
+  const float ngz = e1.x * e2.y - e1.y * e2.x;
+  const float cx = v0.x - o.x, cy = v0.y - o.y, cz = v0.z - o.z;
+  const float rx = d.y * cz - d.z * cy;
+  const float ry = d.z * cx - d.x * cz;
+  const float rz = d.x * cy - d.y * cx;
+  const float den = ngx * d.x + ngy * d.y + ngz * d.z;
+  const float sgn = den < 0.f ? -1.f : 1.f;
+  const float inv = 1.f / fabsf(den);
+  *u = ((rx * e2.x + ry * e2.y + rz * e2.z) * sgn) * inv;
+  *v = ((rx * e1.x + ry * e1.y + rz * e1.z) * sgn) * inv;
+}
+
 // Node and triangle pointers come out of the scene table in memory, so the compiler cannot tell
 // their address space and would issue flat loads; flat loads also count against the LDS wait
 // counter, so every stack access would wait for the node fetches in flight. These casts make

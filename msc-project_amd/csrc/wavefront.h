@@ -147,6 +147,7 @@ class WaveLoop {
   int scene_bits_ = 0, hit_tri_bits_ = 0, ray_bits_ = 32, hit_bits_ = 0, hit_shift_ = 0;
   uint32_t spill_stride_ = 0;
   bool spill_ = false;
+  bool textured_ = false;         // some scene has textured shaders
   uint64_t total_ = 0;            // paths admitted
   uint32_t seq_ = 0, num_ranges_ = 0, range_lo_ = 0, range_cap_ = 0;
   float4* radiance_ = nullptr;

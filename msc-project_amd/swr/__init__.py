@@ -61,6 +61,20 @@ class QuadLightDesc(C.Structure):
                 ("intensity", C.c_float)]
 
 
+class TextureDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("colour", C.c_float * 3), ("image", C.c_int32), ("upper", C.c_int32),
+                ("lower", C.c_int32), ("mask", C.c_int32)]
+
+
+TEXTURE_CONSTANT, TEXTURE_IMAGE, TEXTURE_LAYERED = 0, 1, 2
+
+
+def texture_desc(t: dict, cls=TextureDesc):
+    """A texture node dict {type, colour, image, upper, lower, mask} as the C record."""
+    return cls(t["type"], (C.c_float * 3)(*t.get("colour", (0.0, 0.0, 0.0))), t.get("image", -1),
+               t.get("upper", -1), t.get("lower", -1), t.get("mask", -1))
+
+
 class Tuning(C.Structure):
     _fields_ = [("sort_keys_per_thread", C.c_int32), ("ray_key_bits", C.c_int32), ("hit_key_bits", C.c_int32),
                 ("trace_stack_entries", C.c_int32), ("sort_algorithm", C.c_int32)]
@@ -132,6 +146,9 @@ _SIG = {
     "swr_scene_add_shader": (C.c_int, [_P, C.POINTER(ShaderDesc)]),
     "swr_scene_add_mesh": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_int32]),
     "swr_scene_add_quad_light": (C.c_int, [_P, C.POINTER(QuadLightDesc)]),
+    "swr_scene_add_image": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
+    "swr_scene_add_texture": (C.c_int, [_P, C.POINTER(TextureDesc)]),
+    "swr_scene_set_shader_texture": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "swr_scene_set_environment": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "swr_scene_commit": (C.c_int, [_P]),
     "swr_scene_load_yaml": (C.c_int, [_P, C.c_char_p, C.POINTER(_P), C.POINTER(RenderDesc)]),
@@ -346,6 +363,13 @@ class Scene:
         if desc is not None:
             for sh in desc["shaders"]:
                 self.add_shader(sh)
+            for im in desc.get("images", []):
+                self.add_image(im)
+            for t in desc.get("textures", []):
+                self.add_texture(t)
+            for i, sh in enumerate(desc["shaders"]):
+                if sh.get("texture", -1) >= 0:
+                    self.set_shader_texture(i, sh["texture"])
             for m in desc["meshes"]:
                 self.add_mesh(m["pos4"], m["idx"], m.get("uv"), m["shader"])
             for l in desc["lights"]:
@@ -362,6 +386,20 @@ class Scene:
         d = ShaderDesc(sh["type"], (C.c_float * 3)(*sh["colour"]), sh.get("reflectance", 1.0),
                        sh.get("roughness", 0.0))
         return self._rc(self.lib.swr_scene_add_shader(self.h, C.byref(d)))
+
+    def add_image(self, rgb) -> int:
+        """An (H, W, 3) float image; row 0 is t = 0."""
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        if rgb.ndim != 3 or rgb.shape[2] != 3:
+            raise ValueError("image must be (height, width, 3)")
+        return self._rc(self.lib.swr_scene_add_image(self.h, rgb.shape[1], rgb.shape[0], rgb.ctypes.data))
+
+    def add_texture(self, t: dict) -> int:
+        d = texture_desc(t)
+        return self._rc(self.lib.swr_scene_add_texture(self.h, C.byref(d)))
+
+    def set_shader_texture(self, shader: int, texture: int):
+        self._rc(self.lib.swr_scene_set_shader_texture(self.h, shader, texture))
 
     def add_mesh(self, pos4, idx, uv, shader: int) -> int:
         pos4 = np.ascontiguousarray(pos4, dtype=np.float32)

@@ -201,6 +201,49 @@ def terrain(seed: int = 0x5EED0002, grid: int = 1024, groups: int = 64, width: i
             "render": render}
 
 
+def textured_room(seed: int = 0x5EED0006, width: int = 256, height: int = 256, base: int = 2, max_depth: int = 5,
+                  sphere_level: int = 4):
+    """The box room with image and layered textures (StandardTexture, LayeredTexture,
+    ConstantTexture): a seeded checker image on the floor with texture coordinates scaled to
+    [-0.3, 2.7] (so the lookup wraps), a layered back wall (a gradient over a constant colour,
+    mixed by a seeded noise mask), a left wall whose layer nests another layered texture in its
+    mask, and a noise image on the sphere (whose texture seam wraps in s). Images are seeded
+    float RGB in [0, 1]."""
+    rng = np.random.default_rng(seed)
+    room = box_room(seed=seed, width=width, height=height, base=base, max_depth=max_depth,
+                    sphere_level=sphere_level)
+    shaders = [dict(sh) for sh in room["shaders"]]
+    meshes = [dict(m) for m in room["meshes"]]
+    checker = np.zeros((16, 16, 3), np.float32)
+    cells = rng.uniform(0.15, 0.9, (2, 3)).astype(np.float32)
+    yy, xx = np.mgrid[0:16, 0:16]
+    checker[:] = np.where(((xx // 4 + yy // 4) % 2 == 0)[..., None], cells[0], cells[1])
+    gy, gx = np.mgrid[0:24, 0:40] / np.array([23.0, 39.0])[:, None, None]
+    gradient = np.stack([gx, gy, 1.0 - gx], axis=-1).astype(np.float32) * 0.8 + 0.1
+    mask = rng.uniform(0.0, 1.0, (8, 8, 3)).astype(np.float32)
+    noise = rng.uniform(0.1, 0.9, (32, 64, 3)).astype(np.float32)
+    images = [checker, gradient, mask, noise]
+    textures = [
+        {"type": 1, "image": 0},                                   # 0 checker
+        {"type": 1, "image": 1},                                   # 1 gradient
+        {"type": 0, "colour": rng.uniform(0.2, 0.8, 3).tolist()},  # 2 constant
+        {"type": 1, "image": 2},                                   # 3 mask
+        {"type": 2, "upper": 1, "lower": 2, "mask": 3},            # 4 gradient over constant
+        {"type": 0, "colour": [0.8, 0.2, 0.2]},                    # 5 constant
+        {"type": 2, "upper": 5, "lower": 0, "mask": 4},            # 6 nested layer in the mask
+        {"type": 1, "image": 3},                                   # 7 noise
+    ]
+    names = [m["name"] for m in meshes]
+    floor = meshes[names.index("floor")]
+    floor["uv"] = floor["uv"] * 3.0 - 0.3
+    shaders[floor["shader"]]["texture"] = 0
+    shaders[meshes[names.index("back")]["shader"]]["texture"] = 4
+    shaders[meshes[names.index("left")]["shader"]]["texture"] = 6
+    shaders[meshes[names.index("sphere")]["shader"]]["texture"] = 7
+    return {"shaders": shaders, "meshes": meshes, "lights": room["lights"], "environment": (0.0, 0.0, 0.0),
+            "images": images, "textures": textures, "render": room["render"]}
+
+
 def sphere_cluster(seed: int, count: int, level: int, width: int = 256, height: int = 256, base: int = 2,
                    max_depth: int = 6, bin_exponent: int = 23):
     """An icosphere cluster in the box room: `count` spheres of 20 * 4^level triangles with

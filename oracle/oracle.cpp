@@ -149,6 +149,13 @@ struct oracle_scene {
   std::vector<Mesh> objects;
   std::vector<oracle_shader_desc> shaders;
   std::vector<oracle_quad_light_desc> light_descs;
+  struct Image {
+    int32_t width, height;
+    std::vector<float> rgb;
+  };
+  std::vector<Image> images;
+  std::vector<oracle_texture_desc> textures;
+  std::vector<int32_t> shader_texture;           // per shader: texture node or -1
   V3 env{0.f, 0.f, 0.f};
   bool committed = false;
 
@@ -160,6 +167,7 @@ struct oracle_scene {
   std::vector<int32_t> geom_light;               // light index or -1
   std::vector<uint32_t> geom_first_tri;          // prefix of triangle counts
   std::vector<Tri> tris;                         // global triangle order
+  std::vector<float> tri_uv;                     // 6 floats per triangle: uv of v0, v1, v2
   std::vector<int32_t> tri_geom;
   std::vector<Node> nodes;
   std::vector<uint32_t> order;                   // BVH leaf order -> global triangle id
@@ -608,6 +616,78 @@ void filter_pixel(int32_t filter, int32_t W, int32_t H, int32_t spp, const float
   out[2] = sum[2] / den;
 }
 
+// ---- textures (StandardTexture.cpp:11-41, LayeredTexture.cpp:14-32, ConstantTexture.cpp:22-25) ----
+// OpenImageIO's texture() with WrapPeriodic and zero derivatives samples the finest level with
+// its bicubic filter: texel centres at (i + 0.5) / n, uniform cubic B-spline weights
+//   (1-f)^3/6, (3f^3 - 6f^2 + 4)/6, (-3f^3 + 3f^2 + 3f + 1)/6, f^3/6
+// over texels i-1 .. i+2 around x = s * n - 0.5 = i + f, indices taken modulo n. The coordinate
+// is first reduced to [0, 1] (s - floor(s)), which the periodic wrap leaves unchanged.
+void bspline_axis(float s, int32_t n, int32_t idx[4], float w[4]) {
+  if (!std::isfinite(s)) s = 0.f;
+  const float x = (s - std::floor(s)) * (float)n - 0.5f;
+  const float fx = std::floor(x);
+  const float f = x - fx;
+  const float g = 1.f - f, f2 = f * f, f3 = f2 * f;
+  w[0] = g * g * g * (1.f / 6.f);
+  w[1] = (3.f * f3 - 6.f * f2 + 4.f) * (1.f / 6.f);
+  w[2] = (-3.f * f3 + 3.f * f2 + 3.f * f + 1.f) * (1.f / 6.f);
+  w[3] = f3 * (1.f / 6.f);
+  const int32_t i = (int32_t)fx;
+  for (int k = 0; k < 4; ++k) {
+    int32_t j = (i - 1 + k) % n;
+    idx[k] = j < 0 ? j + n : j;
+  }
+}
+
+V3 image_lookup(const float* rgb, int32_t width, int32_t height, float s, float t) {
+  int32_t xi[4], yi[4];
+  float xw[4], yw[4];
+  bspline_axis(s, width, xi, xw);
+  bspline_axis(t, height, yi, yw);
+  V3 acc = v3(0.f, 0.f, 0.f);
+  for (int j = 0; j < 4; ++j) {
+    V3 row = v3(0.f, 0.f, 0.f);
+    for (int i = 0; i < 4; ++i) {
+      const float* c = rgb + 3 * ((size_t)yi[j] * width + xi[i]);
+      row = row + v3(c[0], c[1], c[2]) * xw[i];
+    }
+    acc = acc + row * yw[j];
+  }
+  return acc;
+}
+
+V3 texture_eval(const oracle_scene* s, int32_t node, float tu, float tv) {
+  const oracle_texture_desc& t = s->textures[node];
+  if (t.type == 0) return v3(t.colour[0], t.colour[1], t.colour[2]);
+  if (t.type == 1) {
+    const oracle_scene::Image& im = s->images[t.image];
+    return image_lookup(im.rgb.data(), im.width, im.height, tu, tv);
+  }
+  const V3 up = texture_eval(s, t.upper, tu, tv);
+  const V3 lo = texture_eval(s, t.lower, tu, tv);
+  const V3 m = texture_eval(s, t.mask, tu, tv);
+  const V3 inv = v3(1.f - m.x, 1.f - m.y, 1.f - m.z);
+  return mul(up, m) + mul(lo, inv);
+}
+
+// The shader at a hit with its texture applied: the texture coordinate of PolygonObject::texture
+// (PolygonObject.cpp:45-63) from the hit barycentrics, then the texture's colour replaces the
+// shader's constant colour (LambertShader.cpp:43, :70).
+oracle_shader_desc shade_with_texture(const oracle_scene* s, int32_t shader, uint32_t prim, float u, float v) {
+  oracle_shader_desc sh = s->all_shaders[shader];
+  const int32_t node = shader < (int32_t)s->shader_texture.size() ? s->shader_texture[shader] : -1;
+  if (node < 0) return sh;
+  const float* uv = &s->tri_uv[6 * (size_t)prim];
+  const float w0 = 1.f - u - v;
+  const float tu = w0 * uv[0] + u * uv[2] + v * uv[4];
+  const float tv = w0 * uv[1] + u * uv[3] + v * uv[5];
+  const V3 c = texture_eval(s, node, tu, tv);
+  sh.colour[0] = c.x;
+  sh.colour[1] = c.y;
+  sh.colour[2] = c.z;
+  return sh;
+}
+
 // One path (Integrator.cpp:12-255 restated per ray, path at a time).
 V3 trace_path(const oracle_scene* s, const oracle_render_desc* d, uint32_t sid, V3 org, V3 dir, uint64_t* n_closest,
               uint64_t* n_shadow) {
@@ -643,7 +723,7 @@ V3 trace_path(const oracle_scene* s, const oracle_render_desc* d, uint32_t sid, 
       if (maxc(le) > kEps) L = L + v3(le.x * mis * w.x, le.y * mis * w.y, le.z * mis * w.z);
       break;
     }
-    const oracle_shader_desc& sh = s->all_shaders[s->geom_shader[g]];
+    const oracle_shader_desc sh = shade_with_texture(s, s->geom_shader[g], prim, u, v);
     const Tri& tr = s->tris[prim];
     V3 ng = cross(tr.v0 - tr.v1, tr.v2 - tr.v0);
     V3 n = normalize(ng) * -1.f;  // Integrator.cpp:121-125
@@ -767,6 +847,53 @@ int oracle_scene_add_quad_light(oracle_scene* s, const oracle_quad_light_desc* d
 
 void oracle_scene_set_environment(oracle_scene* s, const float rgb[3]) { s->env = v3(rgb[0], rgb[1], rgb[2]); }
 
+int oracle_scene_add_image(oracle_scene* s, int32_t width, int32_t height, const float* rgb) {
+  if (!s || s->committed || width <= 0 || height <= 0 || !rgb) return -1;
+  oracle_scene::Image im;
+  im.width = width;
+  im.height = height;
+  im.rgb.assign(rgb, rgb + 3 * (size_t)width * height);
+  s->images.push_back(std::move(im));
+  return (int)s->images.size() - 1;
+}
+
+int oracle_scene_add_texture(oracle_scene* s, const oracle_texture_desc* d) {
+  if (!s || !d || s->committed) return -1;
+  const int32_t n = (int32_t)s->textures.size();
+  if (d->type == 1 && (d->image < 0 || d->image >= (int32_t)s->images.size())) return -1;
+  if (d->type == 2 && (d->upper < 0 || d->upper >= n || d->lower < 0 || d->lower >= n || d->mask < 0 || d->mask >= n))
+    return -1;
+  if (d->type < 0 || d->type > 2) return -1;
+  s->textures.push_back(*d);
+  return n;
+}
+
+int oracle_scene_set_shader_texture(oracle_scene* s, int32_t shader, int32_t texture) {
+  if (!s || s->committed || shader < 0 || shader >= (int32_t)s->shaders.size() || texture < -1 ||
+      texture >= (int32_t)s->textures.size())
+    return -1;
+  if (s->shader_texture.size() < s->shaders.size()) s->shader_texture.resize(s->shaders.size(), -1);
+  s->shader_texture[shader] = texture;
+  return 0;
+}
+
+void oracle_texture_lookup(const float* rgb, int32_t width, int32_t height, float s, float t, float out[3]) {
+  const V3 c = image_lookup(rgb, width, height, s, t);
+  out[0] = c.x;
+  out[1] = c.y;
+  out[2] = c.z;
+}
+
+int oracle_texture_colour(const oracle_scene* s, int32_t shader, uint32_t prim, float u, float v, float out[3]) {
+  if (!s || !s->committed || shader < 0 || shader >= (int32_t)s->all_shaders.size() || prim >= s->tris.size())
+    return -1;
+  const oracle_shader_desc sh = shade_with_texture(s, shader, prim, u, v);
+  out[0] = sh.colour[0];
+  out[1] = sh.colour[1];
+  out[2] = sh.colour[2];
+  return 0;
+}
+
 int oracle_scene_commit(oracle_scene* s) {
   if (!s || s->committed) return -1;
   for (const auto& m : s->objects)
@@ -777,6 +904,7 @@ int oracle_scene_commit(oracle_scene* s) {
   s->geom_light.clear();
   s->geom_first_tri.clear();
   s->tris.clear();
+  s->tri_uv.clear();
   s->tri_geom.clear();
   // Pass 1: objects (Pathtracer.cpp:166-217)
   for (size_t g = 0; g < s->objects.size(); ++g) {
@@ -795,6 +923,11 @@ int oracle_scene_commit(oracle_scene* s) {
       t.v2 = v3(c[0], c[1], c[2]);
       s->tris.push_back(t);
       s->tri_geom.push_back((int32_t)g);
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t vi = m.idx[3 * i + k];
+        s->tri_uv.push_back(m.uv.empty() ? 0.f : m.uv[2 * vi]);
+        s->tri_uv.push_back(m.uv.empty() ? 0.f : m.uv[2 * vi + 1]);
+      }
     }
   }
   // Pass 2: lights, each with a placeholder object, a Null shader and mask 0xF0000000
@@ -820,6 +953,7 @@ int oracle_scene_commit(oracle_scene* s) {
       t.v2 = L.xf(v3(quad[3 * k + 2][0], quad[3 * k + 2][1], quad[3 * k + 2][2]));
       s->tris.push_back(t);
       s->tri_geom.push_back(g);
+      s->tri_uv.insert(s->tri_uv.end(), 6, 0.f);
     }
     s->lights.push_back(L);
   }

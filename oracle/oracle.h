@@ -83,6 +83,16 @@ void oracle_hit_keys(const uint32_t* prim, size_t n, uint32_t* keys);
 /* Stable sort: perm[i] = original index of the i-th smallest key (std::stable_sort). */
 void oracle_stable_sort_perm(const uint32_t* keys, size_t n, uint32_t* perm);
 
+/* Texture nodes (the reference's TextureInterface implementations, decoded by the
+ * convert<> specialisations of ConstantTexture.h, StandardTexture.h and LayeredTexture.h):
+ * 0 Constant (colour), 1 Standard (an image), 2 Layered (indices of earlier nodes). */
+typedef struct {
+  int32_t type;
+  float colour[3];
+  int32_t image;
+  int32_t upper, lower, mask;
+} oracle_texture_desc;
+
 /* ---- scene ----------------------------------------------------------------------------- */
 typedef struct oracle_scene oracle_scene;
 oracle_scene* oracle_scene_create(void);
@@ -93,6 +103,11 @@ int oracle_scene_add_shader(oracle_scene* s, const oracle_shader_desc* d);
 int oracle_scene_add_mesh(oracle_scene* s, const float* pos4, uint32_t nverts,
                           const uint32_t* idx3, uint32_t ntris, const float* uv, int32_t shader);
 int oracle_scene_add_quad_light(oracle_scene* s, const oracle_quad_light_desc* d);
+/* Texture images (width x height RGB floats, row 0 at t = 0), texture nodes and the texture of
+ * a shader (LambertShader's texture, LambertShader.h convert<>). Return an index or -1. */
+int oracle_scene_add_image(oracle_scene* s, int32_t width, int32_t height, const float* rgb);
+int oracle_scene_add_texture(oracle_scene* s, const oracle_texture_desc* d);
+int oracle_scene_set_shader_texture(oracle_scene* s, int32_t shader, int32_t texture);
 void oracle_scene_set_environment(oracle_scene* s, const float rgb[3]);
 /* Lays out geomIDs as the reference does (objects, then one mesh per light; Pathtracer.cpp:166-272)
  * and builds a binned-SAH BVH. */
@@ -109,6 +124,14 @@ int oracle_intersect_brute(const oracle_scene* s, const float* rays, size_t n, u
                            uint32_t* hits);
 /* Any hit in (tmin, tmax) (Integrator.cpp:153-174 -> rtcOccluded): flags[i] = 1 if occluded. */
 int oracle_occluded(const oracle_scene* s, const float* rays, size_t n, uint32_t mask, uint32_t* flags);
+
+/* StandardTexture's image lookup (StandardTexture.cpp:11-41, OpenImageIO texture() with periodic
+ * wrap and zero derivatives): cubic B-spline over the 4 x 4 texels around (s, t). */
+void oracle_texture_lookup(const float* rgb, int32_t width, int32_t height, float s, float t, float out[3]);
+/* Colour of a committed scene's shader at barycentrics (u, v) of global triangle prim: the texture
+ * coordinate of PolygonObject::texture (PolygonObject.cpp:45-63), then the shader's texture
+ * (its constant colour when it has none). */
+int oracle_texture_colour(const oracle_scene* s, int32_t shader, uint32_t prim, float u, float v, float out[3]);
 
 /* ---- render ---------------------------------------------------------------------------- */
 /* Camera rays for sample ids [sid0, sid0+n) (Camera.cpp:6-59): rays n x 8, film xy n x 2. */

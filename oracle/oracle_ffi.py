@@ -35,6 +35,11 @@ class QuadLightDesc(C.Structure):
                 ("intensity", C.c_float)]
 
 
+class TextureDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("colour", C.c_float * 3), ("image", C.c_int32), ("upper", C.c_int32),
+                ("lower", C.c_int32), ("mask", C.c_int32)]
+
+
 class Stats(C.Structure):
     _fields_ = [("camera_rays", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("seconds", C.c_double)]
@@ -54,6 +59,11 @@ _SIG = {
     "oracle_scene_add_mesh": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_int32]),
     "oracle_scene_add_quad_light": (C.c_int, [_P, C.POINTER(QuadLightDesc)]),
     "oracle_scene_set_environment": (None, [_P, _P]),
+    "oracle_scene_add_image": (C.c_int, [_P, C.c_int32, C.c_int32, _P]),
+    "oracle_scene_add_texture": (C.c_int, [_P, C.POINTER(TextureDesc)]),
+    "oracle_scene_set_shader_texture": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "oracle_texture_lookup": (None, [_P, C.c_int32, C.c_int32, C.c_float, C.c_float, _P]),
+    "oracle_texture_colour": (C.c_int, [_P, C.c_int32, C.c_uint32, C.c_float, C.c_float, _P]),
     "oracle_scene_commit": (C.c_int, [_P]),
     "oracle_scene_num_triangles": (C.c_uint32, [_P]),
     "oracle_scene_triangles": (C.c_int, [_P, _P]),
@@ -136,6 +146,16 @@ class Scene:
             d = ShaderDesc(sh["type"], (C.c_float * 3)(*sh["colour"]), sh.get("reflectance", 1.0),
                            sh.get("roughness", 0.0))
             assert L.oracle_scene_add_shader(self.h, C.byref(d)) >= 0
+        for im in desc.get("images", []):
+            im = np.ascontiguousarray(im, dtype=np.float32)
+            assert L.oracle_scene_add_image(self.h, im.shape[1], im.shape[0], _ptr(im)) >= 0
+        for t in desc.get("textures", []):
+            d = TextureDesc(t["type"], (C.c_float * 3)(*t.get("colour", (0.0, 0.0, 0.0))), t.get("image", -1),
+                            t.get("upper", -1), t.get("lower", -1), t.get("mask", -1))
+            assert L.oracle_scene_add_texture(self.h, C.byref(d)) >= 0
+        for i, sh in enumerate(desc["shaders"]):
+            if sh.get("texture", -1) >= 0:
+                assert L.oracle_scene_set_shader_texture(self.h, i, sh["texture"]) == 0
         for m in desc["meshes"]:
             pos = np.ascontiguousarray(m["pos4"], dtype=np.float32)
             idx = np.ascontiguousarray(m["idx"], dtype=np.uint32)
@@ -165,6 +185,11 @@ class Scene:
         lib().oracle_scene_triangles(self.h, _ptr(out))
         return out
 
+    def texture_colour(self, shader: int, prim: int, u: float, v: float):
+        out = np.zeros(3, dtype=np.float32)
+        assert lib().oracle_texture_colour(self.h, shader, prim, u, v, _ptr(out)) == 0
+        return out
+
     def _query(self, fn, rays, mask, words):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         out = np.zeros((rays.shape[0], words) if words > 1 else rays.shape[0], dtype=np.uint32)
@@ -188,6 +213,14 @@ class Scene:
         assert lib().oracle_render(self.h, C.byref(rdesc), _ptr(pix), _ptr(smp), sid_begin, sid_end, threads,
                                    C.byref(st)) == 0
         return pix, smp, st
+
+
+def texture_lookup(rgb, s: float, t: float):
+    """StandardTexture's lookup of an (H, W, 3) image at (s, t)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    lib().oracle_texture_lookup(_ptr(rgb), rgb.shape[1], rgb.shape[0], s, t, _ptr(out))
+    return out
 
 
 def raygen(rdesc: RenderDesc, sid0: int, n: int):
