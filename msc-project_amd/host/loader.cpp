@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -157,7 +158,7 @@ void decode_setup(const Node& setup, swr_render_desc* d) {
 // Constant texture colour (ConstantTexture.h convert<>).
 void decode_texture(const Node& t, float colour[3]) {
   const std::string type = as_str(need(t, "type", "texture"));
-  if (type != "Constant") bad("texture type '" + type + "' needs an image texture system, which this build omits");
+  if (type != "Constant") bad("texture type '" + type + "' is only read as a shader's texture");
   if (t.size() != 2) bad("Constant texture: expected 2 keys");
   as_vec(need(t, "colour", "texture"), colour, 3, "texture colour");
 }
@@ -165,6 +166,54 @@ void decode_texture(const Node& t, float colour[3]) {
 void check(swr_context* ctx, int rc) {
   if (rc < 0) throw Error{rc, ctx->err};
 }
+
+// A texture graph as scene texture nodes (convert<> of StandardTexture.h, LayeredTexture.h and
+// ConstantTexture.h); returns the root node. Standard textures name an image file relative to
+// the scene directory; each file is decoded once.
+struct TextureLoader {
+  swr_context* ctx;
+  swr_scene* scene;
+  std::string dir;
+  std::map<std::string, int> images;
+
+  int node(const Node& t) {
+    const std::string type = as_str(need(t, "type", "texture"));
+    swr_texture_desc d{};
+    if (type == "Constant") {
+      if (t.size() != 2) bad("Constant texture: expected 2 keys");
+      d.type = SWR_TEXTURE_CONSTANT;
+      as_vec(need(t, "colour", "texture"), d.colour, 3, "texture colour");
+    } else if (type == "Standard") {
+      if (t.size() != 2) bad("Standard texture: expected 2 keys");
+      const std::string file = dir + "/" + as_str(need(t, "string", "texture"));
+      auto it = images.find(file);
+      if (it == images.end()) {
+        io::Image im;
+        try {
+          im = io::read_image(file);
+        } catch (const std::exception& e) {
+          throw Error{SWR_E_IO, e.what()};
+        }
+        const int idx = swr_scene_add_image(scene, im.width, im.height, im.rgb.data());
+        check(ctx, idx);
+        it = images.emplace(file, idx).first;
+      }
+      d.type = SWR_TEXTURE_IMAGE;
+      d.image = it->second;
+    } else if (type == "Layered") {
+      if (t.size() != 4) bad("Layered texture: expected 4 keys");
+      d.type = SWR_TEXTURE_LAYERED;
+      d.upper = node(need(t, "upper", "texture"));
+      d.lower = node(need(t, "lower", "texture"));
+      d.mask = node(need(t, "mask", "texture"));
+    } else {
+      bad("unknown texture type '" + type + "'");
+    }
+    const int idx = swr_scene_add_texture(scene, &d);
+    check(ctx, idx);
+    return idx;
+  }
+};
 
 void load_scene(swr_context* ctx, const std::string& path, swr_scene** out, swr_render_desc* desc) {
   std::ifstream f(path);
@@ -189,6 +238,8 @@ void load_scene(swr_context* ctx, const std::string& path, swr_scene** out, swr_
     const Node& root = docs[1];
     if (!root.is_map()) throw Error{SWR_E_PARSE, "scene document must be a mapping"};
     const std::string dir = dir_of(path);
+    TextureLoader textures{ctx, scene, dir, {}};
+    int num_shaders = 0;
     // Pass 1: objects and shaders in document order (Pathtracer.cpp:166-217).
     for (const auto& kv : root.map) {
       const Node& v = kv.second;
@@ -238,9 +289,14 @@ void load_scene(swr_context* ctx, const std::string& path, swr_scene** out, swr_
           } else {
             continue;  // unknown shader types are skipped, as in the reference
           }
-          decode_texture(need(v, "texture", "shader"), sd.colour);
+          // A Constant texture is the shader's colour; other textures become texture nodes.
+          const Node& tex = need(v, "texture", "shader");
+          const bool constant = as_str(need(tex, "type", "texture")) == "Constant";
+          if (constant) decode_texture(tex, sd.colour);
           sd.reflectance = as_float(need(v, "reflectance", "shader"));
           check(ctx, swr_scene_add_shader(scene, &sd));
+          if (!constant) check(ctx, swr_scene_set_shader_texture(scene, num_shaders, textures.node(tex)));
+          ++num_shaders;
         } else if (kv.first == "environment") {  // constant environment extension
           float rgb[3];
           decode_texture(v, rgb);

@@ -1,11 +1,13 @@
 // scene_io.cpp -- YAML subset, OBJ (tinyobj semantics) and EXR writer.
 #include "scene_io.h"
 
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <sstream>
 #include <stdexcept>
@@ -551,6 +553,223 @@ void write_exr_rgba(const std::string& path, const float* rgb, int width, int he
   const size_t w = std::fwrite(b.data(), 1, b.size(), f);
   const int rc = std::fclose(f);
   if (w != b.size() || rc != 0) throw std::runtime_error("short write to " + path);
+}
+
+}  // namespace io
+}  // namespace swr
+
+namespace swr {
+namespace io {
+namespace {
+
+constexpr int64_t kMaxTexels = int64_t(1) << 28;
+
+[[noreturn]] void image_error(const std::string& path, const std::string& what) {
+  throw std::runtime_error("image " + path + ": " + what);
+}
+
+void check_size(const std::string& path, int64_t w, int64_t h) {
+  if (w <= 0 || h <= 0 || w * h > kMaxTexels) image_error(path, "size must be positive and at most 2^28 texels");
+}
+
+// Header token of a PFM / PPM file: whitespace-separated, '#' comments to the end of the line.
+std::string pnm_token(const std::vector<uint8_t>& b, size_t& pos, const std::string& path) {
+  for (;;) {
+    while (pos < b.size() && std::isspace(b[pos])) ++pos;
+    if (pos < b.size() && b[pos] == '#') {
+      while (pos < b.size() && b[pos] != '\n') ++pos;
+      continue;
+    }
+    break;
+  }
+  const size_t start = pos;
+  while (pos < b.size() && !std::isspace(b[pos])) ++pos;
+  if (pos == start) image_error(path, "truncated header");
+  return std::string(b.begin() + start, b.begin() + pos);
+}
+
+int64_t pnm_int(const std::vector<uint8_t>& b, size_t& pos, const std::string& path) {
+  const std::string t = pnm_token(b, pos, path);
+  char* end = nullptr;
+  const long long v = std::strtoll(t.c_str(), &end, 10);
+  if (*end) image_error(path, "bad header number '" + t + "'");
+  return v;
+}
+
+Image read_pfm(const std::vector<uint8_t>& b, const std::string& path) {
+  size_t pos = 0;
+  const std::string magic = pnm_token(b, pos, path);
+  if (magic != "PF" && magic != "Pf") image_error(path, "bad PFM magic");
+  const int ch = magic == "PF" ? 3 : 1;
+  Image im;
+  const int64_t w = pnm_int(b, pos, path), h = pnm_int(b, pos, path);
+  check_size(path, w, h);
+  const std::string st = pnm_token(b, pos, path);
+  char* end = nullptr;
+  const double scale = std::strtod(st.c_str(), &end);
+  if (*end || scale == 0.0) image_error(path, "bad scale '" + st + "'");
+  ++pos;  // the single whitespace before the data
+  const size_t need = (size_t)(w * h * ch) * 4;
+  if (pos > b.size() || b.size() - pos < need) image_error(path, "truncated data");
+  const bool little = scale < 0.0;
+  im.width = (int)w;
+  im.height = (int)h;
+  im.rgb.assign((size_t)w * h * 3, 0.f);
+  for (int64_t y = 0; y < h; ++y)
+    for (int64_t x = 0; x < w; ++x)
+      for (int c = 0; c < ch; ++c) {
+        const uint8_t* q = &b[pos + 4 * (size_t)((y * w + x) * ch + c)];
+        const uint32_t u = little ? (uint32_t)q[0] | (uint32_t)q[1] << 8 | (uint32_t)q[2] << 16 | (uint32_t)q[3] << 24
+                                  : (uint32_t)q[3] | (uint32_t)q[2] << 8 | (uint32_t)q[1] << 16 | (uint32_t)q[0] << 24;
+        float f;
+        std::memcpy(&f, &u, 4);
+        im.rgb[3 * (size_t)((h - 1 - y) * w + x) + c] = f;  // stored bottom-up
+      }
+  return im;
+}
+
+Image read_ppm(const std::vector<uint8_t>& b, const std::string& path) {
+  size_t pos = 0;
+  if (pnm_token(b, pos, path) != "P6") image_error(path, "bad PPM magic");
+  const int64_t w = pnm_int(b, pos, path), h = pnm_int(b, pos, path), maxval = pnm_int(b, pos, path);
+  check_size(path, w, h);
+  if (maxval <= 0 || maxval > 65535) image_error(path, "maxval must be in 1..65535");
+  ++pos;
+  const int bytes = maxval < 256 ? 1 : 2;
+  const size_t need = (size_t)(w * h * 3) * bytes;
+  if (pos > b.size() || b.size() - pos < need) image_error(path, "truncated data");
+  Image im;
+  im.width = (int)w;
+  im.height = (int)h;
+  im.rgb.resize((size_t)w * h * 3);
+  for (size_t i = 0; i < im.rgb.size(); ++i) {
+    const uint32_t v = bytes == 1 ? b[pos + i] : ((uint32_t)b[pos + 2 * i] << 8 | b[pos + 2 * i + 1]);
+    im.rgb[i] = (float)v / (float)maxval;
+  }
+  return im;
+}
+
+template <typename T>
+T get_le(const std::vector<uint8_t>& b, size_t pos, const std::string& path) {
+  if (pos > b.size() || b.size() - pos < sizeof(T)) image_error(path, "truncated file");
+  T v;
+  std::memcpy(&v, &b[pos], sizeof(T));
+  return v;
+}
+
+std::string get_cstr(const std::vector<uint8_t>& b, size_t& pos, const std::string& path) {
+  const size_t start = pos;
+  while (pos < b.size() && b[pos] != 0) ++pos;
+  if (pos >= b.size()) image_error(path, "truncated header");
+  return std::string(b.begin() + start, b.begin() + pos++);
+}
+
+Image read_exr(const std::vector<uint8_t>& b, const std::string& path) {
+  const uint32_t version = get_le<uint32_t>(b, 4, path);
+  if ((version & 0xFFu) != 2u || (version & 0x1E00u) != 0u)
+    image_error(path, "only single-part scanline EXR (version 2, not tiled, deep or multi-part) is read");
+  struct Channel {
+    std::string name;
+    int32_t type;
+  };
+  std::vector<Channel> chans;
+  int compression = -1;
+  int32_t box[4] = {0, 0, -1, -1};
+  bool have_box = false;
+  size_t pos = 8;
+  for (;;) {
+    const std::string name = get_cstr(b, pos, path);
+    if (name.empty()) break;
+    const std::string type = get_cstr(b, pos, path);
+    const int32_t size = get_le<int32_t>(b, pos, path);
+    pos += 4;
+    if (size < 0 || pos + (size_t)size > b.size()) image_error(path, "bad attribute " + name);
+    const size_t end = pos + size;
+    if (name == "channels") {
+      size_t p = pos;
+      while (p < end && b[p] != 0) {
+        Channel c;
+        c.name = get_cstr(b, p, path);
+        c.type = get_le<int32_t>(b, p, path);
+        const int32_t xs = get_le<int32_t>(b, p + 8, path), ys = get_le<int32_t>(b, p + 12, path);
+        if (c.type < 0 || c.type > 2 || xs != 1 || ys != 1) image_error(path, "unsupported channel " + c.name);
+        chans.push_back(c);
+        p += 16;
+      }
+    } else if (name == "compression") {
+      compression = b[pos];
+    } else if (name == "dataWindow") {
+      for (int k = 0; k < 4; ++k) box[k] = get_le<int32_t>(b, pos + 4 * k, path);
+      have_box = true;
+    }
+    pos = end;
+  }
+  if (compression != 0) image_error(path, "only uncompressed EXR is read");
+  if (!have_box || chans.empty()) image_error(path, "missing channels or dataWindow");
+  const int64_t w = (int64_t)box[2] - box[0] + 1, h = (int64_t)box[3] - box[1] + 1;
+  check_size(path, w, h);
+  // RGB from channels R, G, B; a lone Y (luminance) is channel 0.
+  int slot[3] = {-1, -1, -1};
+  for (size_t k = 0; k < chans.size(); ++k) {
+    if (chans[k].name == "R") slot[0] = (int)k;
+    if (chans[k].name == "G") slot[1] = (int)k;
+    if (chans[k].name == "B") slot[2] = (int)k;
+  }
+  if (slot[0] < 0 && slot[1] < 0 && slot[2] < 0) {
+    for (size_t k = 0; k < chans.size(); ++k)
+      if (chans[k].name == "Y") slot[0] = (int)k;
+    if (slot[0] < 0) image_error(path, "no R, G, B or Y channel");
+  }
+  std::vector<size_t> ch_off(chans.size());  // byte offset of each channel within a line
+  size_t line = 0;
+  for (size_t k = 0; k < chans.size(); ++k) {
+    ch_off[k] = line;
+    line += (size_t)w * (chans[k].type == 1 ? 2 : 4);
+  }
+  Image im;
+  im.width = (int)w;
+  im.height = (int)h;
+  im.rgb.assign((size_t)w * h * 3, 0.f);
+  std::vector<bool> seen((size_t)h, false);
+  const size_t table = pos;
+  for (int64_t i = 0; i < h; ++i) {
+    const uint64_t off = get_le<uint64_t>(b, table + 8 * (size_t)i, path);
+    if (off > b.size()) image_error(path, "bad line offset");
+    const int64_t y = (int64_t)get_le<int32_t>(b, off, path) - box[1];
+    const int32_t size = get_le<int32_t>(b, off + 4, path);
+    if (y < 0 || y >= h || seen[(size_t)y] || size < 0 || (size_t)size != line || off + 8 + line > b.size())
+      image_error(path, "bad scanline");
+    seen[(size_t)y] = true;
+    const size_t data = off + 8;
+    for (int c = 0; c < 3; ++c) {
+      if (slot[c] < 0) continue;
+      const Channel& ch = chans[slot[c]];
+      for (int64_t x = 0; x < w; ++x) {
+        float v;
+        if (ch.type == 1) {
+          v = half_to_float(get_le<uint16_t>(b, data + ch_off[slot[c]] + 2 * (size_t)x, path));
+        } else if (ch.type == 2) {
+          v = get_le<float>(b, data + ch_off[slot[c]] + 4 * (size_t)x, path);
+        } else {
+          v = (float)get_le<uint32_t>(b, data + ch_off[slot[c]] + 4 * (size_t)x, path);
+        }
+        im.rgb[3 * (size_t)(y * w + x) + c] = v;
+      }
+    }
+  }
+  return im;
+}
+
+}  // namespace
+
+Image read_image(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open image " + path);
+  std::vector<uint8_t> b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  if (b.size() >= 4 && get_le<int32_t>(b, 0, path) == 20000630) return read_exr(b, path);
+  if (b.size() >= 2 && b[0] == 'P' && (b[1] == 'F' || b[1] == 'f')) return read_pfm(b, path);
+  if (b.size() >= 2 && b[0] == 'P' && b[1] == '6') return read_ppm(b, path);
+  image_error(path, "unknown format (PFM, binary PPM and uncompressed scanline EXR are read)");
 }
 
 }  // namespace io

@@ -53,5 +53,19 @@ float half_to_float(uint16_t h);
 // Single-part scanline OpenEXR, no compression, channels A, B, G, R as HALF, A = 1.
 void write_exr_rgba(const std::string& path, const float* rgb, int width, int height);
 
+// ---- texture images -------------------------------------------------------------------------
+// StandardTexture's file (StandardTexture.h convert<>: the scene directory + "/" + string),
+// decoded as OpenImageIO's texture() sees it: RGB floats, row 0 the top scanline (t = 0).
+// Formats: PFM ("PF" RGB / "Pf" grey, float32, scanlines stored bottom-up, the scale's sign
+// giving the byte order), binary PPM ("P6", 8- or 16-bit samples divided by maxval) and
+// single-part scanline OpenEXR without compression (HALF, FLOAT or UINT channels, the data
+// window as the image). Channels the file lacks read as 0, OpenImageIO's default fill: a
+// one-channel image (PFM "Pf", EXR "Y") gives (v, 0, 0).
+struct Image {
+  int width = 0, height = 0;
+  std::vector<float> rgb;
+};
+Image read_image(const std::string& path);
+
 }  // namespace io
 }  // namespace swr

@@ -390,6 +390,15 @@ def write_obj(path: str, pos4, idx, uv=None):
             np.savetxt(f, idx + 1, fmt="f %d %d %d")
 
 
+def write_pfm(path: str, rgb):
+    """An (H, W, 3) float image as a little-endian PFM (scanlines stored bottom-up)."""
+    rgb = np.asarray(rgb, dtype="<f4")
+    h, w = rgb.shape[:2]
+    with open(path, "wb") as f:
+        f.write(b"PF\n%d %d\n-1.0\n" % (w, h))
+        f.write(np.ascontiguousarray(rgb[::-1]).tobytes())
+
+
 def write_scene(desc, directory: str, name: str = "scene") -> str:
     """Writes <directory>/<name>.yaml plus one OBJ per mesh in the reference schema."""
     os.makedirs(directory, exist_ok=True)
@@ -412,9 +421,30 @@ def write_scene(desc, directory: str, name: str = "scene") -> str:
     lines.append("filter: {type: %s}" % ("Tent" if r["filter"] == 0 else "Box"))
     lines.append("sampler: {type: %s}" % ["Stratified", "Independent", "Grid"][r["sampler"]])
     lines.append("---")
+    image_files = []
+    for k, im in enumerate(desc.get("images", [])):
+        fname = "%s_image%d.pfm" % (name, k)
+        write_pfm(os.path.join(directory, fname), im)
+        image_files.append(fname)
+    textures = desc.get("textures", [])
+
+    def texture_yaml(node):
+        t = textures[node]
+        if t["type"] == 0:
+            return "{type: Constant, colour: %s}" % fmt3(t["colour"])
+        if t["type"] == 1:
+            return "{type: Standard, string: %s}" % image_files[t["image"]]
+        return "{type: Layered, upper: %s, lower: %s, mask: %s}" % (
+            texture_yaml(t["upper"]), texture_yaml(t["lower"]), texture_yaml(t["mask"]))
+
     for sh in desc["shaders"]:
-        lines.append("shader: {type: Lambert, texture: {type: Constant, colour: %s}, reflectance: %.9g}"
-                     % (fmt3(sh["colour"]), sh["reflectance"]))
+        tex = (texture_yaml(sh["texture"]) if sh.get("texture", -1) >= 0
+               else "{type: Constant, colour: %s}" % fmt3(sh["colour"]))
+        if sh["type"] == SHADER_GLOSSY:
+            lines.append("shader: {type: Glossy, texture: %s, reflectance: %.9g, roughness: %.9g}"
+                         % (tex, sh["reflectance"], sh["roughness"]))
+        else:
+            lines.append("shader: {type: Lambert, texture: %s, reflectance: %.9g}" % (tex, sh["reflectance"]))
     for k, m in enumerate(desc["meshes"]):
         fname = "%s_%s.obj" % (name, m.get("name", k))
         write_obj(os.path.join(directory, fname), m["pos4"], m["idx"], m.get("uv"))

@@ -82,6 +82,12 @@ int main(int argc, char** argv) {
       arr("texcoords", m.texcoords);
       arr("indices", m.indices, true);
       std::printf("}\n");
+    } else if (mode == "image") {
+      std::cout.precision(9);
+      auto im = swr::io::read_image(argv[2]);
+      std::printf("{\"width\": %d, \"height\": %d, ", im.width, im.height);
+      arr("rgb", im.rgb, true);
+      std::printf("}\n");
     } else if (mode == "half") {
       for (int i = 2; i < argc; ++i) {
         unsigned u = (unsigned)std::strtoul(argv[i], nullptr, 16);

@@ -1,7 +1,8 @@
 """Host I/O of the product (CPU only): the YAML subset against PyYAML, OBJ loading with
 tinyobj's semantics (tiny_obj_loader.cpp:276-407, 643-890) on hand-written fixtures, IEEE
-half rounding against numpy, and the EXR file layout read back by a reader written from the
-OpenEXR file-layout description."""
+half rounding against numpy, the EXR file layout read back by a reader written from the
+OpenEXR file-layout description, and the texture image readers (PFM, PPM, uncompressed EXR)
+against files written here from the same format descriptions."""
 import json
 import os
 import struct
@@ -190,3 +191,114 @@ def test_yaml_flow_documents_and_block_sequences(harness, tmp_path):
     p = tmp_path / "f.yaml"
     p.write_text(text)
     assert json.loads(run(harness, "yaml", str(p))) == [pyyaml_tree(d) for d in yaml.compose_all(text)]
+
+
+# ---- texture images ----------------------------------------------------------------------------
+def _read_image(harness, path):
+    out = json.loads(run(harness, "image", str(path)))
+    return np.array(out["rgb"], np.float32).reshape(out["height"], out["width"], 3)
+
+
+def _exr_bytes(chans, x0=0, y0=0, compression=0, reverse_lines=False):
+    """An uncompressed single-part scanline EXR built from the OpenEXR file layout: chans maps a
+    channel name to (pixel type, (h, w) array); channels are stored in name order."""
+    names = sorted(chans)
+    h, w = chans[names[0]][1].shape
+    hdr = bytearray(struct.pack("<ii", 20000630, 2))
+
+    def attr(name, typ, value):
+        hdr.extend(name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(value)) + value)
+
+    cl = b"".join(n.encode() + b"\0" + struct.pack("<iBBBBii", chans[n][0], 0, 0, 0, 0, 1, 1) for n in names) + b"\0"
+    attr("channels", "chlist", cl)
+    attr("compression", "compression", bytes([compression]))
+    attr("dataWindow", "box2i", struct.pack("<iiii", x0, y0, x0 + w - 1, y0 + h - 1))
+    attr("displayWindow", "box2i", struct.pack("<iiii", x0, y0, x0 + w - 1, y0 + h - 1))
+    attr("lineOrder", "lineOrder", bytes([1 if reverse_lines else 0]))
+    hdr.append(0)
+    dt = {0: "<u4", 1: "<f2", 2: "<f4"}
+    lines = []
+    for y in range(h):
+        body = b"".join(np.asarray(chans[n][1][y], dt[chans[n][0]]).tobytes() for n in names)
+        lines.append(struct.pack("<ii", y0 + y, len(body)) + body)
+    order = list(range(h))[::-1] if reverse_lines else list(range(h))
+    table_end = len(hdr) + 8 * h
+    offsets, blob, pos = [0] * h, bytearray(), table_end
+    for y in order:
+        offsets[y] = pos
+        blob.extend(lines[y])
+        pos += len(lines[y])
+    return bytes(hdr) + struct.pack("<%dQ" % h, *offsets) + bytes(blob)
+
+
+def test_pfm_rgb_and_grey_both_byte_orders(harness, tmp_path):
+    rng = np.random.default_rng(21)
+    rgb = rng.uniform(-2, 2, (5, 7, 3)).astype(np.float32)
+    for endian, scale in (("<", b"-1.0"), (">", b"1.0")):
+        p = tmp_path / ("c%s.pfm" % ("le" if endian == "<" else "be"))
+        p.write_bytes(b"PF\n7 5\n" + scale + b"\n" + rgb[::-1].astype(endian + "f4").tobytes())
+        np.testing.assert_array_equal(_read_image(harness, p), rgb)  # row 0 = top = stored last
+    grey = rng.uniform(0, 1, (3, 4)).astype(np.float32)
+    p = tmp_path / "g.pfm"
+    p.write_bytes(b"Pf\n4 3\n-1\n" + grey[::-1].astype("<f4").tobytes())
+    want = np.zeros((3, 4, 3), np.float32)
+    want[:, :, 0] = grey  # missing channels read as 0
+    np.testing.assert_array_equal(_read_image(harness, p), want)
+
+
+def test_ppm_8_and_16_bit(harness, tmp_path):
+    rng = np.random.default_rng(22)
+    a8 = rng.integers(0, 256, (4, 6, 3), dtype=np.uint8)
+    p = tmp_path / "a.ppm"
+    p.write_bytes(b"P6\n# made by a test\n6 4\n255\n" + a8.tobytes())
+    np.testing.assert_array_equal(_read_image(harness, p), a8.astype(np.float32) / np.float32(255))
+    a16 = rng.integers(0, 1000, (2, 3, 3), dtype=np.uint16)
+    p = tmp_path / "b.ppm"
+    p.write_bytes(b"P6 3 2 999\n" + a16.astype(">u2").tobytes())
+    np.testing.assert_array_equal(_read_image(harness, p), a16.astype(np.float32) / np.float32(999))
+
+
+def test_exr_channel_types_windows_and_line_order(harness, tmp_path):
+    rng = np.random.default_rng(23)
+    h, w = 4, 5
+    r = rng.uniform(0, 3, (h, w)).astype(np.float32)
+    g = rng.uniform(0, 3, (h, w)).astype(np.float16)
+    b = rng.integers(0, 70000, (h, w)).astype(np.uint32)
+    a = np.ones((h, w), np.float16)
+    p = tmp_path / "m.exr"
+    p.write_bytes(_exr_bytes({"R": (2, r), "G": (1, g), "B": (0, b), "A": (1, a)}, x0=3, y0=-2,
+                             reverse_lines=True))
+    got = _read_image(harness, p)
+    np.testing.assert_array_equal(got[:, :, 0], r)
+    np.testing.assert_array_equal(got[:, :, 1], g.astype(np.float32))
+    np.testing.assert_array_equal(got[:, :, 2], b.astype(np.float32))
+    y = rng.uniform(0, 1, (h, w)).astype(np.float32)
+    p = tmp_path / "y.exr"
+    p.write_bytes(_exr_bytes({"Y": (2, y)}))
+    got = _read_image(harness, p)
+    np.testing.assert_array_equal(got[:, :, 0], y)
+    assert not got[:, :, 1:].any()
+
+
+def test_product_exr_reads_back(harness, tmp_path):
+    import swr
+    rgb = np.random.default_rng(24).uniform(0, 4, (3, 5, 3)).astype(np.float32)
+    path = str(tmp_path / "o.exr")
+    assert swr.load_library().swr_write_exr(path.encode(), rgb.ctypes.data, 5, 3) == 0
+    np.testing.assert_array_equal(_read_image(harness, path), rgb.astype(np.float16).astype(np.float32))
+
+
+def test_image_errors(harness, tmp_path):
+    h = np.ones((2, 2), np.float32)
+    cases = {
+        "zip.exr": _exr_bytes({"R": (2, h)}, compression=3),
+        "nochan.exr": _exr_bytes({"Z": (2, h)}),
+        "short.pfm": b"PF\n4 4\n-1\n" + b"\0" * 10,
+        "zero.pfm": b"PF\n0 4\n-1\n",
+        "what.png": b"\x89PNG\r\n",
+        "big.ppm": b"P6 2 2 70000\n" + b"\0" * 24,
+    }
+    for name, data in cases.items():
+        (tmp_path / name).write_bytes(data)
+        r = subprocess.run([harness, "image", str(tmp_path / name)], capture_output=True, text=True)
+        assert r.returncode == 1 and r.stdout.startswith("ERROR"), name

@@ -31,6 +31,38 @@ def test_yaml_scene_renders_like_the_arrays(ctx, tmp_path):
     np.testing.assert_array_equal(p_file, p_arr)
 
 
+def test_textured_yaml_scene_renders_like_the_arrays(ctx, tmp_path):
+    """Standard, Layered and Constant textures through the scene file (images as PFM, which
+    holds float32 exactly): the same image as the scene built from arrays, bit for bit."""
+    desc = scenes.textured_room(width=40, height=32, base=2, sphere_level=2)
+    path = scenes.write_scene(desc, str(tmp_path), "tex")
+    sc_file, rd_file = swr.load_yaml(ctx, path)
+    sc_arr = swr.Scene(ctx, desc)
+    rd = swr.render_desc(desc["render"])
+    rd_file.seed = rd.seed
+    p_file, _, _ = sc_file.render(rd_file)
+    p_arr, _, _ = sc_arr.render(rd)
+    np.testing.assert_array_equal(p_file, p_arr)
+    sc_file.close()
+    sc_arr.close()
+
+
+def test_yaml_texture_errors(ctx, tmp_path):
+    (tmp_path / "t.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\nf 1/1 2/2 3/3\n")
+    obj = ("object: {type: Polygon, filename: t.obj, translation: [0, 0, 0], rotation: [0, 0, 0], "
+           "scale: [1, 1, 1], shader: 0}\n")
+    cases = [("{type: Standard, string: missing.pfm}", swr.SWR_E_IO),
+             ("{type: Checker, colour: [1, 1, 1]}", swr.SWR_E_PARSE),
+             ("{type: Layered, upper: {type: Constant, colour: [1, 1, 1]}, lower: {type: Constant, "
+              "colour: [0, 0, 0]}}", swr.SWR_E_PARSE)]
+    for k, (tex, code) in enumerate(cases):
+        (tmp_path / ("s%d.yaml" % k)).write_text(
+            "{}\n---\nshader: {type: Lambert, texture: %s, reflectance: 1}\n%s" % (tex, obj))
+        with pytest.raises(swr.SwrError) as e:
+            swr.load_yaml(ctx, str(tmp_path / ("s%d.yaml" % k)))
+        assert e.value.code == code, tex
+
+
 def test_yaml_defaults_and_strict_decoders(ctx, tmp_path):
     (tmp_path / "t.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
     (tmp_path / "ok.yaml").write_text(
