@@ -259,6 +259,42 @@ inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 
 
 }  // namespace
 
+// One node per thread: the quantised copy (NodeQ) of a BVH4 node.
+__global__ void quantize_kernel(const Node4* __restrict__ nodes, uint32_t n, NodeQ* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Node4 nd = nodes[i];
+  const bool valid[4] = {nd.child.x != kNoChild, nd.child.y != kNoChild, nd.child.z != kNoChild,
+                         nd.child.w != kNoChild};
+  const float lx[4] = {nd.lox.x, nd.lox.y, nd.lox.z, nd.lox.w}, hx[4] = {nd.hix.x, nd.hix.y, nd.hix.z, nd.hix.w};
+  const float ly[4] = {nd.loy.x, nd.loy.y, nd.loy.z, nd.loy.w}, hy[4] = {nd.hiy.x, nd.hiy.y, nd.hiy.z, nd.hiy.w};
+  const float lz[4] = {nd.loz.x, nd.loz.y, nd.loz.z, nd.loz.w}, hz[4] = {nd.hiz.x, nd.hiz.y, nd.hiz.z, nd.hiz.w};
+  const QuantAxis ax = quantize_axis(lx, hx, valid), ay = quantize_axis(ly, hy, valid),
+                  az = quantize_axis(lz, hz, valid);
+  NodeQ q;
+  q.px = ax.p;
+  q.py = ay.p;
+  q.pz = az.p;
+  q.exps = ax.e | ay.e << 8 | az.e << 16;
+  q.qlox = ax.qlo;
+  q.qhix = ax.qhi;
+  q.qloy = ay.qlo;
+  q.qhiy = ay.qhi;
+  q.qloz = az.qlo;
+  q.qhiz = az.qhi;
+  q.pad0 = q.pad1 = 0u;
+  q.child = nd.child;
+  out[i] = q;
+}
+
+void quantize_bvh(swr_context* ctx, BvhBuild* out) {
+  const uint32_t n = std::max<uint32_t>(out->num_nodes, 1);
+  ensure(ctx, out->qnodes, (size_t)n * sizeof(NodeQ));
+  quantize_kernel<<<blocks(n), 256, 0, ctx->stream>>>(static_cast<const Node4*>(out->nodes.ptr), n,
+                                                       static_cast<NodeQ*>(out->qnodes.ptr));
+  SWR_HIP(hipGetLastError());
+}
+
 void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const GeomRec* geoms, uint32_t n,
                const float lo[3], const float hi[3], int leaf_max, BvhBuild* out) {
   hipStream_t s = ctx->stream;

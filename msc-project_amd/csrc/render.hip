@@ -179,7 +179,8 @@ __global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const DeviceSc
   }
   auto body = [&](uint32_t s0) {
     const RayState r = make_ray(o, d);
-    const HitResult h = traverse<kTraceBlock, kShort, false, kOne, kOne>(scenes[s0].nodes, scenes[s0].leaf_tris, r,
+    const void* nodes = kOne && kQuantNodes ? (const void*)scenes[s0].qnodes : (const void*)scenes[s0].nodes;
+    const HitResult h = traverse<kTraceBlock, kShort, false, kOne, kOne, kOne && kQuantNodes>(nodes, scenes[s0].leaf_tris, r,
                                                              SWR_MASK_ALL, s_ring + threadIdx.x,
                                                              spill + (size_t)j * spill_stride, s_dist + threadIdx.x);
     reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
@@ -413,7 +414,8 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const DeviceScene* 
   auto body = [&](uint32_t s0) {
     if (!kRingFilter || scene_ring[s0] == (uint32_t)ring) {
       const RayState r = make_ray(make_float4(o.x, o.y, o.z, 0.001f), d);
-      const HitResult h = traverse<kTraceBlock, kShort, true, false, kOne>(scenes[s0].nodes, scenes[s0].leaf_tris, r,
+      const void* nodes = kOne && kQuantNodes ? (const void*)scenes[s0].qnodes : (const void*)scenes[s0].nodes;
+      const HitResult h = traverse<kTraceBlock, kShort, true, false, kOne, kOne && kQuantNodes>(nodes, scenes[s0].leaf_tris, r,
                                                               SWR_MASK_SHADOW, s_ring + threadIdx.x,
                                                               spill + (size_t)i * spill_stride);
       if (h.id == kMiss) {

@@ -216,6 +216,7 @@ void commit_scene(swr_scene* s) {
   if (!texels.empty()) upload(ctx, s->d_texels, texels);
   build_bvh(ctx, static_cast<const float4*>(s->d_pos.ptr), static_cast<const uint4*>(s->d_tri.ptr),
             static_cast<const GeomRec*>(s->d_geoms.ptr), (uint32_t)tri.size(), clo, chi, s->leaf_max, &s->bvh);
+  quantize_bvh(ctx, &s->bvh);
   SWR_HIP(hipStreamSynchronize(ctx->stream));
   if (stack_need(s->bvh.max_depth) > 3 * max_supported_depth() + 4)
     fail(SWR_E_INVALID, "scene: BVH depth " + std::to_string(s->bvh.max_depth) + " exceeds the traversal stack");
@@ -231,6 +232,7 @@ void commit_scene(swr_scene* s) {
   d.shaders = static_cast<const ShaderRec*>(s->d_shaders.ptr);
   d.lights = static_cast<const LightRec*>(s->d_lights.ptr);
   d.nodes = static_cast<const Node4*>(s->bvh.nodes.ptr);
+  d.qnodes = static_cast<const NodeQ*>(s->bvh.qnodes.ptr);
   d.leaf_tris = static_cast<const float4*>(s->bvh.leaf_tris.ptr);
   d.tex_ops = ops.empty() ? nullptr : static_cast<const TexOp*>(s->d_tex_ops.ptr);
   d.tex_images = images.empty() ? nullptr : static_cast<const TexImage*>(s->d_tex_images.ptr);
@@ -250,7 +252,7 @@ void commit_scene(swr_scene* s) {
 
 void swr::detach_scene(swr_scene* s) {
   for (DeviceBuffer* b : {&s->d_pos, &s->d_tri, &s->d_uv, &s->d_geoms, &s->d_shaders, &s->d_lights, &s->bvh.nodes,
-                          &s->bvh.leaf_tris, &s->d_tex_ops, &s->d_tex_images, &s->d_texels})
+                          &s->bvh.qnodes, &s->bvh.leaf_tris, &s->d_tex_ops, &s->d_tex_images, &s->d_texels})
     *b = DeviceBuffer{};
   s->dev = DeviceScene{};
   s->committed = false;
@@ -290,7 +292,7 @@ void swr_scene_destroy(swr_scene* s) {
   if (swr_context* ctx = s->ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (swr::DeviceBuffer* b : {&s->d_pos, &s->d_tri, &s->d_uv, &s->d_geoms, &s->d_shaders, &s->d_lights,
-                                 &s->bvh.nodes, &s->bvh.leaf_tris, &s->d_tex_ops, &s->d_tex_images, &s->d_texels}) {
+                                 &s->bvh.nodes, &s->bvh.qnodes, &s->bvh.leaf_tris, &s->d_tex_ops, &s->d_tex_images, &s->d_texels}) {
       try {
         swr::release(ctx, *b);
       } catch (const Error&) {
@@ -443,8 +445,10 @@ int swr_scene_bvh_info(const swr_scene* s, uint32_t* nodes_out, uint32_t* depth_
   return scene_guard(ctx, [&] {
     const uint32_t nn = s->bvh.num_nodes, nt = s->dev.num_tris;
     std::vector<swr::Node4> nodes(nn);
+    std::vector<swr::NodeQ> qnodes(nn);
     std::vector<float4> lt(3 * (size_t)nt);
     SWR_HIP(hipMemcpy(nodes.data(), s->bvh.nodes.ptr, nn * sizeof(swr::Node4), hipMemcpyDeviceToHost));
+    SWR_HIP(hipMemcpy(qnodes.data(), s->bvh.qnodes.ptr, nn * sizeof(swr::NodeQ), hipMemcpyDeviceToHost));
     if (nt) SWR_HIP(hipMemcpy(lt.data(), s->bvh.leaf_tris.ptr, lt.size() * sizeof(float4), hipMemcpyDeviceToHost));
     bool ok = true;
     std::vector<uint8_t> seen_leaf(nt, 0), seen_id(nt, 0), seen_node(nn, 0);
@@ -472,6 +476,26 @@ int swr_scene_bvh_info(const swr_scene* s, uint32_t* nodes_out, uint32_t* depth_
       const float* ly = &nd.loy.x; const float* hy = &nd.hiy.x;
       const float* lz = &nd.loz.x; const float* hz = &nd.hiz.x;
       const uint32_t* ch = &nd.child.x;
+      // The quantised copy: same children, planes at least one quantum outside every child box.
+      const swr::NodeQ& qn = qnodes[it.node];
+      if (std::memcmp(&qn.child, &nd.child, sizeof nd.child) != 0) ok = false;
+      {
+        const float* los[3] = {lx, ly, lz};
+        const float* his[3] = {hx, hy, hz};
+        const float ps[3] = {qn.px, qn.py, qn.pz};
+        const uint32_t qlo[3] = {qn.qlox, qn.qloy, qn.qloz}, qhi[3] = {qn.qhix, qn.qhiy, qn.qhiz};
+        for (int a = 0; a < 3; ++a) {
+          const uint32_t e = (qn.exps >> (8 * a)) & 0xFFu;
+          if (e == 0u || e == 255u) ok = false;
+          const double q = std::ldexp(1.0, (int)e - 127);
+          for (int c = 0; c < 4; ++c) {
+            if (ch[c] == swr::kNoChild) continue;
+            const double l = (double)ps[a] + (double)((qlo[a] >> (8 * c)) & 0xFFu) * q;
+            const double h = (double)ps[a] + (double)((qhi[a] >> (8 * c)) & 0xFFu) * q;
+            if (!(l <= (double)los[a][c] - q && h >= (double)his[a][c] + q)) ok = false;
+          }
+        }
+      }
       for (int c = 0; c < 4; ++c) {
         if (ch[c] == swr::kNoChild) continue;
         if (ch[c] & swr::kLeafBit) {

@@ -30,6 +30,20 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "BVH4 node must be 128 bytes");
 
+// The same node with quantised child boxes (64 B, half a cache line): per axis the node box
+// origin p and a power-of-two quantum s = 2^(e - 127), and per child its lo and hi planes as bytes
+// q with plane = p + q * s, rounded outward by at least one quantum (so that the rounding of the
+// traversal's slab arithmetic never culls a box). Rows: (px, py, pz, e_x | e_y << 8 | e_z << 16),
+// (qlo_x, qhi_x, qlo_y, qhi_y), (qlo_z, qhi_z, 0, 0), child; byte c of a q word is child c.
+struct alignas(16) NodeQ {
+  float px, py, pz;
+  uint32_t exps;
+  uint32_t qlox, qhix, qloy, qhiy;
+  uint32_t qloz, qhiz, pad0, pad1;
+  uint4 child;
+};
+static_assert(sizeof(NodeQ) == 64, "quantised node must be 64 bytes");
+
 constexpr uint32_t kNoChild = 0xFFFFFFFFu;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr float kEmptyBox = 1e30f;
@@ -90,6 +104,7 @@ struct DeviceScene {
   const ShaderRec* shaders = nullptr;
   const LightRec* lights = nullptr;
   const Node4* nodes = nullptr;
+  const NodeQ* qnodes = nullptr;      // the same nodes quantised
   const float4* leaf_tris = nullptr;  // 3 float4 per triangle in BVH order
   const TexOp* tex_ops = nullptr;
   const TexImage* tex_images = nullptr;
@@ -104,11 +119,56 @@ struct DeviceScene {
 // Builds the BVH4 of the scene's triangles on the device: 30-bit centroid Morton codes, radix
 // sort, Karras hierarchy, bottom-up boxes, BVH4 collapse and the BVH-order triangle array.
 struct BvhBuild {
-  DeviceBuffer nodes, leaf_tris;
+  DeviceBuffer nodes, qnodes, leaf_tris;
   uint32_t num_nodes = 0, max_depth = 0;
 };
 void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const GeomRec* geoms, uint32_t num_tris,
                const float lo[3], const float hi[3], int leaf_max, BvhBuild* out);
+// Writes out->qnodes, the quantised copy of out->nodes.
+void quantize_bvh(swr_context* ctx, BvhBuild* out);
+// Quantum exponent (biased) and outward-rounded plane bytes of one axis of a node whose valid
+// children span [lo[c], hi[c]] (valid[c]); shared by the build and its validation.
+struct QuantAxis {
+  float p;
+  uint32_t e, qlo, qhi;
+};
+__host__ __device__ inline QuantAxis quantize_axis(const float* lo, const float* hi, const bool* valid) {
+  float l = 0.f, h = 0.f;
+  bool any = false;
+  for (int c = 0; c < 4; ++c)
+    if (valid[c]) {
+      l = any ? fminf(l, lo[c]) : lo[c];
+      h = any ? fmaxf(h, hi[c]) : hi[c];
+      any = true;
+    }
+  // s = 2^k: at least 2^-126, (h - l) / s <= 252, and s >= ulp(p) so that p = l - s loses
+  // less than one quantum when rounded down to a float.
+  const double E = (double)h - (double)l;
+  int k = -126;
+  if (E > 0.0 && E < 1e300) k = (int)ceil(log2(E / 252.0));
+  int em = 0;
+  frexp((double)fmaxf(fabsf(l), fabsf(h)), &em);  // |l|, |h| < 2^em
+  k = k > em - 23 ? k : em - 23;
+  k = k > -126 ? k : -126;
+  k = k < 127 ? k : 127;
+  const double sd = ldexp(1.0, k);
+  double pd = (double)l - sd;
+  float p = (float)pd;
+  if ((double)p > pd) p = nextafterf(p, -INFINITY);  // round down
+  QuantAxis a{p, (uint32_t)(k + 127), 0u, 0u};
+  for (int c = 0; c < 4; ++c) {
+    uint32_t ql = 255u, qh = 0u;  // unused slots: an empty interval (masked by the traversal anyway)
+    if (valid[c]) {
+      const double lq = floor(((double)lo[c] - (double)p) / sd) - 1.0;
+      const double hq = ceil(((double)hi[c] - (double)p) / sd) + 1.0;
+      ql = (uint32_t)(!(lq >= 0.0) ? 0.0 : (lq > 255.0 ? 255.0 : lq));
+      qh = (uint32_t)(!(hq >= 0.0) ? 0.0 : (hq > 255.0 ? 255.0 : hq));
+    }
+    a.qlo |= ql << (8 * c);
+    a.qhi |= qh << (8 * c);
+  }
+  return a;
+}
 
 // ---- trace.hip ----
 // Closest hit (hits: t, u, v bits and global triangle id) and any-hit occlusion queries.

@@ -186,6 +186,46 @@ __device__ inline void node_entries(const RayState& r, const PlaneRows& pr, cons
   *t3 = ch->w == kNoChild ? inf : s3;
 }
 
+// Entry distances of the four children of a quantised node (NodeQ, 64 bytes at base + node_off,
+// four 16-byte loads). A plane's distance is fma(q, s / d, (p - o) / d), which rounds outside the
+// exact (p + q s - o) / d by a few ulps of the terms; the planes lie at least one quantum outside
+// the child boxes, and the exit distance gets a slack of 2^-21, so no box that the ray meets is
+// culled. Near and far planes are picked per axis by the sign of the direction, as plane_rows
+// does for full nodes.
+constexpr float kBoxSlackQ = 1.00000048f;
+__device__ inline float qplane(uint32_t w, int c, float a, float b) {
+  return fmaf((float)((w >> (8 * c)) & 0xFFu), a, b);
+}
+__device__ inline void node_entries_q(const RayState& r, const SWR_GLOBAL char* base, uint32_t node_off, float tfar,
+                                      uint4* ch, float* t0, float* t1, float* t2, float* t3) {
+  const gf4 h = *(const SWR_GLOBAL gf4*)(base + node_off);
+  const gu4 qa = *(const SWR_GLOBAL gu4*)(base + (node_off + 16u));
+  const gu4 qb = *(const SWR_GLOBAL gu4*)(base + (node_off + 32u));
+  const gu4 c = *(const SWR_GLOBAL gu4*)(base + (node_off + 48u));
+  *ch = make_uint4(c.x, c.y, c.z, c.w);
+  const uint32_t e = __float_as_uint(h.w);
+  const float ax = __uint_as_float((e & 0xFFu) << 23) * r.ix;
+  const float ay = __uint_as_float(((e >> 8) & 0xFFu) << 23) * r.iy;
+  const float az = __uint_as_float(((e >> 16) & 0xFFu) << 23) * r.iz;
+  const float bx = (h.x - r.ox) * r.ix, by = (h.y - r.oy) * r.iy, bz = (h.z - r.oz) * r.iz;
+  const bool sx = r.ix < 0.f, sy = r.iy < 0.f, sz = r.iz < 0.f;
+  const uint32_t nx = sx ? qa.y : qa.x, fx = sx ? qa.x : qa.y;
+  const uint32_t ny = sy ? qa.w : qa.z, fy = sy ? qa.z : qa.w;
+  const uint32_t nz = sz ? qb.y : qb.x, fz = sz ? qb.x : qb.y;
+  const float inf = __int_as_float(0x7F800000);
+  float s[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float tn = fmaxf(fmaxf(qplane(nx, k, ax, bx), qplane(ny, k, ay, by)), fmaxf(qplane(nz, k, az, bz), r.tmin));
+    const float tf = fminf(fminf(qplane(fx, k, ax, bx), qplane(fy, k, ay, by)), fminf(qplane(fz, k, az, bz), tfar));
+    s[k] = tn <= tf * kBoxSlackQ ? tn : inf;
+  }
+  *t0 = ch->x == kNoChild ? inf : s[0];
+  *t1 = ch->y == kNoChild ? inf : s[1];
+  *t2 = ch->z == kNoChild ? inf : s[2];
+  *t3 = ch->w == kNoChild ? inf : s[3];
+}
+
 __device__ inline void cswap(uint32_t& a, uint32_t& b) {
   const uint32_t lo = min(a, b), hi = max(a, b);
   a = lo;
@@ -268,8 +308,9 @@ struct ShortStack {
 // distance exceeds the closest hit so far by more than the box slack is skipped unfetched:
 // that is the parent's box test run again with the current hit distance, so the result is the
 // one without the skip.
-template <int kBlock, int kShort, bool kAny, bool kPairTris = false, bool kNearFar = false>
-__device__ inline HitResult traverse(const Node4* __restrict__ nodes, const float4* __restrict__ leaf_tris,
+// kQuant: nodes is the quantised node array (NodeQ) instead of the full one (Node4).
+template <int kBlock, int kShort, bool kAny, bool kPairTris = false, bool kNearFar = false, bool kQuant = false>
+__device__ inline HitResult traverse(const void* __restrict__ nodes, const float4* __restrict__ leaf_tris,
                                      const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col,
                                      uint16_t* dist_col = nullptr) {
   HitResult best;
@@ -279,7 +320,7 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
   best.leaf = kMiss;
   const float inf = __int_as_float(0x7F800000);
   ShortStack<kBlock, kShort, !kAny> st(lds_col, spill_col, dist_col);
-  const SWR_GLOBAL char* gnodes = (const SWR_GLOBAL char*)global_nodes(nodes);
+  const SWR_GLOBAL char* gnodes = (const SWR_GLOBAL char*)nodes;
   const SWR_GLOBAL gf4* gtris = global_f4(leaf_tris);
   const PlaneRows pr = plane_rows(r);
   uint32_t node = 0;  // root
@@ -288,7 +329,10 @@ __device__ inline HitResult traverse(const Node4* __restrict__ nodes, const floa
       // Unused slots (kNoChild) are never entered, whatever their box.
       uint4 ch;
       float t0, t1, t2, t3;
-      node_entries<kNearFar>(r, pr, gnodes, node * 128u, best.t, &ch, &t0, &t1, &t2, &t3);
+      if (kQuant)
+        node_entries_q(r, gnodes, node * 64u, best.t, &ch, &t0, &t1, &t2, &t3);
+      else
+        node_entries<kNearFar>(r, pr, gnodes, node * 128u, best.t, &ch, &t0, &t1, &t2, &t3);
       if (kAny) {
         if (t0 != inf) st.push(ch.x);
         if (t1 != inf) st.push(ch.y);
@@ -533,6 +577,11 @@ __device__ inline void trace_chunk(uint32_t begin, uint32_t end, uint32_t mask, 
 // chunks of 64 / 128 / 256 rays per wave. Unsorted rays (the stand-alone queries) gain from
 // refill instead: 8 M random rays take 2.46 ms with 64-ray chunks and 1.86 ms with 256.
 constexpr int kTraceBlock = 128;
+// The one-scene wavefront kernels traverse the quantised nodes.
+#ifndef SWR_QUANT_NODES
+#define SWR_QUANT_NODES 1
+#endif
+constexpr bool kQuantNodes = SWR_QUANT_NODES != 0;
 constexpr uint32_t kRefillChunk = 256;  // rays per wave of the stand-alone queries
 inline uint32_t refill_waves(uint32_t n) { return std::max(1u, (n + kRefillChunk - 1) / kRefillChunk); }
 inline uint32_t trace_blocks(uint32_t waves) { return (waves + kTraceBlock / 64 - 1) / (kTraceBlock / 64); }
