@@ -149,7 +149,29 @@ __global__ void raygen_rays_kernel(FrameParams p, uint32_t sid0, uint32_t k, flo
 // kRingFilter, only rays whose scene uses this launch's ring size are traced.
 // The scene of ray j is the top scene_bits bits of its sorted key (sorted_keys[j] >> scene_shift):
 // the ray record's scene word is overwritten with the hit t by whichever launch traces it.
-template <int kShort, bool kRingFilter, bool kOne>
+//
+// Tile modes (several scenes in the loop, whose keys carry the scene on top, so the rays of a
+// scene are contiguous in sorted order and almost every tile of kTraceBlock rays holds one scene):
+//   kTileAll:   every tile, each lane with its ray's scene (scene-uniform groups of lanes in turn);
+//   kTileOne:   only tiles whose rays all belong to one scene, traversed with that scene's
+//               constants in scalar registers and the one-scene traversal (as kOne);
+//   kTileMixed: only the tiles that hold rays of several scenes, per lane as kTileAll; workgroup b
+//               takes the tile holding the first ray of scene b + 1.
+// kTileOne plus kTileMixed trace every ray exactly once.
+constexpr int kTileAll = 0, kTileOne = 1, kTileMixed = 2;
+
+// First position of [0, n) whose sorted key is >= v (n when none); every lane computes the same.
+__device__ inline uint32_t lower_bound_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int kShort, bool kRingFilter, bool kOne, int kTile = kTileAll>
 __global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const DeviceScene* __restrict__ scenes,
                                                                   const uint8_t* __restrict__ scene_ring, int ring,
                                                                   const uint32_t* __restrict__ perm,
@@ -162,16 +184,36 @@ __global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const DeviceSc
                                                                   uint32_t* __restrict__ spill, uint32_t spill_stride) {
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
   __shared__ uint16_t s_dist[kShort * kTraceBlock];
-  const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
   const uint32_t n = w->n;
-  if (blockIdx.x * kTraceBlock >= n) return;
+  uint32_t tile = blockIdx.x;
+  if (kTile == kTileMixed) {
+    const uint32_t b = blockIdx.x + 1u;  // scene whose first ray opens the boundary
+    const uint32_t start = lower_bound_u32(sorted_keys, n, b << scene_shift);
+    if (start >= n || start % kTraceBlock == 0u) return;  // no boundary inside a tile here
+    const uint32_t prev = lower_bound_u32(sorted_keys, n, (b - 1u) << scene_shift);
+    // The tile is taken by its lowest boundary: skip when scene b - 1 also starts inside it.
+    if (b > 1u && prev / kTraceBlock == start / kTraceBlock && prev % kTraceBlock != 0u) return;
+    tile = start / kTraceBlock;
+  }
+  if (tile * kTraceBlock >= n) return;
+  uint32_t tile_scene = 0;
+  if (kTile == kTileOne) {
+    const uint32_t last = min(tile * kTraceBlock + kTraceBlock, n) - 1u;
+    const uint32_t a = __builtin_amdgcn_readfirstlane(sorted_keys[tile * kTraceBlock] >> scene_shift);
+    const uint32_t z = __builtin_amdgcn_readfirstlane(sorted_keys[last] >> scene_shift);
+    if (a != z) return;  // a mixed tile
+    if (kRingFilter && scene_ring[a] != (uint32_t)ring) return;
+    tile_scene = a;
+  }
+  constexpr bool kUniform = kOne || kTile == kTileOne;
+  const uint32_t j = tile * kTraceBlock + threadIdx.x;
   bool pending = j < n;
   uint32_t i = 0, scene = 0;
   float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
   if (pending) {
     i = perm[j];
-    if (!kOne) scene = sorted_keys[j] >> scene_shift;
-    if (kRingFilter && scene_ring[scene] != (uint32_t)ring) pending = false;
+    if (!kUniform) scene = sorted_keys[j] >> scene_shift;
+    if (!kUniform && kRingFilter && scene_ring[scene] != (uint32_t)ring) pending = false;
   }
   if (pending) {
     o = q[i].o;
@@ -179,15 +221,16 @@ __global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const DeviceSc
   }
   auto body = [&](uint32_t s0) {
     const RayState r = make_ray(o, d);
-    const void* nodes = kOne && kQuantNodes ? (const void*)scenes[s0].qnodes : (const void*)scenes[s0].nodes;
-    const HitResult h = traverse<kTraceBlock, kShort, false, kOne, kOne, kOne && kQuantNodes>(nodes, scenes[s0].leaf_tris, r,
-                                                             SWR_MASK_ALL, s_ring + threadIdx.x,
-                                                             spill + (size_t)j * spill_stride, s_dist + threadIdx.x);
+    constexpr bool kQ = kUniform && kQuantNodes;
+    const void* nodes = kQ ? (const void*)scenes[s0].qnodes : (const void*)scenes[s0].nodes;
+    const HitResult h = traverse<kTraceBlock, kShort, false, kUniform, kUniform, kQ>(
+        nodes, scenes[s0].leaf_tris, r, SWR_MASK_ALL, s_ring + threadIdx.x, spill + (size_t)j * spill_stride,
+        s_dist + threadIdx.x);
     reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
     hit_keys[i] = ((s0 << hit_tri_bits) | (h.id + 1u)) >> hit_shift;
   };
-  if (kOne) {
-    if (pending) body(0u);
+  if (kUniform) {
+    if (pending) body(tile_scene);
   } else {
     SWR_BY_UNIFORM(pending, scene, s0, { body(s0); })
   }
@@ -397,6 +440,7 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const DeviceScene* 
                                                              float4* __restrict__ radiance,
                                                              uint32_t* __restrict__ spill, uint32_t spill_stride) {
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
+  __shared__ uint32_t s_scenes[2];
   const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
   const uint32_t n = w->n;
   if (blockIdx.x * kTraceBlock >= n) return;
@@ -411,25 +455,48 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const DeviceScene* 
     tag = __float_as_uint(o.w);
     scene = tag >> 24;
   }
-  auto body = [&](uint32_t s0) {
-    if (!kRingFilter || scene_ring[s0] == (uint32_t)ring) {
-      const RayState r = make_ray(make_float4(o.x, o.y, o.z, 0.001f), d);
-      const void* nodes = kOne && kQuantNodes ? (const void*)scenes[s0].qnodes : (const void*)scenes[s0].nodes;
-      const HitResult h = traverse<kTraceBlock, kShort, true, false, kOne, kOne && kQuantNodes>(nodes, scenes[s0].leaf_tris, r,
-                                                              SWR_MASK_SHADOW, s_ring + threadIdx.x,
-                                                              spill + (size_t)i * spill_stride);
-      if (h.id == kMiss) {
-        const float4 c = sr[2];
-        const uint64_t s = ((uint64_t)(tag & 0xFFFFFFu) << 32) | __float_as_uint(c.w);
-        float4 acc = radiance[s];
-        acc.x = acc.x + c.x;
-        acc.y = acc.y + c.y;
-        acc.z = acc.z + c.z;
-        radiance[s] = acc;
-      }
+  // Several scenes: shadow rays come in hit order, whose keys carry the scene on top, so most
+  // workgroups hold rays of one scene; those trace them with that scene's constants in scalar
+  // registers and the one-scene traversal, the others lane by lane.
+  uint32_t lo = 0, hi = 0;
+  if (!kOne && kSceneTiles) {
+    if (threadIdx.x == 0) {
+      s_scenes[0] = 0xFFFFFFFFu;
+      s_scenes[1] = 0u;
+    }
+    __syncthreads();
+    if (pending) {
+      atomicMin(&s_scenes[0], scene);
+      atomicMax(&s_scenes[1], scene);
+    }
+    __syncthreads();
+    lo = __builtin_amdgcn_readfirstlane(s_scenes[0]);
+    hi = __builtin_amdgcn_readfirstlane(s_scenes[1]);
+  }
+  auto finish = [&](const HitResult& h) {
+    if (h.id == kMiss) {
+      const float4 c = sr[2];
+      const uint64_t s = ((uint64_t)(tag & 0xFFFFFFu) << 32) | __float_as_uint(c.w);
+      float4 acc = radiance[s];
+      acc.x = acc.x + c.x;
+      acc.y = acc.y + c.y;
+      acc.z = acc.z + c.z;
+      radiance[s] = acc;
     }
   };
-  if (pending) body(kOne ? 0u : scene);
+  if (kOne || (kSceneTiles && lo >= hi)) {  // one scene (lo > hi: no shadow ray in the workgroup)
+    const uint32_t s0 = kOne ? 0u : lo;
+    if (pending && (!kRingFilter || scene_ring[s0] == (uint32_t)ring)) {
+      const RayState r = make_ray(make_float4(o.x, o.y, o.z, 0.001f), d);
+      const void* nodes = kQuantNodes ? (const void*)scenes[s0].qnodes : (const void*)scenes[s0].nodes;
+      finish(traverse<kTraceBlock, kShort, true, false, true, kQuantNodes>(
+          nodes, scenes[s0].leaf_tris, r, SWR_MASK_SHADOW, s_ring + threadIdx.x, spill + (size_t)i * spill_stride));
+    }
+  } else if (pending && (!kRingFilter || scene_ring[scene] == (uint32_t)ring)) {
+    const RayState r = make_ray(make_float4(o.x, o.y, o.z, 0.001f), d);
+    finish(traverse<kTraceBlock, kShort, true, false, false>(
+        scenes[scene].nodes, scenes[scene].leaf_tris, r, SWR_MASK_SHADOW, s_ring + threadIdx.x, spill + (size_t)i * spill_stride));
+  }
 }
 
 // After a wave: every job admitted before the oldest job that still has a live path (a
@@ -906,18 +973,37 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
     const uint32_t scene_shift = (uint32_t)(ray_bits_ - scene_bits_);  // used only with scene_bits_ > 0
     for (int ring : ring_set_) {
       with_ring(ring, [&](auto kShort) {
-        if (filter)
-          trace_queue_kernel<kShort.value, true, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
-              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
-              spill_stride_);
-        else if (one_scene)
+        if (one_scene) {
           trace_queue_kernel<kShort.value, false, true><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
               stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
               spill_stride_);
-        else
+        } else if (kSceneTiles) {
+          // Scene-uniform tiles, then the tiles at scene boundaries (at most one per scene).
+          const uint32_t nscenes = (uint32_t)scenes_.size();
+          if (filter) {
+            trace_queue_kernel<kShort.value, true, false, kTileOne><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_,
+                spill, spill_stride_);
+            trace_queue_kernel<kShort.value, true, false, kTileMixed><<<nscenes - 1, kTraceBlock, 0, st>>>(
+                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_,
+                spill, spill_stride_);
+          } else {
+            trace_queue_kernel<kShort.value, false, false, kTileOne><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_,
+                spill, spill_stride_);
+            trace_queue_kernel<kShort.value, false, false, kTileMixed><<<nscenes - 1, kTraceBlock, 0, st>>>(
+                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_,
+                spill, spill_stride_);
+          }
+        } else if (filter) {
+          trace_queue_kernel<kShort.value, true, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
+              spill_stride_);
+        } else {
           trace_queue_kernel<kShort.value, false, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
               stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
               spill_stride_);
+        }
       });
     }
     mark(3);

@@ -577,11 +577,18 @@ __device__ inline void trace_chunk(uint32_t begin, uint32_t end, uint32_t mask, 
 // chunks of 64 / 128 / 256 rays per wave. Unsorted rays (the stand-alone queries) gain from
 // refill instead: 8 M random rays take 2.46 ms with 64-ray chunks and 1.86 ms with 256.
 constexpr int kTraceBlock = 128;
-// The one-scene wavefront kernels traverse the quantised nodes.
+// Build switches (both on; off gives the variants they were measured against): the one-scene
+// traversal reads the quantised nodes, and multi-scene loops trace scene-uniform tiles with the
+// one-scene traversal. The per-lane multi-scene traversal keeps the full nodes: quantised, it
+// needs more registers (79 against 76) and was slower.
 #ifndef SWR_QUANT_NODES
 #define SWR_QUANT_NODES 1
 #endif
 constexpr bool kQuantNodes = SWR_QUANT_NODES != 0;
+#ifndef SWR_SCENE_TILES
+#define SWR_SCENE_TILES 1
+#endif
+constexpr bool kSceneTiles = SWR_SCENE_TILES != 0;
 constexpr uint32_t kRefillChunk = 256;  // rays per wave of the stand-alone queries
 inline uint32_t refill_waves(uint32_t n) { return std::max(1u, (n + kRefillChunk - 1) / kRefillChunk); }
 inline uint32_t trace_blocks(uint32_t waves) { return (waves + kTraceBlock / 64 - 1) / (kTraceBlock / 64); }
