@@ -171,8 +171,11 @@ __device__ inline uint32_t lower_bound_u32(const uint32_t* __restrict__ a, uint3
   return lo;
 }
 
+// Held to 8 waves per SIMD (64 VGPRs): the one-scene traversal fits without spilling (63), and
+// more waves hide more of the node-fetch latency; the per-lane multi-scene form, left for tiles at
+// scene boundaries, spills 24 bytes.
 template <int kShort, bool kRingFilter, bool kOne, int kTile = kTileAll>
-__global__ __launch_bounds__(kTraceBlock) void trace_queue_kernel(const DeviceScene* __restrict__ scenes,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void trace_queue_kernel(const DeviceScene* __restrict__ scenes,
                                                                   const uint8_t* __restrict__ scene_ring, int ring,
                                                                   const uint32_t* __restrict__ perm,
                                                                   const uint32_t* __restrict__ sorted_keys,
