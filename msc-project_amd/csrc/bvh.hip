@@ -170,7 +170,7 @@ __device__ inline uint32_t ref_size(const Bvh2View& v, uint32_t r) {
 __global__ void collapse_kernel(Bvh2View v, const uint2* __restrict__ frontier, const uint32_t* __restrict__ count_in,
                                 uint2* __restrict__ next, uint32_t* __restrict__ count_next,
                                 uint32_t* __restrict__ node_alloc, Node4* __restrict__ nodes, uint32_t depth,
-                                uint32_t leaf_max) {
+                                uint32_t leaf_max, uint32_t capacity, uint32_t* __restrict__ overflow) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= *count_in) return;
   const uint2 item = frontier[i];
@@ -199,6 +199,16 @@ __global__ void collapse_kernel(Bvh2View v, const uint2* __restrict__ frontier, 
   Node4 nd;
   float lo[3][4], hi[3][4];
   uint32_t cc[4];
+  // The internal children of a node get consecutive indices from an even one (allocations come in
+  // pairs; the allocator starts at 2), so that quantised siblings (64 B) share a 128-byte line.
+  uint32_t internal = 0;
+  for (int c = 0; c < cnt; ++c)
+    if (!(ref[c] & kBvh2Leaf) && ref_size(v, ref[c]) > leaf_max) ++internal;
+  uint32_t next_idx = internal ? atomicAdd(node_alloc, (internal + 1u) & ~1u) : 0u;
+  if (internal && next_idx + internal > capacity) {  // no room: the host fails the build
+    atomicOr(overflow, 1u);
+    return;
+  }
   for (int c = 0; c < 4; ++c) {
     if (c >= cnt) {
       lo[0][c] = lo[1][c] = lo[2][c] = kEmptyBox;
@@ -223,7 +233,7 @@ __global__ void collapse_kernel(Bvh2View v, const uint2* __restrict__ frontier, 
       const uint2 g = v.range[r];
       cc[c] = leaf_code(g.x, g.y - g.x + 1u);
     } else {
-      const uint32_t idx = atomicAdd(node_alloc, 1u);
+      const uint32_t idx = next_idx++;
       cc[c] = idx;
       next[atomicAdd(count_next, 1u)] = make_uint2(idx, r);
     }
@@ -397,7 +407,7 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
     ensure(ctx, fr_a, nn * 8);
     ensure(ctx, fr_b, nn * 8);
     ensure(ctx, counters, 16);
-    uint32_t init[4] = {1u, 1u, 0u, 0u};
+    uint32_t init[4] = {2u, 1u, 0u, 0u};  // node 0 is the root; node 1 is left unused
     uint2 root_item = make_uint2(0u, 0u);
     SWR_HIP(hipMemcpyAsync(counters.ptr, init, 16, hipMemcpyHostToDevice, s));
     SWR_HIP(hipMemcpyAsync(fr_a.ptr, &root_item, 8, hipMemcpyHostToDevice, s));
@@ -412,7 +422,7 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
     while (frontier) {
       SWR_HIP(hipMemsetAsync(cnt + nxt_slot, 0, 4, s));
       collapse_kernel<<<blocks(frontier), 256, 0, s>>>(view, cur, cnt + cur_slot, nxt, cnt + nxt_slot, cnt, nodes, depth,
-                                                       (uint32_t)leaf_max);
+                                                       (uint32_t)leaf_max, (uint32_t)nn, cnt + 3);
       SWR_HIP(hipGetLastError());
       SWR_HIP(hipMemcpyAsync(&frontier, cnt + nxt_slot, 4, hipMemcpyDeviceToHost, s));
       SWR_HIP(hipStreamSynchronize(s));
@@ -421,9 +431,11 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
       if (frontier) ++depth;
       if (depth > 256) fail(SWR_E_INVALID, "bvh: collapse did not terminate");
     }
-    uint32_t num_nodes = 0;
+    uint32_t num_nodes = 0, overflow = 0;
     SWR_HIP(hipMemcpyAsync(&num_nodes, cnt, 4, hipMemcpyDeviceToHost, s));
+    SWR_HIP(hipMemcpyAsync(&overflow, cnt + 3, 4, hipMemcpyDeviceToHost, s));
     SWR_HIP(hipStreamSynchronize(s));
+    if (overflow) fail(SWR_E_INVALID, "bvh: node array too small for the collapsed hierarchy");
     out->num_nodes = num_nodes;
     out->max_depth = depth;
   } catch (...) {
