@@ -66,6 +66,36 @@ def test_batch_equals_solo_renders(ctx, mixed, wave_rays, paced):
         b.close()
 
 
+def test_batch_of_tiny_jobs_on_many_scenes_equals_solo(ctx):
+    """Scene segments shorter than a traversal tile (128 rays): tiles hold two, three or more
+    scenes (and scenes drop out of later waves as their jobs finish), which exercises the split
+    of a multi-scene wave into scene-uniform tiles and tiles at scene boundaries."""
+    descs = [scenes.box_room(seed=100 + k, width=8, height=8, base=1, sphere_level=1) for k in range(4)]
+    scs = [swr.Scene(ctx, d) for d in descs]
+    jobs = []
+    for k in range(12):
+        s = k % 4
+        jobs.append((s, _desc(descs[s], width=3 + k % 5, height=2 + k % 3, base=1, spp=1 + k % 4, sampler=1,
+                              seed=500 + k)))
+    try:
+        solo = [scs[s].render(rd, samples=True) for s, rd in jobs]
+        for wave_rays in (0, 64, 300):
+            b = swr.Batch(ctx)
+            try:
+                for s, rd in jobs:
+                    b.add(scs[s], rd)
+                b.run(wave_rays)
+                for j, (pix, smp, _) in enumerate(solo):
+                    p, sm = b.result(j, samples=True)
+                    np.testing.assert_array_equal(sm, smp)
+                    np.testing.assert_array_equal(p, pix)
+            finally:
+                b.close()
+    finally:
+        for sc in scs:
+            sc.close()
+
+
 def test_batch_autotune_keeps_images(ctx, mixed):
     scs, jobs, solo = mixed
     before = ctx.tuning()
