@@ -3,7 +3,7 @@
 Both sides draw every random number from Philox keyed by (seed, sample, iteration, depth,
 slot), so the GPU and the oracle trace the same paths: per-sample radiance agrees except for
 the rare path whose direction differs by a last-ulp transcendental and so hits elsewhere.
-The image test of the north star (independent streams, 3 sigma of the per-pixel Monte Carlo
+The converged-image test (independent streams, 3 sigma of the per-pixel Monte Carlo
 standard error) is checked as well, and the result must not depend on how rays are grouped
 into waves (queue capacity), which is the batched-equals-solo property."""
 import math
