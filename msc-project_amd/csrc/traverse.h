@@ -465,10 +465,15 @@ struct Walker {
       node_entries<true>(r, plane_rows(r), (const SWR_GLOBAL char*)global_nodes(nodes), node * 128u, best.t, &ch,
                          &t0, &t1, &t2, &t3);
       if (kAny) {
-        if (t0 != inf) st.push(ch.x);
-        if (t1 != inf) st.push(ch.y);
-        if (t2 != inf) st.push(ch.z);
-        if (t3 != inf) st.push(ch.w);
+        // As in traverse: descend into the first hit child, push the others.
+        const bool h0 = t0 != inf, h1 = t1 != inf, h2 = t2 != inf, h3 = t3 != inf;
+        if (h1 && h0) st.push(ch.y);
+        if (h2 && (h0 || h1)) st.push(ch.z);
+        if (h3 && (h0 || h1 || h2)) st.push(ch.w);
+        if (h0 || h1 || h2 || h3) {
+          node = h0 ? ch.x : (h1 ? ch.y : (h2 ? ch.z : ch.w));
+          return false;
+        }
       } else {
         // Entry distances are >= tmin >= 0, so their bit patterns order like the floats; the low
         // two bits carry the child slot.
