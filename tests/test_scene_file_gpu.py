@@ -94,3 +94,31 @@ def test_cli_writes_exr(tmp_path):
     assert r.returncode == 0, r.stderr
     data = out.read_bytes()
     assert data[:4] == bytes([0x76, 0x2F, 0x31, 0x01])
+
+
+def test_cli_progressive_preview_is_the_viewer_image(tmp_path):
+    """Without -o the command line runs the viewer's progressive loop headless: -n iterations,
+    each adding one process() to the framebuffer, and -p writes the viewer's 8-bit image,
+    min(1, sum / iterations) * 255 truncated (main.cpp:163-168), which must match the sum of the
+    same iterations rendered through the Python facade."""
+    exe = os.path.join(ROOT, "msc-project_amd", "bin", "swr")
+    if not os.path.exists(exe):
+        pytest.skip("command-line renderer not built")
+    desc = scenes.box_room(width=20, height=12, base=2, sphere_level=1)
+    path = scenes.write_scene(desc, str(tmp_path), "room")
+    out = tmp_path / "preview.ppm"
+    r = subprocess.run([exe, "-n", "3", "-p", str(out), path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("is complete") == 3
+    data = out.read_bytes()
+    header = b"P6\n20 12\n255\n"
+    assert data[:len(header)] == header
+    got = np.frombuffer(data[len(header):], np.uint8).reshape(12, 20, 3)
+    pt = swr.Pathtracer(path)
+    for _ in range(3):
+        pt.process()
+    total, _, _ = pt.image()
+    want = (np.minimum(np.float32(1.0), total / np.float32(3.0)) * np.float32(255.0)).astype(np.uint8)
+    np.testing.assert_array_equal(got, want)
+    bad = subprocess.run([exe, "-n", "0", path], capture_output=True, text=True, timeout=60)
+    assert bad.returncode != 0 and "positive integer" in bad.stderr
