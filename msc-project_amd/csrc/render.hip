@@ -186,7 +186,6 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
                                                                   uint32_t hit_tri_bits, uint32_t hit_shift,
                                                                   uint32_t* __restrict__ spill, uint32_t spill_stride) {
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
-  __shared__ uint16_t s_dist[kShort * kTraceBlock];
   const uint32_t n = w->n;
   uint32_t tile = blockIdx.x;
   if (kTile == kTileMixed) {
@@ -227,8 +226,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
     constexpr bool kQ = kUniform && kQuantNodes;
     const void* nodes = kQ ? (const void*)scenes[s0].qnodes : (const void*)scenes[s0].nodes;
     const HitResult h = traverse<kTraceBlock, kShort, false, kUniform, kUniform, kQ>(
-        nodes, scenes[s0].leaf_tris, r, SWR_MASK_ALL, s_ring + threadIdx.x, spill + (size_t)j * spill_stride,
-        s_dist + threadIdx.x);
+        nodes, scenes[s0].leaf_tris, r, SWR_MASK_ALL, s_ring + threadIdx.x, spill + (size_t)j * spill_stride);
     reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
     hit_keys[i] = ((s0 << hit_tri_bits) | (h.id + 1u)) >> hit_shift;
   };

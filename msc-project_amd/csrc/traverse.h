@@ -242,24 +242,19 @@ struct HitResult {
 // lds[(k % kShort) * kBlock], so a wave's accesses hit 64 consecutive words); entries pushed
 // deeper than that spill, oldest first, to the lane's column in HBM and come back on pop.
 // A small ring keeps occupancy high; the spill makes any BVH depth correct.
-// kDist: ring entries also keep the entry distance of their box, as the top 16 bits of the
-// float (a truncation, so never above the distance); spilled entries come back with 0.
-template <int kBlock, int kShort, bool kDist = false>
+template <int kBlock, int kShort>
 struct ShortStack {
   static_assert((kShort & (kShort - 1)) == 0, "ring size must be a power of two");
   uint32_t* lds;    // this lane's ring column (stride kBlock)
-  uint16_t* dist;   // this lane's distance ring column (stride kBlock), with kDist
   uint32_t* spill;  // this lane's spill column (entries [0, base))
   int sp, base;
-  __device__ ShortStack(uint32_t* l, uint32_t* s, uint16_t* dl = nullptr)
-      : lds(l), dist(dl), spill(s), sp(0), base(0) {}
-  __device__ void push(uint32_t v, uint32_t tbits = 0) {
+  __device__ ShortStack(uint32_t* l, uint32_t* s) : lds(l), spill(s), sp(0), base(0) {}
+  __device__ void push(uint32_t v) {
     if (sp - base == kShort) {
       spill[base] = lds[(base & (kShort - 1)) * kBlock];
       ++base;
     }
     lds[(sp & (kShort - 1)) * kBlock] = v;
-    if (kDist) dist[(sp & (kShort - 1)) * kBlock] = (uint16_t)(tbits >> 16);
     ++sp;
   }
   // Makes room for n unchecked pushes: spills the oldest ring entries while the ring would
@@ -270,9 +265,8 @@ struct ShortStack {
       ++base;
     }
   }
-  __device__ void push_reserved(uint32_t v, uint32_t tbits = 0) {
+  __device__ void push_reserved(uint32_t v) {
     lds[(sp & (kShort - 1)) * kBlock] = v;
-    if (kDist) dist[(sp & (kShort - 1)) * kBlock] = (uint16_t)(tbits >> 16);
     ++sp;
   }
   __device__ bool empty() const { return sp == 0; }
@@ -284,17 +278,6 @@ struct ShortStack {
     }
     return lds[(sp & (kShort - 1)) * kBlock];
   }
-  // Pops an entry with a lower bound of its entry distance (0 for an entry that was spilled).
-  __device__ uint32_t pop(float* entry) {
-    --sp;
-    if (sp < base) {
-      base = sp;
-      *entry = 0.f;
-      return spill[sp];
-    }
-    *entry = __uint_as_float((uint32_t)dist[(sp & (kShort - 1)) * kBlock] << 16);
-    return lds[(sp & (kShort - 1)) * kBlock];
-  }
 };
 
 // Traverses one ray. kAny: returns on the first hit (id of that triangle), else the closest.
@@ -304,22 +287,19 @@ struct ShortStack {
 // round trips, more registers: worth it where the kernel has registers to spare). kNearFar: box
 // tests from the ray's near and far planes (see node_entries; same results, fewer operations,
 // more registers).
-// dist_col (closest hit only): this lane's ring of entry distances. A popped node whose entry
-// distance exceeds the closest hit so far by more than the box slack is skipped unfetched:
-// that is the parent's box test run again with the current hit distance, so the result is the
-// one without the skip.
+// Popped nodes are fetched and their children tested against the closest hit so far; keeping
+// each entry's distance to skip far ones unfetched cost more in stack traffic than it saved.
 // kQuant: nodes is the quantised node array (NodeQ) instead of the full one (Node4).
 template <int kBlock, int kShort, bool kAny, bool kPairTris = false, bool kNearFar = false, bool kQuant = false>
 __device__ inline HitResult traverse(const void* __restrict__ nodes, const float4* __restrict__ leaf_tris,
-                                     const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col,
-                                     uint16_t* dist_col = nullptr) {
+                                     const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col) {
   HitResult best;
   best.t = r.tmax;
   best.u = best.v = __int_as_float(0x7FC00000);
   best.id = kMiss;
   best.leaf = kMiss;
   const float inf = __int_as_float(0x7F800000);
-  ShortStack<kBlock, kShort, !kAny> st(lds_col, spill_col, dist_col);
+  ShortStack<kBlock, kShort> st(lds_col, spill_col);
   const SWR_GLOBAL char* gnodes = (const SWR_GLOBAL char*)nodes;
   const SWR_GLOBAL gf4* gtris = global_f4(leaf_tris);
   const PlaneRows pr = plane_rows(r);
@@ -362,9 +342,9 @@ __device__ inline HitResult traverse(const void* __restrict__ nodes, const float
         // hits k0..k2), then descend into the nearest.
         const bool h1 = (k1 & ~3u) < miss_key, h2 = (k2 & ~3u) < miss_key, h3 = (k3 & ~3u) < miss_key;
         st.reserve((int)h1 + (int)h2 + (int)h3);
-        if (h3) st.push_reserved(cs[k3 & 3u], k3 & ~3u);
-        if (h2) st.push_reserved(cs[k2 & 3u], k2 & ~3u);
-        if (h1) st.push_reserved(cs[k1 & 3u], k1 & ~3u);
+        if (h3) st.push_reserved(cs[k3 & 3u]);
+        if (h2) st.push_reserved(cs[k2 & 3u]);
+        if (h1) st.push_reserved(cs[k1 & 3u]);
         if ((k0 & ~3u) < miss_key) {
           node = cs[k0 & 3u];
           continue;
@@ -410,21 +390,8 @@ __device__ inline HitResult traverse(const void* __restrict__ nodes, const float
         }
       }
     }
-    if (kAny) {
-      if (st.empty()) break;
-      node = st.pop();
-    } else {
-      bool more = false;
-      while (!st.empty()) {
-        float entry;
-        node = st.pop(&entry);
-        if (!(entry > best.t * kBoxSlack)) {
-          more = true;
-          break;
-        }
-      }
-      if (!more) break;
-    }
+    if (st.empty()) break;
+    node = st.pop();
   }
   return best;
 }
@@ -490,9 +457,9 @@ struct Walker {
         const uint32_t cs[4] = {ch.x, ch.y, ch.z, ch.w};
         // Push hit children far to near except the nearest (keys ascend, so a hit k3 implies
         // hits k0..k2), then descend into the nearest.
-        if ((k3 & ~3u) < miss_key) st.push(cs[k3 & 3u], k3 & ~3u);
-        if ((k2 & ~3u) < miss_key) st.push(cs[k2 & 3u], k2 & ~3u);
-        if ((k1 & ~3u) < miss_key) st.push(cs[k1 & 3u], k1 & ~3u);
+        if ((k3 & ~3u) < miss_key) st.push(cs[k3 & 3u]);
+        if ((k2 & ~3u) < miss_key) st.push(cs[k2 & 3u]);
+        if ((k1 & ~3u) < miss_key) st.push(cs[k1 & 3u]);
         if ((k0 & ~3u) < miss_key) {
           node = cs[k0 & 3u];
           return false;
