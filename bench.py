@@ -380,8 +380,9 @@ def batched_vs_plain(ctx, args, plain=True):
     """The batched workload (256 jobs over 8 resident scenes of 10k-2M triangles, pixel counts
     log-uniform in [128^2, 1024^2], spp uniform in {1..64}, depth 6) traced twice: job by job
     through the plain loop, and all together in one persistent graph-replayed loop. Both times
-    run from each loop's first raygen to its last resolved framebuffer (device events); images
-    of every 8th job are compared bit for bit."""
+    run from each loop's first raygen to its last resolved framebuffer (device events), each the
+    faster of two runs (per job for the plain side); images of every 8th job are compared bit for
+    bit."""
     import numpy as np
 
     import swr
@@ -409,18 +410,21 @@ def batched_vs_plain(ctx, args, plain=True):
         tuning = b.autotune()
     tune_s = time.time() - t0
 
-    # plain: one render per job through the host-paced loop, with the tuning above
+    # plain: one render per job through the host-paced loop, with the tuning above; each job is
+    # rendered twice and the faster device time counted (the batched side below is likewise the
+    # faster of two timed runs)
     plain_ms, plain_rays, plain_pix = 0.0, 0, {}
     wall = time.perf_counter()
     for j, ((s, _), rd) in enumerate(zip(jobs, rds)):
         if not plain:
             break
         pix, _, st = scs[s].render(rd)
-        plain_ms += st["ms_total"]
+        _, _, st2 = scs[s].render(rd, pixels=False)
+        plain_ms += min(st["ms_total"], st2["ms_total"])
         plain_rays += st["closest_rays"] + st["shadow_rays"]
         if j % 8 == 0:
             plain_pix[j] = pix
-    plain_wall = time.perf_counter() - wall
+    plain_wall = (time.perf_counter() - wall) / 2
 
     wave_rays = args.batch_wave_rays
     if not wave_rays:  # the batched loop's own wave size: the fastest of 2^24 .. 2^28 rays
@@ -436,9 +440,13 @@ def batched_vs_plain(ctx, args, plain=True):
         tune_s += time.time() - t0
     paced = b.run(wave_rays, paced=True)  # the same batch launched wave by wave from the host: per-stage times
     b.run(wave_rays)  # warm-up of the graph path
-    wall = time.perf_counter()
-    st = b.run(wave_rays)
-    batch_wall = time.perf_counter() - wall
+    st, batch_wall = None, None
+    for _ in range(2):
+        wall = time.perf_counter()
+        run = b.run(wave_rays)
+        w_s = time.perf_counter() - wall
+        if st is None or run["ms_total"] < st["ms_total"]:
+            st, batch_wall = run, w_s
     batch_rays = st["closest_rays"] + st["shadow_rays"]
     same = all(np.array_equal(b.result(j)[0], p) for j, p in plain_pix.items())
     b.close()
