@@ -593,6 +593,10 @@ def cpu_baseline(desc, rd, args):
     start = (total // 2 // n) * n if n < total else 0
     _, _, st = oc.render(od, samples=False, sid_begin=start, sid_end=start + n, threads=threads, pixels=False)
     rays = st.closest_rays + st.shadow_rays
+    # the same restatement on one core, over 1/threads of the sample band (same CPU time)
+    n1 = max(1, n // threads)
+    _, _, st1 = oc.render(od, samples=False, sid_begin=start, sid_end=start + n1, threads=1, pixels=False)
+    rays1 = st1.closest_rays + st1.shadow_rays
     return {
         "value": round(rays / st.seconds / 1e6, 3),
         "unit": "Mrays/s",
@@ -600,7 +604,22 @@ def cpu_baseline(desc, rd, args):
         "kind": "port",
         "sample": f"{n} of {total} sample ids (ids {start}..{start + n - 1}) of the same frame; "
                   f"{rays} rays in {st.seconds:.2f} s; CPU BVH build {build_s:.1f} s excluded",
+        "one_core": {"value": round(rays1 / st1.seconds / 1e6, 3), "cores": 1,
+                     "sample": f"ids {start}..{start + n1 - 1}; {rays1} rays in {st1.seconds:.2f} s"},
+        "cpu_model": _cpu_model(),
+        "host_cpus": os.cpu_count(),
     }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":
