@@ -211,7 +211,7 @@ def main():
             "traffic": None,
             "avg_launch_ms": round(agg["ms_trace"] / agg["waves"], 4),
             "rays_per_launch": agg["closest_rays"] // agg["waves"],
-            "note": "algorithmic 48 B/ray (ray in + hit out), node traffic excluded; traversal is bound by node-fetch latency (L2 hit 47%), not HBM bandwidth",
+            "note": "algorithmic 48 B/ray (ray in + hit out), node traffic excluded; traversal is bound by node-fetch latency (L2 hit 50% in the committed profile), not HBM bandwidth",
         },
         "sort_roofline": {
             "kernel": f"in-loop ray sort ({ray_passes} passes of 8 bits, "
