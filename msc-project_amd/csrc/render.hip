@@ -145,7 +145,7 @@ __global__ void raygen_rays_kernel(FrameParams p, uint32_t sid0, uint32_t k, flo
 
 
 // Closest hit of every queued ray, visited in ray-key order (perm). The hit (t, leaf) goes into
-// the ray's record, the hit key ((scene, global triangle id + 1), misses 0) into hit_keys. With
+// the ray's record, the hit key (the scene's hit_base + global triangle id + 1; a miss hit_base) into hit_keys. With
 // kRingFilter, only rays whose scene uses this launch's ring size are traced.
 // The scene of ray j is the top scene_bits bits of its sorted key (sorted_keys[j] >> scene_shift):
 // the ray record's scene word is overwritten with the hit t by whichever launch traces it.
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
                                                                   const WaveState* __restrict__ w,
                                                                   RayRec* __restrict__ q,
                                                                   uint32_t* __restrict__ hit_keys,
-                                                                  uint32_t hit_tri_bits, uint32_t hit_shift,
+                                                                  uint32_t hit_shift,
                                                                   uint32_t* __restrict__ spill, uint32_t spill_stride) {
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
   const uint32_t n = w->n;
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
     const HitResult h = traverse<kTraceBlock, kShort, false, kUniform, kUniform, kQ>(
         nodes, scenes[s0].leaf_tris, r, SWR_MASK_ALL, s_ring + threadIdx.x, spill + (size_t)j * spill_stride);
     reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
-    hit_keys[i] = ((s0 << hit_tri_bits) | (h.id + 1u)) >> hit_shift;
+    hit_keys[i] = (scenes[s0].hit_base + h.id + 1u) >> hit_shift;
   };
   if (kUniform) {
     if (pending) body(tile_scene);
@@ -808,17 +808,22 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
   if (scenes.empty() || scenes.size() > 256) fail(SWR_E_INVALID, "wavefront: 1 to 256 scenes");
   if (max_jobs == 0 || max_jobs > 65536) fail(SWR_E_INVALID, "wavefront: 1 to 65536 job slots");
   if (capacity < 64 || capacity > (1u << 28)) fail(SWR_E_INVALID, "wavefront: wave capacity must be 64 .. 2^28");
-  uint32_t max_tris = 1, max_depth = 0;
+  uint32_t max_depth = 0;
   for (const swr_scene* s : scenes) {
     if (!s || s->ctx != ctx) fail(SWR_E_INVALID, "wavefront: scene belongs to another (or a destroyed) context");
     if (!s->committed) fail(SWR_E_INVALID, "wavefront: scene not committed");
-    max_tris = std::max(max_tris, s->dev.num_tris);
     max_depth = std::max(max_depth, s->dev.max_depth);
   }
   scene_bits_ = scenes.size() > 1 ? bits_for(scenes.size() - 1) : 0;
-  hit_tri_bits_ = bits_for((uint64_t)max_tris);  // keys are prim + 1 <= num_tris
-  const int hit_full = scene_bits_ + hit_tri_bits_;
-  if (hit_full > 32) fail(SWR_E_INVALID, "wavefront: scene count and triangle count exceed a 32-bit hit key");
+  // Hit keys: scene s takes keys hit_base[s] .. hit_base[s] + num_tris (0 = miss), in scene order.
+  std::vector<uint32_t> hit_base(scenes.size());
+  uint64_t key_end = 0;
+  for (size_t i = 0; i < scenes.size(); ++i) {
+    hit_base[i] = (uint32_t)key_end;
+    key_end += (uint64_t)scenes[i]->dev.num_tris + 1u;
+    if (key_end > (1ull << 32)) fail(SWR_E_INVALID, "wavefront: the scenes' triangles exceed a 32-bit hit key");
+  }
+  const int hit_full = bits_for(key_end - 1u);
   const Tuning& tu = ctx->tuning;
   ray_bits_ = std::max(scene_bits_ + 3, std::min(tu.ray_key_bits, 32));
   hit_bits_ = tu.hit_key_bits > 0 ? std::min(tu.hit_key_bits, hit_full) : hit_full;
@@ -863,7 +868,10 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
       for (auto& e : row) SWR_HIP(hipEventCreate(&e));
   }
   std::vector<DeviceScene> tab;
-  for (const swr_scene* s : scenes) tab.push_back(s->dev);
+  for (size_t i = 0; i < scenes.size(); ++i) {
+    tab.push_back(scenes[i]->dev);
+    tab.back().hit_base = hit_base[i];
+  }
   SWR_HIP(hipMemcpyAsync(b_.scene_tab.ptr, tab.data(), sizeof(DeviceScene) * tab.size(), hipMemcpyHostToDevice,
                          stream_));
   SWR_HIP(hipMemcpyAsync(b_.scene_ring.ptr, ring_tab.data(), ring_tab.size(), hipMemcpyHostToDevice, stream_));
@@ -976,33 +984,33 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
       with_ring(ring, [&](auto kShort) {
         if (one_scene) {
           trace_queue_kernel<kShort.value, false, true><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
-              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
+              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_, spill,
               spill_stride_);
         } else if (kSceneTiles) {
           // Scene-uniform tiles, then the tiles at scene boundaries (at most one per scene).
           const uint32_t nscenes = (uint32_t)scenes_.size();
           if (filter) {
             trace_queue_kernel<kShort.value, true, false, kTileOne><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
-                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_,
+                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_,
                 spill, spill_stride_);
             trace_queue_kernel<kShort.value, true, false, kTileMixed><<<nscenes - 1, kTraceBlock, 0, st>>>(
-                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_,
+                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_,
                 spill, spill_stride_);
           } else {
             trace_queue_kernel<kShort.value, false, false, kTileOne><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
-                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_,
+                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_,
                 spill, spill_stride_);
             trace_queue_kernel<kShort.value, false, false, kTileMixed><<<nscenes - 1, kTraceBlock, 0, st>>>(
-                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_,
+                stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_,
                 spill, spill_stride_);
           }
         } else if (filter) {
           trace_queue_kernel<kShort.value, true, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
-              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
+              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_, spill,
               spill_stride_);
         } else {
           trace_queue_kernel<kShort.value, false, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
-              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_tri_bits_, hit_shift_, spill,
+              stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_, spill,
               spill_stride_);
         }
       });

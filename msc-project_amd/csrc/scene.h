@@ -111,6 +111,9 @@ struct DeviceScene {
   const float4* texels = nullptr;
   uint32_t num_tris = 0, num_geoms = 0, num_lights = 0, num_nodes = 0;
   uint32_t max_depth = 0;             // BVH4 depth (root = 0)
+  uint32_t hit_base = 0;              // in a wavefront loop's scene table: the scene's hit keys are
+                                      // hit_base + triangle + 1 (hit_base for a miss), so keys order
+                                      // hits by (scene, triangle) in the fewest bits
   float env[3] = {0.f, 0.f, 0.f};
   float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};  // scene bounds
 };

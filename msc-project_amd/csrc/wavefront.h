@@ -144,7 +144,7 @@ class WaveLoop {
   std::vector<int> rings_;        // ring size per scene
   std::vector<int> ring_set_;     // distinct ring sizes
   uint32_t max_jobs_, cap_;
-  int scene_bits_ = 0, hit_tri_bits_ = 0, ray_bits_ = 32, hit_bits_ = 0, hit_shift_ = 0;
+  int scene_bits_ = 0, ray_bits_ = 32, hit_bits_ = 0, hit_shift_ = 0;
   uint32_t spill_stride_ = 0;
   bool spill_ = false;
   bool textured_ = false;         // some scene has textured shaders
