@@ -1152,4 +1152,47 @@ int oracle_render(const oracle_scene* s, const oracle_render_desc* d, float* pix
   return 0;
 }
 
+// ---- the BSDFs alone, vectorised over direction pairs (for the lobes' property tests) ------
+void oracle_ggx_d_n(float alpha, size_t n, const float* cos_h, float* d) {
+  for (size_t i = 0; i < n; ++i) d[i] = ggx_d(alpha * alpha, cos_h[i]);
+}
+
+void oracle_ggx_g1_n(float alpha, size_t n, const float* cos_v, float* g) {
+  for (size_t i = 0; i < n; ++i) g[i] = ggx_g1(alpha * alpha, cos_v[i]);
+}
+
+void oracle_bsdf_eval_n(const oracle_shader_desc* sh, size_t n, const float* in3, const float* out3,
+                        const float* n3, float* f3, float* cos, float* pdf) {
+  for (size_t i = 0; i < n; ++i) {
+    V3 in = v3(in3[3 * i], in3[3 * i + 1], in3[3 * i + 2]);
+    V3 out = v3(out3[3 * i], out3[3 * i + 1], out3[3 * i + 2]);
+    V3 nn = v3(n3[3 * i], n3[3 * i + 1], n3[3 * i + 2]);
+    Eval e = bsdf_eval(*sh, in, out, nn);
+    f3[3 * i] = e.f.x;
+    f3[3 * i + 1] = e.f.y;
+    f3[3 * i + 2] = e.f.z;
+    cos[i] = e.cos;
+    pdf[i] = e.pdf;
+  }
+}
+
+void oracle_bsdf_sample_n(const oracle_shader_desc* sh, size_t n, const float* u2, const float* out3,
+                          const float* n3, float* in3, float* f3, float* cos, float* pdf, uint8_t* ok) {
+  for (size_t i = 0; i < n; ++i) {
+    V3 out = v3(out3[3 * i], out3[3 * i + 1], out3[3 * i + 2]);
+    V3 nn = v3(n3[3 * i], n3[3 * i + 1], n3[3 * i + 2]);
+    V3 in = v3(0.f, 0.f, 0.f);
+    Eval e{v3(0.f, 0.f, 0.f), 0.f, 0.f};
+    ok[i] = bsdf_sample(*sh, u2[2 * i], u2[2 * i + 1], out, nn, &in, &e) ? 1 : 0;
+    in3[3 * i] = in.x;
+    in3[3 * i + 1] = in.y;
+    in3[3 * i + 2] = in.z;
+    f3[3 * i] = e.f.x;
+    f3[3 * i + 1] = e.f.y;
+    f3[3 * i + 2] = e.f.z;
+    cos[i] = e.cos;
+    pdf[i] = e.pdf;
+  }
+}
+
 }  // extern "C"

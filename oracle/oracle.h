@@ -147,6 +147,20 @@ int oracle_render(const oracle_scene* s, const oracle_render_desc* d, float* pix
 void oracle_filter(int32_t filter, int32_t width, int32_t height, int32_t spp, const float* samples,
                    float* pixels);
 
+/* ---- BSDFs alone (the shading kernel's lobes: LambertShader.cpp:27-73, NullShader.cpp:25-54,
+ * and the GGX reflection lobe of the Glossy shader), vectorised over n direction pairs. ------ */
+/* GGX normal distribution D(cos theta_h) and Smith masking G1(cos theta_v) for roughness alpha. */
+void oracle_ggx_d_n(float alpha, size_t n, const float* cos_h, float* d);
+void oracle_ggx_g1_n(float alpha, size_t n, const float* cos_v, float* g);
+/* ShaderInterface::evaluate: in3/out3/n3 are n x 3 unit vectors (light direction, view direction,
+ * shading normal); f3 n x 3, cos and pdf n each. */
+void oracle_bsdf_eval_n(const oracle_shader_desc* sh, size_t n, const float* in3, const float* out3,
+                        const float* n3, float* f3, float* cos, float* pdf);
+/* ShaderInterface::sample from the uniforms u2 (n x 2): the sampled direction in in3 with its
+ * evaluation; ok[i] = 0 when the sample carries no energy (the path ends there). */
+void oracle_bsdf_sample_n(const oracle_shader_desc* sh, size_t n, const float* u2, const float* out3,
+                          const float* n3, float* in3, float* f3, float* cos, float* pdf, uint8_t* ok);
+
 #ifdef __cplusplus
 }
 #endif

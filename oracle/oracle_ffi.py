@@ -74,6 +74,10 @@ _SIG = {
     "oracle_render": (C.c_int, [_P, C.POINTER(RenderDesc), _P, _P, C.c_uint32, C.c_uint32, C.c_int,
                                 C.POINTER(Stats)]),
     "oracle_filter": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
+    "oracle_ggx_d_n": (None, [C.c_float, C.c_size_t, _P, _P]),
+    "oracle_ggx_g1_n": (None, [C.c_float, C.c_size_t, _P, _P]),
+    "oracle_bsdf_eval_n": (None, [C.POINTER(ShaderDesc), C.c_size_t, _P, _P, _P, _P, _P, _P]),
+    "oracle_bsdf_sample_n": (None, [C.POINTER(ShaderDesc), C.c_size_t, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -83,16 +87,22 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def load(path: str) -> C.CDLL:
+    """Binds every oracle symbol of the library at path (the oracle, or a copy built with a
+    planted mistake by a test)."""
+    L = C.CDLL(path)
+    for name, (res, args) in _SIG.items():
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
+    return L
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        for name, (res, args) in _SIG.items():
-            f = getattr(L, name)
-            f.restype, f.argtypes = res, args
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
 
 
@@ -235,3 +245,52 @@ def filter_image(filt, width, height, spp, samples):
     pix = np.zeros((height, width, 3), dtype=np.float32)
     lib().oracle_filter(filt, width, height, spp, _ptr(samples), _ptr(pix))
     return pix
+
+
+class Bsdf:
+    """One shader's lobe from the oracle (or from a library given by L), vectorised over
+    direction pairs: eval(in, out, n) and sample(u, out, n) as in ShaderInterface."""
+
+    def __init__(self, type_, colour=(1.0, 1.0, 1.0), reflectance=1.0, roughness=0.0, L=None):
+        self.L = L if L is not None else lib()
+        self.d = ShaderDesc(type_, (C.c_float * 3)(*colour), reflectance, roughness)
+        self.alpha = roughness
+
+    @staticmethod
+    def _v(a, n):
+        a = np.asarray(a, dtype=np.float32)
+        return np.ascontiguousarray(np.broadcast_to(a, (n, 3)) if a.ndim == 1 else a)
+
+    def eval(self, wi, wo, nrm):
+        n = max(np.asarray(wi).reshape(-1, 3).shape[0], np.asarray(wo).reshape(-1, 3).shape[0])
+        wi, wo, nrm = self._v(wi, n), self._v(wo, n), self._v(nrm, n)
+        f = np.zeros((n, 3), np.float32)
+        c = np.zeros(n, np.float32)
+        p = np.zeros(n, np.float32)
+        self.L.oracle_bsdf_eval_n(C.byref(self.d), n, _ptr(wi), _ptr(wo), _ptr(nrm), _ptr(f), _ptr(c), _ptr(p))
+        return f, c, p
+
+    def sample(self, u, wo, nrm):
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 2)
+        n = u.shape[0]
+        wo, nrm = self._v(wo, n), self._v(nrm, n)
+        wi = np.zeros((n, 3), np.float32)
+        f = np.zeros((n, 3), np.float32)
+        c = np.zeros(n, np.float32)
+        p = np.zeros(n, np.float32)
+        ok = np.zeros(n, np.uint8)
+        self.L.oracle_bsdf_sample_n(C.byref(self.d), n, _ptr(u), _ptr(wo), _ptr(nrm), _ptr(wi), _ptr(f), _ptr(c),
+                                    _ptr(p), _ptr(ok))
+        return wi, f, c, p, ok.astype(bool)
+
+    def ggx_d(self, cos_h):
+        x = np.ascontiguousarray(cos_h, dtype=np.float32).ravel()
+        out = np.zeros_like(x)
+        self.L.oracle_ggx_d_n(self.alpha, x.size, _ptr(x), _ptr(out))
+        return out.reshape(np.shape(cos_h))
+
+    def ggx_g1(self, cos_v):
+        x = np.ascontiguousarray(cos_v, dtype=np.float32).ravel()
+        out = np.zeros_like(x)
+        self.L.oracle_ggx_g1_n(self.alpha, x.size, _ptr(x), _ptr(out))
+        return out.reshape(np.shape(cos_v))
