@@ -119,11 +119,12 @@ typedef struct {
   uint64_t camera_rays;
   uint64_t closest_rays;   /* closest-hit queries traced (camera + continuation) */
   uint64_t shadow_rays;    /* occlusion queries traced */
-  uint64_t contributions;  /* radiance records reduced into samples */
   uint32_t waves;          /* wavefront iterations */
   uint32_t max_wave;       /* largest wave (rays) */
   double ms_total;         /* first raygen to resolved framebuffer, device time */
-  double ms_raygen, ms_sort_rays, ms_trace, ms_sort_hits, ms_shade, ms_shadow, ms_accumulate, ms_filter;
+  /* Device time per stage. Radiance is added to each sample in place by the shading and shadow
+   * kernels (Integrator.cpp:66-71, 171-173), so accumulation is part of ms_shade / ms_shadow. */
+  double ms_raygen, ms_sort_rays, ms_trace, ms_sort_hits, ms_shade, ms_shadow, ms_filter;
 } swr_stats;
 
 /* Pass structures of the radix sort: onesweep (one kernel per digit, decoupled look-back over
@@ -174,7 +175,9 @@ int swr_memcpy_h2d(swr_context* ctx, void* dst_dev, const void* src, size_t byte
 int swr_memcpy_d2h(swr_context* ctx, void* dst, const void* src_dev, size_t bytes);
 
 /* ---- sort and key primitives (stages of Pathtracer::process) ------------------------------ */
-/* Stable LSD radix sort of (key, value) pairs in place, onesweep with decoupled look-back.
+/* Stable LSD radix sort of (key, value) pairs in place, 8-bit digits, one pass per digit, each
+ * pass onesweep with decoupled look-back or reduce-then-scan, as the context tuning's
+ * sort_algorithm says (onesweep on a new context; swr_autotune may choose either).
  * Replaces the ray sort (RaySort.cpp:7-118, Pathtracer.cpp:321-327) and the hit sort
  * (tbb::parallel_sort with CompareHit, Pathtracer.cpp:335-339). Only the low key_bits bits
  * are sorted (1..32). With iota_values = 1 the input values are not read and the output

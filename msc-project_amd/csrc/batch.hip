@@ -165,8 +165,6 @@ int swr_batch_add(swr_batch* b, const swr_scene* scene, const swr_render_desc* d
     }
     // Validate now, so that a bad job is refused here and not at run time.
     swr::FrameParams p = swr::make_params(*desc, &scene->dev);
-    if ((uint64_t)p.width * p.height * p.spp > 0xFFFFFFFFull)
-      swr::fail(SWR_E_INVALID, "render: more than 2^32 - 1 samples in one iteration");
     b->jobs.push_back({(int)(it - b->scenes.begin()), *desc, p});
     b->done = false;
     if (job) *job = (int32_t)b->jobs.size() - 1;

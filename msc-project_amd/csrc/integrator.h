@@ -324,7 +324,7 @@ __device__ inline uint32_t slot_of_sid(const FrameParams& p, uint32_t sid) {
 
 // Host side (render.hip): frame constants of one job on one scene.
 CameraRec make_camera(const swr_camera_desc& d);
-uint32_t spp_of(const swr_render_desc& d);
+uint64_t spp_of(const swr_render_desc& d);
 FrameParams make_params(const swr_render_desc& d, const DeviceScene* s);
 int bits_for(uint64_t max_value);
 

@@ -728,11 +728,16 @@ CameraRec make_camera(const swr_camera_desc& d) {
   return c;
 }
 
-uint32_t spp_of(const swr_render_desc& d) { return d.spp > 0 ? (uint32_t)d.spp : (uint32_t)(d.base * d.base); }
+uint64_t spp_of(const swr_render_desc& d) {
+  return d.spp > 0 ? (uint64_t)d.spp : d.base > 0 ? (uint64_t)d.base * (uint64_t)d.base : 0u;
+}
 
 FrameParams make_params(const swr_render_desc& d, const DeviceScene* s) {
   if (d.width <= 0 || d.height <= 0) fail(SWR_E_INVALID, "render: width and height must be positive");
   if (d.base <= 0 && d.spp <= 0) fail(SWR_E_INVALID, "render: sample base or spp must be positive");
+  if (d.base > 65535) fail(SWR_E_INVALID, "render: sample base above 65535 (spp = base^2 must fit 32 bits)");
+  if ((uint64_t)d.width * (uint64_t)d.height * spp_of(d) > 0xFFFFFFFFull)
+    fail(SWR_E_INVALID, "render: more than 2^32 - 1 samples in one iteration");
   if (d.sampler < 0 || d.sampler > 2 || d.filter < 0 || d.filter > 1 || d.camera.type < 0 || d.camera.type > 1)
     fail(SWR_E_INVALID, "render: unknown sampler, filter or camera type");
   if (!(d.threshold > 0.f)) fail(SWR_E_INVALID, "render: threshold must be positive");
@@ -742,7 +747,7 @@ FrameParams make_params(const swr_render_desc& d, const DeviceScene* s) {
   FrameParams p{};
   p.width = (uint32_t)d.width;
   p.height = (uint32_t)d.height;
-  p.spp = spp_of(d);
+  p.spp = (uint32_t)spp_of(d);
   p.base = d.base > 0 ? (uint32_t)d.base : 1u;
   if (d.sampler != SWR_SAMPLER_INDEPENDENT && p.base * p.base != p.spp)
     fail(SWR_E_INVALID, "render: stratified and grid samplers need spp = base^2 (Image.h:65)");
@@ -895,8 +900,6 @@ FrameParams WaveLoop::params(uint32_t scene, const swr_render_desc& desc, uint64
                              uint64_t pixel_base) const {
   if (scene >= scenes_.size()) fail(SWR_E_INVALID, "wavefront: scene index out of range");
   FrameParams p = make_params(desc, &scenes_[scene]->dev);
-  if ((uint64_t)p.width * p.height * p.spp > 0xFFFFFFFFull)
-    fail(SWR_E_INVALID, "render: more than 2^32 - 1 samples in one iteration");
   if (p.max_depth > 65534) fail(SWR_E_INVALID, "render: max depth above 65534");
   p.ray_key_shift = (uint32_t)(32 - ray_bits_);
   p.key_scene_bits = (uint32_t)scene_bits_;

@@ -420,8 +420,6 @@ int swr_server_submit(swr_server* s, int32_t scene, const swr_render_desc* desc,
     if (scene < 0 || scene >= (int32_t)s->scenes.size()) swr::fail(SWR_E_INVALID, "server: scene index out of range");
     if (iterations < 1) swr::fail(SWR_E_INVALID, "server: iterations must be positive");
     const swr::FrameParams p = swr::make_params(*desc, &s->scenes[scene]->dev);
-    if ((uint64_t)p.width * p.height * p.spp > 0xFFFFFFFFull)
-      swr::fail(SWR_E_INVALID, "render: more than 2^32 - 1 samples in one iteration");
     if (p.max_depth > 65534) swr::fail(SWR_E_INVALID, "render: max depth above 65534");
     auto r = std::make_unique<swr::Request>();
     r->scene = scene;

@@ -8,6 +8,7 @@
 //   clear(), active(), terminate() as in Pathtracer.cpp:402-423.
 #pragma once
 
+#include <atomic>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -44,9 +45,11 @@ class Pathtracer {
     std::fill(pixels_.begin(), pixels_.end(), 0.f);
     iteration_ = 0;
   }
-  bool active() const { return !terminated_; }
+  // terminate() may come from another thread than process() (the reference's viewer thread,
+  // main.cpp:83-102, through boost::atomic<bool> m_terminate, Pathtracer.h:98).
+  bool active() const { return !terminated_.load(std::memory_order_acquire); }
   void terminate() {
-    terminated_ = true;
+    terminated_.store(true, std::memory_order_release);
     swr_cancel(ctx_);
   }
   int process() {
@@ -67,7 +70,7 @@ class Pathtracer {
   swr_stats stats_{};
   std::vector<float> pixels_, frame_;
   int iteration_ = 0;
-  bool terminated_ = false;
+  std::atomic<bool> terminated_{false};
 };
 
 }  // namespace swr

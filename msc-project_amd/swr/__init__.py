@@ -108,10 +108,10 @@ class ServerTotals(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("camera_rays", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
-                ("contributions", C.c_uint64), ("waves", C.c_uint32), ("max_wave", C.c_uint32),
+                ("waves", C.c_uint32), ("max_wave", C.c_uint32),
                 ("ms_total", C.c_double), ("ms_raygen", C.c_double), ("ms_sort_rays", C.c_double),
                 ("ms_trace", C.c_double), ("ms_sort_hits", C.c_double), ("ms_shade", C.c_double),
-                ("ms_shadow", C.c_double), ("ms_accumulate", C.c_double), ("ms_filter", C.c_double)]
+                ("ms_shadow", C.c_double), ("ms_filter", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

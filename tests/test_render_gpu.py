@@ -169,6 +169,19 @@ def test_render_errors_and_cancel(ctx):
     rd, _ = _pair(desc, base=3, spp=8, sampler=0)  # stratified needs spp = base^2
     with pytest.raises(swr.SwrError):
         sc.render(rd)
+    # spp = base^2 must fit 32 bits (base 65536 would wrap to 0 in 32-bit arithmetic), and so
+    # must the samples of one iteration: refused before anything is allocated or launched
+    for over in (dict(base=65536), dict(base=46341), dict(width=4096, height=4096, base=300),
+                 dict(base=0, spp=2 ** 31 - 1, sampler=1)):
+        rd, _ = _pair(desc, **over)
+        with pytest.raises(swr.SwrError) as e:
+            sc.render(rd)
+        assert e.value.code == swr.SWR_E_INVALID, over
+        b = swr.Batch(ctx)
+        with pytest.raises(swr.SwrError) as e:
+            b.add(sc, rd)
+        assert e.value.code == swr.SWR_E_INVALID, over
+        b.close()
     rd, _ = _pair(desc)
     ctx.cancel()
     with pytest.raises(swr.SwrError) as e:
