@@ -972,13 +972,16 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
   const bool one_job = max_jobs_ == 1 && scenes_.size() == 1, one_scene = scenes_.size() == 1;
   if (kb) {
     if (one_job)
-      raygen_kernel<true><<<blocks(kb), 256, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
+      raygen_kernel<true><<<checked_grid(blocks(kb), 256, "raygen"), 256, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
     else
-      raygen_kernel<false><<<blocks(kb), 256, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
+      raygen_kernel<false><<<checked_grid(blocks(kb), 256, "raygen"), 256, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
   }
   mark(1);
-  const uint32_t shade_blocks = blocks(nb, one_job ? kShadeBlock : kShadeBlockMulti);
+  const uint32_t shade_blocks = nb ? checked_grid(blocks(nb, one_job ? kShadeBlock : kShadeBlockMulti),
+                                                 one_job ? kShadeBlock : kShadeBlockMulti, "shade")
+                                    : 0u;
   if (nb) {
+    const uint32_t trace_grid = checked_grid(blocks(nb, kTraceBlock), kTraceBlock, "trace");
     uint32_t *rk = nullptr, *rperm = nullptr;
     sort_pairs(ctx_, keys[cur], perm, alt_k, alt_v, nb, ray_bits_, true, &rk, &rperm, st, n_dev);
     mark(2);
@@ -986,33 +989,33 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
     for (int ring : ring_set_) {
       with_ring(ring, [&](auto kShort) {
         if (one_scene) {
-          trace_queue_kernel<kShort.value, false, true><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+          trace_queue_kernel<kShort.value, false, true><<<trace_grid, kTraceBlock, 0, st>>>(
               stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_, spill,
               spill_stride_);
         } else if (kSceneTiles) {
           // Scene-uniform tiles, then the tiles at scene boundaries (at most one per scene).
           const uint32_t nscenes = (uint32_t)scenes_.size();
           if (filter) {
-            trace_queue_kernel<kShort.value, true, false, kTileOne><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+            trace_queue_kernel<kShort.value, true, false, kTileOne><<<trace_grid, kTraceBlock, 0, st>>>(
                 stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_,
                 spill, spill_stride_);
-            trace_queue_kernel<kShort.value, true, false, kTileMixed><<<nscenes - 1, kTraceBlock, 0, st>>>(
+            trace_queue_kernel<kShort.value, true, false, kTileMixed><<<checked_grid(nscenes - 1, kTraceBlock, "trace (scene boundaries)"), kTraceBlock, 0, st>>>(
                 stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_,
                 spill, spill_stride_);
           } else {
-            trace_queue_kernel<kShort.value, false, false, kTileOne><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+            trace_queue_kernel<kShort.value, false, false, kTileOne><<<trace_grid, kTraceBlock, 0, st>>>(
                 stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_,
                 spill, spill_stride_);
-            trace_queue_kernel<kShort.value, false, false, kTileMixed><<<nscenes - 1, kTraceBlock, 0, st>>>(
+            trace_queue_kernel<kShort.value, false, false, kTileMixed><<<checked_grid(nscenes - 1, kTraceBlock, "trace (scene boundaries)"), kTraceBlock, 0, st>>>(
                 stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_,
                 spill, spill_stride_);
           }
         } else if (filter) {
-          trace_queue_kernel<kShort.value, true, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+          trace_queue_kernel<kShort.value, true, false><<<trace_grid, kTraceBlock, 0, st>>>(
               stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_, spill,
               spill_stride_);
         } else {
-          trace_queue_kernel<kShort.value, false, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+          trace_queue_kernel<kShort.value, false, false><<<trace_grid, kTraceBlock, 0, st>>>(
               stab, sring, ring, rperm, rk, scene_shift, w, queue[cur], hit_keys, hit_shift_, spill,
               spill_stride_);
         }
@@ -1044,13 +1047,13 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
     for (int ring : ring_set_) {
       with_ring(ring, [&](auto kShort) {
         if (filter)
-          shadow_kernel<kShort.value, true, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+          shadow_kernel<kShort.value, true, false><<<trace_grid, kTraceBlock, 0, st>>>(
               stab, sring, ring, w, so.shadow, radiance_, spill, spill_stride_);
         else if (one_scene)
-          shadow_kernel<kShort.value, false, true><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+          shadow_kernel<kShort.value, false, true><<<trace_grid, kTraceBlock, 0, st>>>(
               stab, sring, ring, w, so.shadow, radiance_, spill, spill_stride_);
         else
-          shadow_kernel<kShort.value, false, false><<<blocks(nb, kTraceBlock), kTraceBlock, 0, st>>>(
+          shadow_kernel<kShort.value, false, false><<<trace_grid, kTraceBlock, 0, st>>>(
               stab, sring, ring, w, so.shadow, radiance_, spill, spill_stride_);
       });
     }

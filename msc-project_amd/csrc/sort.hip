@@ -443,7 +443,8 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
   const bool scanned = ctx->tuning.sort_algorithm == SWR_SORT_REDUCE_SCAN;
   const int kpt = ctx->tuning.sort_kpt;
   const uint32_t tile_keys = (uint32_t)(kSortBlock * kpt);
-  const uint32_t tiles = (uint32_t)((n + tile_keys - 1) / tile_keys);
+  const uint32_t tiles = checked_grid((n + tile_keys - 1) / tile_keys, kSortBlock, "sort pass");
+  const uint32_t up_groups = checked_grid((tiles + kUpTiles - 1) / kUpTiles, kSortBlock, "sort upsweep");
   if (!scanned) {
     sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters, passes);
     const int hist_blocks = (int)std::max<size_t>(
@@ -460,7 +461,7 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
     // totals land in hist), then the pass kernel with the scanned per-tile offsets.
 #define SWR_PASS(KPT)                                                                                           \
   if (scanned) {                                                                                                \
-    upsweep_kernel<KPT><<<(tiles + kUpTiles - 1) / kUpTiles, kSortBlock, 0, stream>>>(sk, (uint32_t)n, n_dev,    \
+    upsweep_kernel<KPT><<<up_groups, kSortBlock, 0, stream>>>(sk, (uint32_t)n, n_dev,    \
         8 * p, tile_counts, ws.tile_stride);                                                                    \
     tile_scan_kernel<<<kRadix, 256, 0, stream>>>(tile_counts, ws.tile_stride, (uint32_t)n, n_dev, tile_keys,     \
                                                  hist + p * kRadix);                                            \

@@ -24,6 +24,17 @@ struct Error {
 
 [[noreturn]] inline void fail(int code, const std::string& msg) { throw Error{code, msg}; }
 
+// Launch shape of a kernel, checked on the host before the launch is issued or captured into a
+// graph: a grid of at least one workgroup, 1..1024 lanes per workgroup and fewer than 2^32
+// work-items in all (the dispatch packet's grid size is a 32-bit count of work-items). A shape
+// outside these is refused with SWR_E_INVALID instead of reaching the hardware queue.
+inline uint32_t checked_grid(uint64_t groups, uint32_t block, const char* kernel) {
+  if (groups == 0 || block == 0 || block > 1024 || groups * block > 0xFFFFFFFFull)
+    fail(SWR_E_INVALID, std::string("launch shape out of range for ") + kernel + ": " + std::to_string(groups) +
+                            " workgroups of " + std::to_string(block) + " lanes");
+  return (uint32_t)groups;
+}
+
 #define SWR_HIP(call)                                                                         \
   do {                                                                                        \
     hipError_t e_ = (call);                                                                   \

@@ -216,9 +216,15 @@ class Scene:
         return self._query(lib().oracle_occluded, rays, mask, 1)
 
     def render(self, rdesc: RenderDesc, samples=False, sid_begin=0, sid_end=0, threads=0, pixels=True):
+        """samples: False, True (a new array) or a float32 array of W*H*spp x 5 to fill."""
         spp = rdesc.spp if rdesc.spp > 0 else rdesc.base * rdesc.base
         pix = np.zeros((rdesc.height, rdesc.width, 3), dtype=np.float32) if pixels else None
-        smp = np.zeros((rdesc.width * rdesc.height * spp, 5), dtype=np.float32) if samples else None
+        if isinstance(samples, np.ndarray):
+            assert samples.dtype == np.float32 and samples.flags.c_contiguous
+            assert samples.shape == (rdesc.width * rdesc.height * spp, 5)
+            smp = samples
+        else:
+            smp = np.zeros((rdesc.width * rdesc.height * spp, 5), dtype=np.float32) if samples else None
         st = Stats()
         assert lib().oracle_render(self.h, C.byref(rdesc), _ptr(pix), _ptr(smp), sid_begin, sid_end, threads,
                                    C.byref(st)) == 0
