@@ -179,11 +179,21 @@ int swr_memcpy_d2h(swr_context* ctx, void* dst, const void* src_dev, size_t byte
  * pass onesweep with decoupled look-back or reduce-then-scan, as the context tuning's
  * sort_algorithm says (onesweep on a new context; swr_autotune may choose either).
  * Replaces the ray sort (RaySort.cpp:7-118, Pathtracer.cpp:321-327) and the hit sort
- * (tbb::parallel_sort with CompareHit, Pathtracer.cpp:335-339). Only the low key_bits bits
- * are sorted (1..32). With iota_values = 1 the input values are not read and the output
+ * (tbb::parallel_sort with CompareHit, Pathtracer.cpp:335-339). key_bits (1..32) is the key
+ * width: keys must lie below 2^key_bits, and the passes order them by their ceil(key_bits/8) low
+ * bytes. With iota_values = 1 the input values are not read and the output
  * values are the stable permutation (original index of each sorted key). n <= 2^30. */
 int swr_sort_pairs_u32(swr_context* ctx, uint32_t* keys_dev, uint32_t* vals_dev, size_t n, int key_bits,
                        int iota_values);
+/* The same sort (same arguments, same result) without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Its workspace is a file-scope device array of the library
+ * (about 8.3 MiB per device, held until the process ends): at most 2^20 pairs per call, larger n
+ * is refused with SWR_E_INVALID. Always the reduce-then-scan pass structure, at the context's
+ * keys-per-lane tuning. Every call on one device shares that array, so calls that overlap on a
+ * device (two contexts, two threads) race; calls on one context are stream-ordered and safe. */
+int swr_sort_pairs_u32_noalloc(swr_context* ctx, uint32_t* keys_dev, uint32_t* vals_dev, size_t n, int key_bits,
+                               int iota_values);
 /* Ray binning key: octant(3 bits) << 29 | morton29(quantised origin), the octant+Morton
  * replacement for the cardinal bins (Camera.cpp:42-46, Integrator.cpp:240-244).
  * rays_dev: n x 8 floats (ox,oy,oz,tmin,dx,dy,dz,tmax). scale[i] = 2^b_i/(hi_i-lo_i) with

@@ -95,6 +95,7 @@ int swr_create(int device, swr_context** out) {
     ctx->num_cus = prop.multiProcessorCount;
     SWR_HIP(hipEventCreate(&ctx->ev0));
     SWR_HIP(hipEventCreate(&ctx->ev1));
+    swr::sort_scratch_init(ctx);  // with `device` current
   });
   if (rc != SWR_OK) {
     delete ctx;
@@ -231,6 +232,20 @@ int swr_sort_pairs_u32(swr_context* ctx, uint32_t* keys_dev, uint32_t* vals_dev,
       SWR_HIP(hipMemcpyAsync(keys_dev, ok, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
       SWR_HIP(hipMemcpyAsync(vals_dev, ov, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
     }
+    SWR_HIP(hipStreamSynchronize(ctx->stream));
+    timer.finish();
+  });
+}
+
+int swr_sort_pairs_u32_noalloc(swr_context* ctx, uint32_t* keys_dev, uint32_t* vals_dev, size_t n, int key_bits,
+                               int iota_values) {
+  if (!ctx) return SWR_E_INVALID;
+  return guarded(ctx, [&] {
+    if (n && (!keys_dev || !vals_dev)) swr::fail(SWR_E_INVALID, "sort: null buffer");
+    SWR_HIP(hipSetDevice(ctx->device));
+    swr::Timer timer(ctx);
+    swr::sort_pairs_noalloc(ctx, keys_dev, vals_dev, n, key_bits, iota_values != 0, ctx->stream);
+    timer.stop();
     SWR_HIP(hipStreamSynchronize(ctx->stream));
     timer.finish();
   });

@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
     }
   }
   const uint64_t tag = (uint64_t)epoch << 32;
-  uint64_t* my_status = status + (size_t)tile * kRadix + t;
+  uint64_t* my_status = kScanned ? nullptr : status + (size_t)tile * kRadix + t;
   if (!kScanned && t < kRadix)
     __hip_atomic_store(my_status, tag | (uint64_t)((tile == 0 ? kFlagPre : kFlagAgg) | total), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
@@ -416,6 +416,81 @@ void sort_epoch_guard(swr_context* ctx, uint64_t passes_ahead, hipStream_t strea
   if (ctx->sort.epoch + passes_ahead >= 0xFFFFFF00ull) sort_reset_epochs(ctx, stream);
 }
 
+namespace {
+
+// Device buffers of one sort besides the pairs and their ping-pong partners.
+struct PassBuffers {
+  uint32_t* hist;         // digit totals (onesweep: kHistCopies copies summed by the pass kernel)
+  uint32_t* counters;     // onesweep tile tickets and epoch counter
+  uint64_t* status;       // onesweep look-back words
+  uint32_t* tile_counts;  // reduce-then-scan per-tile digit counts, 256 rows of tile_stride
+  uint32_t tile_stride;
+};
+
+// Launches the passes of one sort (histogram first for onesweep); shapes checked on the host.
+void launch_passes(const PassBuffers& b, bool scanned, int kpt, int num_cus, uint32_t* keys, uint32_t* vals,
+                   uint32_t* alt_keys, uint32_t* alt_vals, size_t n, int passes, bool iota, hipStream_t stream,
+                   const uint32_t* n_dev) {
+  uint32_t* hist = b.hist;
+  uint32_t* counters = b.counters;
+  uint64_t* status = b.status;
+  uint32_t* tile_counts = b.tile_counts;
+  const uint32_t tile_stride = b.tile_stride;
+  const uint32_t tile_keys = (uint32_t)(kSortBlock * kpt);
+  const uint32_t tiles = checked_grid((n + tile_keys - 1) / tile_keys, kSortBlock, "sort pass");
+  const uint32_t up_groups = checked_grid((tiles + kUpTiles - 1) / kUpTiles, kSortBlock, "sort upsweep");
+  if (scanned && (uint64_t)up_groups * kUpTiles > tile_stride)
+    fail(SWR_E_INVALID, "sort: per-tile count rows shorter than the tile count");
+  if (!scanned) {
+    sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters, passes);
+    const int hist_blocks = (int)std::max<size_t>(
+        1, std::min<size_t>((n + 8 * kHistBlock - 1) / (8 * kHistBlock), (size_t)std::max(num_cus, 1)));
+    if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0)
+      sort_histogram_kernel<true><<<hist_blocks, kHistBlock, 0, stream>>>(keys, n, n_dev, passes, hist);
+    else
+      sort_histogram_kernel<false><<<hist_blocks, kHistBlock, 0, stream>>>(keys, n, n_dev, passes, hist);
+  }
+  uint32_t *sk = keys, *sv = vals, *dk = alt_keys, *dv = alt_vals;
+  for (int p = 0; p < passes; ++p) {
+    const bool first_iota = iota && p == 0;
+    // Onesweep: the pass kernel alone. Reduce-then-scan: upsweep, per-digit scan (the digit
+    // totals land in hist), then the pass kernel with the scanned per-tile offsets.
+#define SWR_PASS(KPT)                                                                                           \
+  if (scanned) {                                                                                                \
+    upsweep_kernel<KPT><<<up_groups, kSortBlock, 0, stream>>>(sk, (uint32_t)n, n_dev, 8 * p, tile_counts,        \
+                                                              tile_stride);                                     \
+    tile_scan_kernel<<<kRadix, 256, 0, stream>>>(tile_counts, tile_stride, (uint32_t)n, n_dev, tile_keys,        \
+                                                 hist + p * kRadix);                                            \
+    if (first_iota)                                                                                             \
+      onesweep_kernel<true, KPT, true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,     \
+          8 * p, hist + p * kRadix, 1, counters, p, passes, status, tile_counts, tile_stride);                   \
+    else                                                                                                        \
+      onesweep_kernel<false, KPT, true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,    \
+          8 * p, hist + p * kRadix, 1, counters, p, passes, status, tile_counts, tile_stride);                   \
+  } else if (first_iota) {                                                                                      \
+    onesweep_kernel<true, KPT, false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,      \
+        8 * p, hist + p * kRadix, kHistCopies, counters, p, passes, status, tile_counts, tile_stride);           \
+  } else {                                                                                                      \
+    onesweep_kernel<false, KPT, false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,     \
+        8 * p, hist + p * kRadix, kHistCopies, counters, p, passes, status, tile_counts, tile_stride);           \
+  }
+    switch (kpt) {
+      case 8: SWR_PASS(8) break;
+      case 12: SWR_PASS(12) break;
+      case 16: SWR_PASS(16) break;
+      case 20: SWR_PASS(20) break;
+      case 24: SWR_PASS(24) break;
+      default: fail(SWR_E_INVALID, "sort: keys per thread must be 8, 12, 16, 20 or 24");
+    }
+#undef SWR_PASS
+    std::swap(sk, dk);
+    std::swap(sv, dv);
+  }
+  SWR_HIP(hipGetLastError());
+}
+
+}  // namespace
+
 void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals, size_t n,
                 int key_bits, bool iota, uint32_t** out_keys, uint32_t** out_vals, hipStream_t stream,
                 const uint32_t* n_dev) {
@@ -436,61 +511,58 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
     sort_epoch_guard(ctx, (uint64_t)passes, stream);
   }
   ws.epoch += (uint64_t)passes;  // host mirror of the device epoch counter
-  uint32_t* hist = static_cast<uint32_t*>(ws.hist.ptr);
-  uint32_t* counters = static_cast<uint32_t*>(ws.counters.ptr);
-  uint64_t* status = static_cast<uint64_t*>(ws.status.ptr);
-  uint32_t* tile_counts = static_cast<uint32_t*>(ws.tile_counts.ptr);
-  const bool scanned = ctx->tuning.sort_algorithm == SWR_SORT_REDUCE_SCAN;
-  const int kpt = ctx->tuning.sort_kpt;
-  const uint32_t tile_keys = (uint32_t)(kSortBlock * kpt);
-  const uint32_t tiles = checked_grid((n + tile_keys - 1) / tile_keys, kSortBlock, "sort pass");
-  const uint32_t up_groups = checked_grid((tiles + kUpTiles - 1) / kUpTiles, kSortBlock, "sort upsweep");
-  if (!scanned) {
-    sort_prepare_kernel<<<1, kRadix, 0, stream>>>(hist, counters, passes);
-    const int hist_blocks = (int)std::max<size_t>(
-        1, std::min<size_t>((n + 8 * kHistBlock - 1) / (8 * kHistBlock), (size_t)std::max(ctx->num_cus, 1)));
-    if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0)
-      sort_histogram_kernel<true><<<hist_blocks, kHistBlock, 0, stream>>>(keys, n, n_dev, passes, hist);
-    else
-      sort_histogram_kernel<false><<<hist_blocks, kHistBlock, 0, stream>>>(keys, n, n_dev, passes, hist);
+  const PassBuffers b{static_cast<uint32_t*>(ws.hist.ptr), static_cast<uint32_t*>(ws.counters.ptr),
+                      static_cast<uint64_t*>(ws.status.ptr), static_cast<uint32_t*>(ws.tile_counts.ptr),
+                      ws.tile_stride};
+  launch_passes(b, ctx->tuning.sort_algorithm == SWR_SORT_REDUCE_SCAN, ctx->tuning.sort_kpt, ctx->num_cus, keys,
+                vals, alt_keys, alt_vals, n, passes, iota, stream, n_dev);
+}
+
+// ---- the sort without allocation ----
+// Everything the reduce-then-scan passes need besides the caller's pairs lives in one
+// file-scope device array: the ping-pong partners of keys and values, the per-tile digit counts
+// (tiles of the smallest size, rounded up to whole upsweep groups) and the digit totals. No word
+// of it needs clearing: each pass writes every count and total it reads. Each device that loads
+// the library holds its own copy until the process ends.
+namespace {
+constexpr size_t kScratchTiles =
+    ((kNoAllocMaxPairs + kSortBlock * kSortMinKeysPerThread - 1) / (kSortBlock * kSortMinKeysPerThread) + kUpTiles -
+     1) / kUpTiles * kUpTiles;
+constexpr size_t kScratchWords = 2 * kNoAllocMaxPairs + kRadix * kScratchTiles + 4 * kRadix;
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+}  // namespace
+
+size_t sort_scratch_bytes() { return sizeof(uint32_t) * kScratchWords; }
+
+void sort_scratch_init(swr_context* ctx) {
+  void* p = nullptr;
+  SWR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) fail(SWR_E_HIP, "sort: no device address for the sort scratch array");
+  ctx->sort_scratch = static_cast<uint32_t*>(p);
+}
+
+void sort_pairs_noalloc(swr_context* ctx, uint32_t* keys, uint32_t* vals, size_t n, int key_bits, bool iota,
+                        hipStream_t stream) {
+  if (key_bits < 1 || key_bits > 32) fail(SWR_E_INVALID, "sort: key_bits must be in 1..32");
+  if (n > kNoAllocMaxPairs)
+    fail(SWR_E_INVALID, "sort without allocation: " + std::to_string(n) + " pairs exceed its maximum of " +
+                            std::to_string(kNoAllocMaxPairs));
+  if (n == 0) return;
+  uint32_t* base = ctx->sort_scratch;  // the device address of `scratch` on this context's device
+  if (!base) fail(SWR_E_INVALID, "sort without allocation: scratch address not set up");
+  uint32_t* alt_keys = base;
+  uint32_t* alt_vals = base + kNoAllocMaxPairs;
+  uint32_t* tile_counts = base + 2 * kNoAllocMaxPairs;
+  uint32_t* hist = tile_counts + kRadix * kScratchTiles;
+  const int passes = (key_bits + 7) / 8;
+  // Reduce-then-scan only: its passes use no look-back words, tickets or epochs.
+  const PassBuffers b{hist, nullptr, nullptr, tile_counts, (uint32_t)kScratchTiles};
+  launch_passes(b, true, ctx->tuning.sort_kpt, ctx->num_cus, keys, vals, alt_keys, alt_vals, n, passes, iota, stream,
+                nullptr);
+  if (passes & 1) {
+    SWR_HIP(hipMemcpyAsync(keys, alt_keys, n * 4, hipMemcpyDeviceToDevice, stream));
+    SWR_HIP(hipMemcpyAsync(vals, alt_vals, n * 4, hipMemcpyDeviceToDevice, stream));
   }
-  uint32_t *sk = keys, *sv = vals, *dk = alt_keys, *dv = alt_vals;
-  for (int p = 0; p < passes; ++p) {
-    const bool first_iota = iota && p == 0;
-    // Onesweep: the pass kernel alone. Reduce-then-scan: upsweep, per-digit scan (the digit
-    // totals land in hist), then the pass kernel with the scanned per-tile offsets.
-#define SWR_PASS(KPT)                                                                                           \
-  if (scanned) {                                                                                                \
-    upsweep_kernel<KPT><<<up_groups, kSortBlock, 0, stream>>>(sk, (uint32_t)n, n_dev,    \
-        8 * p, tile_counts, ws.tile_stride);                                                                    \
-    tile_scan_kernel<<<kRadix, 256, 0, stream>>>(tile_counts, ws.tile_stride, (uint32_t)n, n_dev, tile_keys,     \
-                                                 hist + p * kRadix);                                            \
-    if (first_iota)                                                                                             \
-      onesweep_kernel<true, KPT, true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,     \
-          8 * p, hist + p * kRadix, scanned ? 1 : kHistCopies, counters, p, passes, status, tile_counts, ws.tile_stride);                                   \
-    else                                                                                                        \
-      onesweep_kernel<false, KPT, true><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,    \
-          8 * p, hist + p * kRadix, scanned ? 1 : kHistCopies, counters, p, passes, status, tile_counts, ws.tile_stride);                                   \
-  } else if (first_iota) {                                                                                      \
-    onesweep_kernel<true, KPT, false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,      \
-        8 * p, hist + p * kRadix, scanned ? 1 : kHistCopies, counters, p, passes, status, tile_counts, ws.tile_stride);                                     \
-  } else {                                                                                                      \
-    onesweep_kernel<false, KPT, false><<<tiles, kSortBlock, 0, stream>>>(sk, sv, dk, dv, (uint32_t)n, n_dev,     \
-        8 * p, hist + p * kRadix, scanned ? 1 : kHistCopies, counters, p, passes, status, tile_counts, ws.tile_stride);                                     \
-  }
-    switch (kpt) {
-      case 8: SWR_PASS(8) break;
-      case 12: SWR_PASS(12) break;
-      case 16: SWR_PASS(16) break;
-      case 20: SWR_PASS(20) break;
-      case 24: SWR_PASS(24) break;
-      default: fail(SWR_E_INVALID, "sort: keys per thread must be 8, 12, 16, 20 or 24");
-    }
-#undef SWR_PASS
-    std::swap(sk, dk);
-    std::swap(sv, dv);
-  }
-  SWR_HIP(hipGetLastError());
 }
 
 }  // namespace swr

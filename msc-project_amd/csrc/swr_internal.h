@@ -90,6 +90,7 @@ struct swr_context {
   swr::DeviceBuffer trace_spill;             // traversal stack spill of the stand-alone queries
   std::unordered_set<swr_scene*> scenes;     // live scenes; destroying the context detaches them
   int num_cus = 0;
+  uint32_t* sort_scratch = nullptr;          // device address of sort.hip's scratch array on `device`
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // brackets of the last timed entry point
   double last_ms = 0.0;                     // its device time
 };
@@ -138,6 +139,16 @@ size_t sort_status_bytes(size_t n);
 void sort_reserve(swr_context* ctx, size_t n);
 constexpr int kSortBlock = 512;
 constexpr int kSortMinKeysPerThread = 8;
+// The sort without allocation (swr_sort_pairs_u32_noalloc): at most this many pairs, with all
+// of its scratch in a file-scope device array of sort.hip (sort_scratch_bytes(), about 8.3 MiB).
+constexpr size_t kNoAllocMaxPairs = size_t(1) << 20;
+size_t sort_scratch_bytes();
+// Looks up that array's device address on the current device (once per context, at creation).
+void sort_scratch_init(swr_context* ctx);
+// Reduce-then-scan passes with the scratch array as workspace; the sorted pairs end in keys/vals.
+// Stream-ordered; calls on one device share the array, so calls that overlap on a device race.
+void sort_pairs_noalloc(swr_context* ctx, uint32_t* keys, uint32_t* vals, size_t n, int key_bits, bool iota,
+                        hipStream_t stream);
 
 // ---- keys.hip ----
 void launch_ray_keys(const float4* rays, size_t n, float3 lo, float3 scale, uint32_t* keys, hipStream_t s);

@@ -137,6 +137,7 @@ _SIG = {
     "swr_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "swr_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "swr_sort_pairs_u32": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_int, C.c_int]),
+    "swr_sort_pairs_u32_noalloc": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_int, C.c_int]),
     "swr_ray_keys": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_float), C.POINTER(C.c_float), _P]),
     "swr_hit_keys": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "swr_mesh_face_keys": (C.c_int, [_P, _P, _P, C.c_size_t, _P]),
@@ -296,6 +297,11 @@ class Context:
     # -- primitives -----------------------------------------------------------------------
     def sort_pairs(self, keys: "DeviceArray", vals: "DeviceArray", n: int, key_bits: int = 32, iota: bool = True):
         self.check(self.lib.swr_sort_pairs_u32(self.h, keys.ptr, vals.ptr, n, key_bits, int(iota)))
+
+    def sort_pairs_noalloc(self, keys: "DeviceArray", vals: "DeviceArray", n: int, key_bits: int = 32,
+                           iota: bool = True):
+        """swr_sort_pairs_u32_noalloc: the same sort with the library's static scratch (n <= 2^20)."""
+        self.check(self.lib.swr_sort_pairs_u32_noalloc(self.h, keys.ptr, vals.ptr, n, key_bits, int(iota)))
 
     def ray_keys(self, rays: "DeviceArray", n: int, lo, scale, keys: "DeviceArray"):
         lo_a = (C.c_float * 3)(*lo)
