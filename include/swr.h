@@ -271,7 +271,12 @@ int swr_raygen(swr_context* ctx, const swr_render_desc* desc, uint32_t sid0, siz
 /* One iteration: every sample traced through the sorted wavefront loop, reduced per sample and
  * filtered. pixels: host W*H*3 floats, the filtered iteration (row-major y*W+x as Pixel in
  * Image.h:16-20), or NULL to leave it in device memory; samples: optional host W*H*spp*5 floats
- * (x, y, r, g, b) in reference sampleID order (Camera.cpp:28). stats may be NULL. */
+ * (x, y, r, g, b) in reference sampleID order (Camera.cpp:28). stats may be NULL.
+ * Out of core (the role of the reference's bin files, DirectionalBins.cpp:37-115): when the
+ * frame's per-sample state (24 B per sample, 44 B with samples requested) does not fit the
+ * context budget, the frame is traced as column bands one after another, each band holding its
+ * columns plus the tent filter's neighbour columns; the image and sample records are identical
+ * to an unbanded render. SWR_E_BUDGET if the framebuffer or one column of samples cannot fit. */
 int swr_render(swr_context* ctx, const swr_scene* scene, const swr_render_desc* desc, float* pixels, float* samples,
                swr_stats* stats);
 
@@ -338,7 +343,10 @@ int swr_server_create(swr_context* ctx, const swr_scene* const* scenes, int32_t 
 /* Queues a request: desc gives one iteration (image size, spp, sampler, filter, camera, depth,
  * seed; iteration i uses desc->iteration + i as its RNG counter word) and iterations its count.
  * pixels (host, W*H*3 floats) receives the result and must stay valid until swr_server_wait
- * returns; it may be NULL. Thread-safe. SWR_E_BUDGET if the request can never fit. */
+ * returns; it may be NULL. Thread-safe. An iteration whose per-sample state exceeds the server's
+ * per-sample arena runs as column bands (each a job of its own, at most half the arena each,
+ * the tent's neighbour columns traced by both neighbours), with the same result as unbanded.
+ * SWR_E_BUDGET if the framebuffer or one column of samples can never fit. */
 int swr_server_submit(swr_server* server, int32_t scene, const swr_render_desc* desc, int32_t iterations,
                       float* pixels, uint64_t* request);
 /* Blocks until the request has finished (or failed); thread-safe. */

@@ -69,6 +69,10 @@ struct FrameParams {
   int32_t env_nonzero;
   uint64_t sample_base;    // first slot of this job in the per-sample arenas
   uint64_t pixel_base;     // first pixel of this job in the framebuffer arena
+  // Column band: the job traces the samples of image columns [band_x0, band_x0 + band_cols)
+  // (sample ids sid_base + [0, band_cols * height * spp), column-major as Camera.cpp:28 numbers
+  // them) and its filter writes columns [out_x0, out_x1). A whole frame is the band 0..width.
+  uint32_t sid_base, band_x0, band_cols, out_x0, out_x1;
   CameraRec cam;
 };
 
@@ -315,17 +319,23 @@ static_assert(sizeof(RayRec) == 64, "ray record must be one 64-byte line");
 __host__ __device__ inline uint32_t sample_slot(uint32_t x, uint32_t y, uint32_t k, uint32_t H, uint32_t spp) {
   return (x * spp + k) * H + y;
 }
+// Slot of a sample in its job's arena range: the band's columns, from its first one.
 __device__ inline uint32_t slot_of_sid(const FrameParams& p, uint32_t sid) {
   const uint32_t x = sid / (p.height * p.spp);
   const uint32_t rem = sid - x * p.height * p.spp;
   const uint32_t y = rem / p.spp;
-  return sample_slot(x, y, rem - y * p.spp, p.height, p.spp);
+  return sample_slot(x - p.band_x0, y, rem - y * p.spp, p.height, p.spp);
 }
 
 // Host side (render.hip): frame constants of one job on one scene.
 CameraRec make_camera(const swr_camera_desc& d);
 uint64_t spp_of(const swr_render_desc& d);
 FrameParams make_params(const swr_render_desc& d, const DeviceScene* s);
+// Restricts a job to the column band whose filter writes columns [x0, x1): the band's samples
+// are those columns plus, for the tent filter, one neighbour column on each side inside the image.
+void set_band(FrameParams* p, uint32_t x0, uint32_t x1);
+// Samples of a job (its band's columns x height x spp).
+inline uint64_t band_samples(const FrameParams& p) { return (uint64_t)p.band_cols * p.height * p.spp; }
 int bits_for(uint64_t max_value);
 
 }  // namespace swr

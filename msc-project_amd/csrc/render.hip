@@ -99,16 +99,17 @@ __global__ __launch_bounds__(256) void raygen_kernel(const FrameParams* __restri
   const uint32_t k = w->k;
   if (blockIdx.x * blockDim.x >= k) return;
   bool pending = t < k;
-  uint32_t job = 0, sid = 0;
+  uint32_t job = 0, sid_local = 0;
   if (pending) {
     const uint64_t g = w->raygen_base + t;
     const JobRange rg = ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, g)];
     job = rg.job;
-    sid = (uint32_t)(g - rg.path_begin);
+    sid_local = (uint32_t)(g - rg.path_begin);  // index within the job; its sample id adds sid_base
   }
   const uint32_t slot = w->n - k + t;
   auto body = [&](uint32_t j0) {
     const FrameParams& p = jobs[j0];
+    const uint32_t sid = p.sid_base + sid_local;
     V3 o, d;
     float sx, sy;
     camera_ray(p, sid, &o, &d, &sx, &sy);
@@ -569,7 +570,9 @@ __global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, con
   __shared__ float4 s_rad[2][kHaloSlots];
   __shared__ float2 s_xy[2][kHaloSlots];
   const int W = (int)p.width, H = (int)p.height, spp = (int)p.spp;
-  const int x0 = (int)blockIdx.y * kFilterCols, y0 = (int)blockIdx.x * kFilterRows;
+  // Columns out_x0.. of the job's band; its samples are those of columns band_x0.. band_x1 - 1.
+  const int bx0 = (int)p.band_x0, bx1 = (int)(p.band_x0 + p.band_cols);
+  const int x0 = (int)p.out_x0 + (int)blockIdx.y * kFilterCols, y0 = (int)blockIdx.x * kFilterRows;
   const int lx = (int)threadIdx.x / kFilterRows, ly = (int)threadIdx.x % kFilterRows;
   const int px = x0 + lx, py = y0 + ly;
   const bool tent = p.filter != SWR_FILTER_BOX;
@@ -581,8 +584,9 @@ __global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, con
     const int e = (int)threadIdx.x + j * kFilterBlock;
     const int c = e / kHaloRows, r = e - c * kHaloRows;
     const int qx = x0 - 1 + c, qy = y0 - 1 + r;
-    const bool in = e < kHaloSlots && qx >= 0 && qx < W && qy >= 0 && qy < H && (tent || (c >= 1 && c <= kFilterCols && r >= 1 && r <= kFilterRows));
-    base[j] = in ? (int64_t)p.sample_base + (int64_t)qx * spp * H + qy : -1;
+    const bool in = e < kHaloSlots && qx >= bx0 && qx < bx1 && qy >= 0 && qy < H &&
+                    (tent || (c >= 1 && c <= kFilterCols && r >= 1 && r <= kFilterRows));
+    base[j] = in ? (int64_t)p.sample_base + (int64_t)(qx - bx0) * spp * H + qy : -1;
   }
   float4 pr[kHaloPerLane];
   float2 pq[kHaloPerLane];
@@ -644,7 +648,7 @@ __global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, con
     }
     if (k + 1 < spp) stage(buf ^ 1);
   }
-  if (px >= W || py >= H) return;
+  if (px >= (int)p.out_x1 || py >= H) return;
   float* o = pixels + 3 * (p.pixel_base + (size_t)py * W + px);
   float v0, v1, v2;
   if (tent) {
@@ -667,14 +671,14 @@ __global__ __launch_bounds__(kFilterBlock) void filter_kernel(FrameParams p, con
 }
 
 // Sample records (x, y, r, g, b) of one job in reference sample-id order (Image.h:10-14).
-__global__ void pack_samples_kernel(FrameParams p, size_t n, const float2* __restrict__ sxy,
+__global__ void pack_samples_kernel(FrameParams p, uint32_t sid0, size_t n, const float2* __restrict__ sxy,
                                     const float4* __restrict__ radiance, float* __restrict__ out) {
-  const size_t sid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (sid >= n) return;
-  const uint64_t s = p.sample_base + slot_of_sid(p, (uint32_t)sid);
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t s = p.sample_base + slot_of_sid(p, sid0 + (uint32_t)i);
   const float2 q = sxy[s];
   const float4 L = radiance[s];
-  float* o = out + 5 * sid;
+  float* o = out + 5 * i;
   o[0] = q.x;
   o[1] = q.y;
   o[2] = L.x;
@@ -775,7 +779,18 @@ FrameParams make_params(const swr_render_desc& d, const DeviceScene* s) {
     p.key_lo = make_float3(lo[0], lo[1], lo[2]);
     p.key_scale = make_float3(sc[0], sc[1], sc[2]);
   }
+  set_band(&p, 0, p.width);
   return p;
+}
+
+void set_band(FrameParams* p, uint32_t x0, uint32_t x1) {
+  if (x0 >= x1 || x1 > p->width) fail(SWR_E_INVALID, "render: column band out of range");
+  const uint32_t halo = p->filter == SWR_FILTER_BOX ? 0u : 1u;  // the tent reads one column aside
+  p->out_x0 = x0;
+  p->out_x1 = x1;
+  p->band_x0 = x0 >= halo ? x0 - halo : 0u;
+  p->band_cols = std::min(p->width, x1 + halo) - p->band_x0;
+  p->sid_base = p->band_x0 * p->height * p->spp;
 }
 
 int bits_for(uint64_t max_value) {
@@ -921,7 +936,7 @@ uint32_t WaveLoop::admit(uint32_t slot, FrameParams p) {
   if (num_ranges_ - range_lo_ >= range_cap_) fail(SWR_E_BUSY, "wavefront: job range table full");
   p.job = slot;
   p.seq = seq_++;
-  const uint64_t paths = (uint64_t)p.width * p.height * p.spp;
+  const uint64_t paths = band_samples(p);
   JobRange r{total_, slot, p.seq};
   total_ += paths;
   range_begin_.push_back(r.path_begin);
@@ -1195,7 +1210,7 @@ void WaveLoop::run_graphs(int check_every) {
 }
 
 void WaveLoop::filter(const FrameParams& p, bool accumulate) {
-  dim3 fgrid(blocks(p.height, kFilterRows), blocks(p.width, kFilterCols));
+  dim3 fgrid(blocks(p.height, kFilterRows), blocks(p.out_x1 - p.out_x0, kFilterCols));
   filter_kernel<<<fgrid, kFilterBlock, 0, stream_>>>(p, radiance_, sxy_, pixels_, accumulate ? 1 : 0);
   SWR_HIP(hipGetLastError());
 }
@@ -1211,16 +1226,20 @@ void WaveLoop::copy_out(const FrameParams& p, float* pixels_host, float* samples
   if (pixels_host)
     SWR_HIP(hipMemcpyAsync(pixels_host, pixels_ + 3 * p.pixel_base, 12ull * p.width * p.height,
                            hipMemcpyDeviceToHost, st));
-  if (samples_host) {
-    const uint64_t total = (uint64_t)p.width * p.height * p.spp;
-    ensure(ctx_, b_.staging, 20ull * total);
-    pack_samples_kernel<<<blocks(total), 256, 0, st>>>(p, total, sxy_, radiance_, static_cast<float*>(b_.staging.ptr));
-    SWR_HIP(hipGetLastError());
-    SWR_HIP(hipMemcpyAsync(samples_host, b_.staging.ptr, 20ull * total, hipMemcpyDeviceToHost, st));
-    SWR_HIP(hipStreamSynchronize(st));
-    release(ctx_, b_.staging);
-  }
+  if (samples_host) copy_samples(p, samples_host);
   SWR_HIP(hipStreamSynchronize(st));
+}
+
+void WaveLoop::copy_samples(const FrameParams& p, float* samples_host) {
+  hipStream_t st = stream_;
+  const uint32_t per_col = p.height * p.spp, sid0 = p.out_x0 * per_col;
+  const uint64_t n = (uint64_t)(p.out_x1 - p.out_x0) * per_col;
+  ensure(ctx_, b_.staging, 20ull * n);
+  pack_samples_kernel<<<blocks(n), 256, 0, st>>>(p, sid0, n, sxy_, radiance_, static_cast<float*>(b_.staging.ptr));
+  SWR_HIP(hipGetLastError());
+  SWR_HIP(hipMemcpyAsync(samples_host + 5ull * sid0, b_.staging.ptr, 20ull * n, hipMemcpyDeviceToHost, st));
+  SWR_HIP(hipStreamSynchronize(st));
+  release(ctx_, b_.staging);
 }
 
 LoopTotals WaveLoop::totals() {
@@ -1249,6 +1268,25 @@ uint32_t wave_capacity(const swr_render_desc& desc, uint64_t total) {
   return (uint32_t)std::min<uint64_t>(uint64_t(1) << exp, std::max<uint64_t>(total, 64));
 }
 
+// Image columns per band when a frame's per-sample state does not fit the context budget: the
+// arenas may take what the budget leaves after everything else the context holds and the
+// framebuffer; a band holds its columns plus the tent's neighbour columns. The whole frame
+// (width columns) when there is no budget or it fits.
+uint32_t band_columns(swr_context* ctx, const LoopBuffers& b, const FrameParams& p, bool samples_out) {
+  const uint64_t per_sample = 24 + (samples_out ? 20 : 0);  // radiance + film position (+ staging)
+  const uint64_t per_col = (uint64_t)p.height * p.spp;
+  if (!ctx->budget) return p.width;
+  const uint64_t held = b.radiance.bytes + b.sample_xy.bytes + b.pixels.bytes + b.staging.bytes;
+  const uint64_t others = ctx->in_use - std::min(ctx->in_use, held);
+  const uint64_t fb = 12ull * p.width * p.height + 256;
+  if (ctx->budget <= others + fb) fail(SWR_E_BUDGET, "render: the framebuffer does not fit the memory budget");
+  const uint64_t cols = (ctx->budget - others - fb) / (per_sample * per_col + 512);
+  const uint64_t halo = p.filter == SWR_FILTER_BOX ? 0 : 2;
+  if (cols >= p.width) return p.width;
+  if (cols <= halo) fail(SWR_E_BUDGET, "render: one column of samples does not fit the memory budget");
+  return (uint32_t)(cols - halo);
+}
+
 void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_desc& desc, float* pixels_host,
                   float* samples_host, swr_stats* stats) {
   SWR_HIP(hipSetDevice(ctx->device));
@@ -1256,10 +1294,23 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   LoopBuffers& b = *ctx->loop;
   const uint64_t total = WaveLoop::job_samples(desc);
   WaveLoop loop(ctx, b, {scene}, 1, wave_capacity(desc, total));
-  const FrameParams p = loop.params(0, desc, 0, 0);
-  ensure(ctx, b.radiance, 16ull * total);
-  ensure(ctx, b.sample_xy, 8ull * total);
-  ensure(ctx, b.pixels, 12ull * p.width * p.height);
+  const FrameParams full = loop.params(0, desc, 0, 0);
+  // Out of core: a frame whose per-sample state exceeds the budget is traced in column bands,
+  // one after another through the same arenas (the reference spills its ray bins to disk
+  // instead, DirectionalBins.cpp:37-115). Paths are pure functions of their sample ids and each
+  // pixel is filtered from the same samples in the same order, so the image is the same.
+  const uint32_t cols = band_columns(ctx, b, full, samples_host != nullptr);
+  uint64_t band_max = total;
+  if (cols < full.width) {
+    // an interior band is the largest: its columns and, for the tent, a neighbour column each side
+    const uint32_t halo = full.filter == SWR_FILTER_BOX ? 0u : 2u;
+    band_max = std::min<uint64_t>(total, (uint64_t)(cols + halo) * full.height * full.spp);
+    release(ctx, b.radiance);  // make room for the smaller arenas
+    release(ctx, b.sample_xy);
+  }
+  ensure(ctx, b.radiance, 16ull * band_max);
+  ensure(ctx, b.sample_xy, 8ull * band_max);
+  ensure(ctx, b.pixels, 12ull * full.width * full.height);
   loop.set_arenas(static_cast<float4*>(b.radiance.ptr), static_cast<float2*>(b.sample_xy.ptr),
                   static_cast<float*>(b.pixels.ptr));
   hipStream_t st = ctx->stream;
@@ -1272,14 +1323,28 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     }
   } guard{ev};
   StageTimes times;
+  float filter_ms = 0.f;
   SWR_HIP(hipEventRecord(ev[0], st));
-  loop.admit(0, p);
-  loop.run_ahead(&times);
-  SWR_HIP(hipEventRecord(ev[1], st));
-  loop.filter(p);
-  SWR_HIP(hipEventRecord(ev[2], st));
-  loop.copy_out(p, pixels_host, samples_host);
+  for (uint32_t x0 = 0; x0 < full.width; x0 += cols) {
+    FrameParams p = full;
+    set_band(&p, x0, std::min(full.width, x0 + cols));
+    loop.admit(0, p);
+    loop.run_ahead(&times);
+    loop.retire_ranges(loop.paths_admitted());
+    SWR_HIP(hipEventRecord(ev[1], st));
+    loop.filter(p);
+    SWR_HIP(hipEventRecord(ev[2], st));
+    if (samples_host) loop.copy_samples(p, samples_host);
+    if (cols < full.width) {
+      float f = 0.f;
+      SWR_HIP(hipEventSynchronize(ev[2]));
+      SWR_HIP(hipEventElapsedTime(&f, ev[1], ev[2]));
+      filter_ms += f;
+    }
+  }
+  loop.copy_out(full, pixels_host, nullptr);
   SWR_HIP(hipStreamSynchronize(st));
+  if (cols >= full.width) SWR_HIP(hipEventElapsedTime(&filter_ms, ev[1], ev[2]));
   const LoopTotals t = loop.totals();
   if (stats) {
     swr_stats acc{};
@@ -1294,10 +1359,9 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
     acc.ms_sort_hits = times.sort_hits;
     acc.ms_shade = times.shade;
     acc.ms_shadow = times.shadow;
-    float f = 0.f, tot = 0.f;
-    SWR_HIP(hipEventElapsedTime(&f, ev[1], ev[2]));
+    float tot = 0.f;
     SWR_HIP(hipEventElapsedTime(&tot, ev[0], ev[2]));
-    acc.ms_filter = f;
+    acc.ms_filter = filter_ms;
     acc.ms_total = tot;
     *stats = acc;
   }

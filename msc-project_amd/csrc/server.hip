@@ -53,6 +53,10 @@ struct Request {
   int scene = 0;
   swr_render_desc desc{};
   int iterations = 1;
+  // Column bands per iteration (1 unless the iteration's per-sample state exceeds the arena):
+  // band b filters columns [b * band_cols, (b + 1) * band_cols); each band is a job of its own.
+  int bands = 1;
+  uint32_t band_cols = 0;
   float* dst = nullptr;
   Clock::time_point t_submit, t_admit, t_done;
   int status = SWR_OK;
@@ -127,20 +131,46 @@ hipEvent_t take_event(swr_server* s) {
   return e;
 }
 
-// Starts iteration r->started of an admitted request if a slot and a sample range are free
-// (caller holds the lock for the allocation; the device calls run without it).
+// Column band width for a request: the whole frame when its per-sample state fits the arena,
+// else bands of at most half the arena (two in flight), or the whole arena, holding their columns
+// and the tent filter's neighbour columns; 0 when not even one column fits.
+uint32_t plan_band_cols(const swr_render_desc& d, uint64_t arena_samples) {
+  const uint64_t per_col = (uint64_t)d.height * spp_of(d);
+  if ((uint64_t)d.width * per_col <= arena_samples) return (uint32_t)d.width;
+  const uint64_t halo = d.filter == SWR_FILTER_BOX ? 0 : 2;
+  for (uint64_t room : {arena_samples / 2, arena_samples}) {
+    const uint64_t cols = room / per_col;
+    if (cols > halo) return (uint32_t)std::min<uint64_t>(cols - halo, (uint64_t)d.width);
+  }
+  return 0;
+}
+
+// Samples one band job of the request needs in the arena (an interior band is the largest).
+uint64_t band_job_samples(const Request& r) {
+  const uint64_t per_col = (uint64_t)r.desc.height * spp_of(r.desc);
+  if (r.bands == 1) return (uint64_t)r.desc.width * per_col;
+  const uint32_t halo = r.desc.filter == SWR_FILTER_BOX ? 0u : 2u;
+  return (uint64_t)std::min<uint32_t>((uint32_t)r.desc.width, r.band_cols + halo) * per_col;
+}
+
+// Starts the next band job (iteration r->started / bands, band r->started % bands) of an
+// admitted request if a slot and a sample range are free (caller holds the lock for the
+// allocation; the device calls run without it).
 bool start_iteration(swr_server* s, Request* r, std::unique_lock<std::mutex>& lk) {
   if (s->free_slots.empty()) return false;
   uint64_t off = 0;
   if (!s->sample_space.alloc(r->sample_n, &off)) return false;
   Iter it;
-  it.index = r->started;
+  it.index = r->started / r->bands;
+  const uint32_t band = (uint32_t)(r->started % r->bands);
   it.slot = s->free_slots.back();
   s->free_slots.pop_back();
   it.sample_off = off;
   lk.unlock();
   it.p = s->loop->params((uint32_t)r->scene, r->desc, it.sample_off, r->pixel_off);
   it.p.iteration = r->desc.iteration + (uint32_t)it.index;
+  if (r->bands > 1)
+    set_band(&it.p, band * r->band_cols, std::min<uint32_t>(it.p.width, (band + 1) * r->band_cols));
   it.seq = s->loop->admit(it.slot, it.p);
   lk.lock();
   r->started += 1;
@@ -162,7 +192,7 @@ void admit(swr_server* s) {
   while (s->loop->paths_admitted() - s->last_cursor < limit && !s->free_slots.empty()) {
     if (!s->pending.empty()) {
       Request* r = s->pending.front();
-      const uint64_t ns = WaveLoop::job_samples(r->desc), np = WaveLoop::job_pixels(r->desc);
+      const uint64_t ns = band_job_samples(*r), np = WaveLoop::job_pixels(r->desc);
       if (s->sample_space.fits(ns) && s->pixel_space.fits(np)) {
         s->pending.pop_front();
         s->pixel_space.alloc(np, &r->pixel_off);
@@ -183,7 +213,7 @@ void admit(swr_server* s) {
     const size_t n = s->active.size();
     for (size_t k = 0; k < n && !started; ++k) {
       Request* r = s->active[(s->rr + k) % n];
-      if (r->started < r->iterations && (int)r->iters.size() < window && start_iteration(s, r, lk)) {
+      if (r->started < r->iterations * r->bands && (int)r->iters.size() < window && start_iteration(s, r, lk)) {
         started = true;
         s->rr = (s->rr + k + 1) % n;
       }
@@ -216,7 +246,7 @@ void retire(swr_server* s, const WaveState& w, uint32_t seq_limit) {
       r->iters.pop_front();
       r->accumulated += 1;
     }
-    if (r->accumulated < r->iterations) {
+    if (r->accumulated < r->iterations * r->bands) {
       still.push_back(r);
       continue;
     }
@@ -430,11 +460,15 @@ int swr_server_submit(swr_server* s, int32_t scene, const swr_render_desc* desc,
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->error) swr::fail(s->error, "server: stopped after an error: " + s->error_msg);
     s->stats.submitted += 1;
-    if (swr::WaveLoop::job_samples(*desc) > s->sample_space.capacity() ||
-        swr::WaveLoop::job_pixels(*desc) > s->pixel_space.capacity()) {
+    // An iteration whose per-sample state exceeds the arena runs as column bands (out of core);
+    // the framebuffer must fit whole.
+    r->band_cols = swr::plan_band_cols(*desc, s->sample_space.capacity());
+    if (!r->band_cols || swr::WaveLoop::job_pixels(*desc) > s->pixel_space.capacity()) {
       s->stats.refused += 1;
       swr::fail(SWR_E_BUDGET, "server: request larger than the state budget");
     }
+    r->bands = (int)((desc->width + r->band_cols - 1) / r->band_cols);
+    if ((int64_t)r->bands * iterations > 0x7FFFFFFF) swr::fail(SWR_E_INVALID, "server: too many band jobs");
     r->id = s->next_id++;
     *request = r->id;
     s->pending.push_back(r.get());

@@ -126,6 +126,9 @@ class WaveLoop {
   // Copies the job's pixels and, if asked, its sample records (x, y, r, g, b in reference
   // sample-id order) to the host (synchronises).
   void copy_out(const FrameParams& p, float* pixels_host, float* samples_host);
+  // Copies the sample records of the columns job `p` filters (out_x0..out_x1) to their place in
+  // a whole frame's records (synchronises).
+  void copy_samples(const FrameParams& p, float* samples_host);
 
   LoopTotals totals();  // running totals from a fresh snapshot (synchronises)
   hipStream_t stream() const { return stream_; }
