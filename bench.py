@@ -56,16 +56,18 @@ def parse():
                     help="serving: admit while fewer paths than this many waves wait (0 = library default)")
     ap.add_argument("--serve-waves-per-check", type=int, default=0,
                     help="serving: graph waves between host checks (0 = library default)")
-    ap.add_argument("--serve-as-batch", type=int, default=0,
-                    help="serving workload: also trace the first N requests' iterations as one batch (capacity check)")
+    ap.add_argument("--serve-saturation", type=int, default=64,
+                    help="serving: the first N requests submitted at once (served throughput at saturation), "
+                         "the same iterations as one batch, and job by job through the plain loop (0 = skip)")
     ap.add_argument("--serve-state-gib", type=float, default=32.0,
                     help="device memory budget for request state (per-sample arenas + framebuffers)")
     ap.add_argument("--batch-tuning", default=None,
                     help="batched workload: fixed launch shapes (as --tuning, ring 0 = per scene by depth) "
                          "instead of the batch autotuner")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first run a plain copy of known size (gather_kernel<4>, identity permutation, 2^26 "
-                         "records: 1280 MiB read, 1024 MiB written) to scale rocprof FETCH_SIZE/WRITE_SIZE")
+                    help="accepted for older profiling commands: the copy of known size (gather_kernel<4>, "
+                         "identity permutation, 2^26 records: 1280 MiB read, 1024 MiB written) that scales "
+                         "rocprof FETCH_SIZE/WRITE_SIZE now runs in every terrain or box-room run")
     return ap.parse_args()
 
 
@@ -131,7 +133,7 @@ def main():
     gen_s = time.time() - t0
 
     ctx = swr.Context(0)
-    copy_cal = calibrate_copy(ctx) if args.calibrate_copy else None
+    copy_cal = calibrate_copy(ctx)
     t0 = time.time()
     scene = swr.Scene(ctx, desc)
     build_s = time.time() - t0
@@ -203,7 +205,8 @@ def main():
         },
         "roofline": {
             "kernel": "trace_queue_kernel (closest-hit BVH4 traversal)",
-            "bound": "hbm",
+            "bound": "latency",
+            "yardstick": "hbm (no MFMA work on this path; frac is algorithmic bytes over HBM peak)",
             "achieved": round(trace_gbs, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -211,7 +214,10 @@ def main():
             "traffic": None,
             "avg_launch_ms": round(agg["ms_trace"] / agg["waves"], 4),
             "rays_per_launch": agg["closest_rays"] // agg["waves"],
-            "note": "algorithmic 48 B/ray (ray in + hit out), node traffic excluded; traversal is bound by node-fetch latency (L2 hit 50% in the committed profile), not HBM bandwidth",
+            "note": "algorithmic 48 B/ray (ray in + hit out), node traffic excluded; traversal is bound by the "
+                    "latency of node fetches that miss L2, not by HBM bandwidth",
+            "mrays_s": round(agg["closest_rays"] / (agg["ms_trace"] * 1e-3) / 1e6, 1),
+            "l2_hit": None,
         },
         "sort_roofline": {
             "kernel": f"in-loop ray sort ({ray_passes} passes of 8 bits, "
@@ -241,20 +247,25 @@ def main():
         "image_mean": float(np.mean(pix)),
     }
 
-    if copy_cal:
-        out["copy_calibration"] = copy_cal
+    # measured device-copy ceiling: gather_kernel<4> over 2^26 16-byte records, identity permutation
+    out["copy_ceiling"] = copy_cal
     traffic_file = os.path.join(ROOT, "profiles", "trace_queue_kernel_terrain_traffic.md")
     if os.path.exists(traffic_file) and args.workload == "terrain":
         # HBM bytes per traversal ray from the committed rocprofv3 --pmc summary (FETCH_SIZE +
         # WRITE_SIZE per dispatch, scaled by the copy calibration), times this run's rays per launch.
-        per_ray = None
+        per_ray = l2_hit = None
         with open(traffic_file) as f:
             for line in f:
                 if line.startswith("hbm_bytes_per_ray:"):
                     per_ray = float(line.split(":", 1)[1])
+                if line.startswith("l2_hit_pct:"):
+                    l2_hit = float(line.split(":", 1)[1])
         if per_ray:
             out["roofline"]["traffic"] = round(per_ray * agg["closest_rays"] / agg["waves"], 1)
             out["roofline"]["traffic_per_ray"] = per_ray
+        if l2_hit is not None:
+            out["roofline"]["l2_hit"] = round(l2_hit / 100.0, 3)
+            out["roofline"]["l2_hit_source"] = "profiles/trace_queue_kernel_terrain_traffic.md (TCC_HIT/(TCC_HIT+TCC_MISS))"
 
     if not args.no_kernels:
         out["kernels"] = kernels_bench(ctx, scene, desc, args)
@@ -397,6 +408,31 @@ def batched_vs_plain(ctx, args, plain=True):
     rds = [swr.render_desc(r) for _, r in jobs]
     saved = ctx.tuning()
 
+    # plain: one render per job through the plain loop (run-ahead waves), each scene with the
+    # launch shapes the plain autotuner (swr_autotune) picks for it at that scene's largest job;
+    # every job is rendered twice and the faster device time counted (the batched side below is
+    # likewise the faster of two timed runs)
+    t0 = time.time()
+    paths_of = lambda rd: rd.width * rd.height * (rd.spp or rd.base * rd.base)
+    plain_tunings = {}
+    for s in sorted({s for s, _ in jobs}) if plain else ():
+        big = max((rd for (js, _), rd in zip(jobs, rds) if js == s), key=paths_of)
+        plain_tunings[s] = ctx.autotune(scs[s], big)
+    plain_tune_s = time.time() - t0
+    plain_ms, plain_rays, plain_pix = 0.0, 0, {}
+    wall = time.perf_counter()
+    for j, ((s, _), rd) in enumerate(zip(jobs, rds)):
+        if not plain:
+            break
+        ctx.set_tuning(**plain_tunings[s])
+        pix, _, st = scs[s].render(rd)
+        _, _, st2 = scs[s].render(rd, pixels=False)
+        plain_ms += min(st["ms_total"], st2["ms_total"])
+        plain_rays += st["closest_rays"] + st["shadow_rays"]
+        if j % 8 == 0:
+            plain_pix[j] = pix
+    plain_wall = (time.perf_counter() - wall) / 2
+
     b = swr.Batch(ctx)
     for (s, _), rd in zip(jobs, rds):
         b.add(scs[s], rd)
@@ -405,26 +441,10 @@ def batched_vs_plain(ctx, args, plain=True):
         ctx.set_tuning(*[int(v) for v in args.batch_tuning.split(",")])
         tuning = ctx.tuning()
     else:
-        # per-scene stack rings + sort tile and key widths at the plain operating point (the
-        # default wave size); the batched loop is tuned again at the wave size it runs
+        # per-scene stack rings + sort tile and key widths at the default wave size; the batched
+        # loop is tuned again at the wave size it runs
         tuning = b.autotune()
     tune_s = time.time() - t0
-
-    # plain: one render per job through the host-paced loop, with the tuning above; each job is
-    # rendered twice and the faster device time counted (the batched side below is likewise the
-    # faster of two timed runs)
-    plain_ms, plain_rays, plain_pix = 0.0, 0, {}
-    wall = time.perf_counter()
-    for j, ((s, _), rd) in enumerate(zip(jobs, rds)):
-        if not plain:
-            break
-        pix, _, st = scs[s].render(rd)
-        _, _, st2 = scs[s].render(rd, pixels=False)
-        plain_ms += min(st["ms_total"], st2["ms_total"])
-        plain_rays += st["closest_rays"] + st["shadow_rays"]
-        if j % 8 == 0:
-            plain_pix[j] = pix
-    plain_wall = (time.perf_counter() - wall) / 2
 
     wave_rays = args.batch_wave_rays
     if not wave_rays:  # the batched loop's own wave size: the fastest of 2^24 .. 2^28 rays
@@ -474,9 +494,12 @@ def batched_vs_plain(ctx, args, plain=True):
         "paced_stage_ms": {k[3:]: round(paced[k], 2) for k in paced if k.startswith("ms_")},
         "images_equal_sampled": bool(same and plain_rays == batch_rays) if plain else None,
         "scene_rings": rings,
-        "plain_tuning": tuning,
+        "plain_tuning": {f"scene{s}": t for s, t in plain_tunings.items()},
+        "plain_tuning_note": "swr_autotune per scene at the scene's largest job (sort pass structure and tile, "
+                             "ray and hit key widths, stack ring)",
         "batch_tuning": batch_tuning,
         "autotune_s": round(tune_s, 2),
+        "plain_autotune_s": round(plain_tune_s, 2),
         "scene_gen_s": round(gen_s, 2),
         "scene_upload_bvh_s": round(build_s, 2),
     }
@@ -500,19 +523,7 @@ def serving(ctx, args):
     gen_s = time.time() - t0
     scs = [swr.Scene(ctx, d) for d in descs]
     rds = [(s, swr.render_desc(r), it) for s, r, it in reqs]
-    as_batch = None
-    if args.serve_as_batch:
-        b = swr.Batch(ctx)
-        for s, rd, it in rds[:args.serve_as_batch]:
-            for i in range(it):
-                d = swr.RenderDesc.from_buffer_copy(rd)
-                d.iteration = i
-                b.add(scs[s], d)
-        b.run()
-        st = b.run()
-        as_batch = {"requests": args.serve_as_batch,
-                    "mrays_s": round((st["closest_rays"] + st["shadow_rays"]) / (st["ms_total"] * 1e-3) / 1e6, 2)}
-        b.close()
+    saturation = serve_saturation(ctx, scs, rds[:args.serve_saturation], args) if args.serve_saturation else None
     srv = swr.Server(ctx, scs, state_bytes=int(args.serve_state_gib * (1 << 30)), max_requests=256,
                      backlog_waves=args.serve_backlog_waves, waves_per_check=args.serve_waves_per_check)
     arrivals = []
@@ -556,7 +567,7 @@ def serving(ctx, args):
         "rate_per_s": args.serve_rate,
         "backlog_waves": args.serve_backlog_waves or 2,
         "waves_per_check": args.serve_waves_per_check or 2,
-        "as_batch": as_batch,
+        "saturation": saturation,
         "mrays_s": round(rays / wall / 1e6, 2),
         "p50_ms": round(pct(lat, 0.50), 2),
         "p99_ms": round(pct(lat, 0.99), 2),
@@ -572,6 +583,88 @@ def serving(ctx, args):
         "state_gib": round(tot["state_bytes"] / (1 << 30), 2),
         "queue_gib": round(tot["queue_bytes"] / (1 << 30), 2),
         "scene_gen_s": round(gen_s, 2),
+    }
+
+
+def serve_saturation(ctx, scs, reqs, args):
+    """Throughput of the served form on the same requests three ways: all submitted at once to a
+    serving loop (wall time from the first submission to the last result in host memory), their
+    iterations as one batch (device time), and every iteration rendered on its own through the
+    plain loop with that scene's plain-autotuned shapes (device time, the faster of two renders
+    per iteration). The served result of each request is compared bit for bit with the mean of
+    its plain iterations for the first request on every scene."""
+    import numpy as np
+
+    import swr
+
+    saved = ctx.tuning()
+    paths_of = lambda rd: rd.width * rd.height * (rd.spp or rd.base * rd.base)
+    tunings = {}
+    for s in sorted({s for s, _, _ in reqs}):
+        big = max((rd for js, rd, _ in reqs if js == s), key=paths_of)
+        tunings[s] = ctx.autotune(scs[s], big)
+    plain_ms, plain_rays, plain_mean = 0.0, 0, {}
+    first = {}
+    for k, (s, rd, it) in enumerate(reqs):
+        first.setdefault(s, k)
+    for k, (s, rd, it) in enumerate(reqs):
+        ctx.set_tuning(**tunings[s])
+        d = swr.RenderDesc.from_buffer_copy(rd)
+        acc = None
+        for i in range(it):
+            d.iteration = rd.iteration + i
+            keep = first[s] == k
+            pix, _, st = scs[s].render(d, pixels=keep)
+            _, _, st2 = scs[s].render(d, pixels=False)
+            plain_ms += min(st["ms_total"], st2["ms_total"])
+            plain_rays += st["closest_rays"] + st["shadow_rays"]
+            if keep:
+                acc = pix if acc is None else acc + pix
+        if acc is not None:
+            plain_mean[k] = acc * np.float32(1.0 / it)
+    ctx.set_tuning(**saved)
+
+    b = swr.Batch(ctx)
+    for s, rd, it in reqs:
+        for i in range(it):
+            d = swr.RenderDesc.from_buffer_copy(rd)
+            d.iteration = rd.iteration + i
+            b.add(scs[s], d)
+    b.run()
+    bst = b.run()
+    b.close()
+    batch_rays = bst["closest_rays"] + bst["shadow_rays"]
+
+    best = None
+    for _ in range(2):
+        srv = swr.Server(ctx, scs, state_bytes=int(args.serve_state_gib * (1 << 30)), max_requests=256,
+                         backlog_waves=args.serve_backlog_waves, waves_per_check=args.serve_waves_per_check)
+        t = time.perf_counter()
+        ids = [srv.submit(s, rd, it, keep=k in plain_mean) for k, (s, rd, it) in enumerate(reqs)]
+        res = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - t
+        tot = srv.totals()
+        srv.close()
+        if best is None or wall < best[0]:
+            best = (wall, tot, res)
+    wall, tot, res = best
+    same = all(np.array_equal(res[k][0], m) for k, m in plain_mean.items())
+    served_rays = tot["closest_rays"] + tot["shadow_rays"]
+    served_v = served_rays / wall / 1e6
+    plain_v = plain_rays / (plain_ms * 1e-3) / 1e6
+    return {
+        "requests": len(reqs),
+        "iterations": int(sum(it for _, _, it in reqs)),
+        "rays": int(served_rays),
+        "served_mrays_s": round(served_v, 2),
+        "served_wall_s": round(wall, 3),
+        "as_batch_mrays_s": round(batch_rays / (bst["ms_total"] * 1e-3) / 1e6, 2),
+        "plain_mrays_s": round(plain_v, 2),
+        "served_vs_plain": round(served_v / plain_v, 3),
+        "rays_equal": bool(served_rays == plain_rays == batch_rays),
+        "results_equal_plain_mean": bool(same),
+        "note": "served: wall time, all requests submitted at once (the faster of two runs); as_batch and "
+                "plain: device time; plain with swr_autotune's shapes per scene",
     }
 
 
