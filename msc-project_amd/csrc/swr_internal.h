@@ -137,7 +137,10 @@ void sort_reset_epochs(swr_context* ctx, hipStream_t stream);
 // Scratch sizes the sort needs for n elements.
 size_t sort_status_bytes(size_t n);
 void sort_reserve(swr_context* ctx, size_t n);
-constexpr int kSortBlock = 512;
+#ifndef SWR_SORT_BLOCK
+#define SWR_SORT_BLOCK 512
+#endif
+constexpr int kSortBlock = SWR_SORT_BLOCK;  // lanes per workgroup of the sort passes
 constexpr int kSortMinKeysPerThread = 8;
 // The sort without allocation (swr_sort_pairs_u32_noalloc): at most this many pairs, with all
 // of its scratch in a file-scope device array of sort.hip (sort_scratch_bytes(), about 8.3 MiB).
