@@ -146,9 +146,14 @@ __global__ __launch_bounds__(kSortBlock) void upsweep_kernel(const uint32_t* __r
   constexpr int kTile = kSortBlock * kKpt;
   __shared__ uint32_t s_h[kUpTiles][kRadix];
   const uint32_t n = n_dev ? *n_dev : n_host;
-  const uint32_t tile0 = blockIdx.x * (uint32_t)kUpTiles;
-  if (tile0 * (uint32_t)kTile >= n) return;
   const uint32_t tiles = (n + kTile - 1) / kTile;
+  // The same contiguous run of tiles per XCD as the scanned pass kernel (see there), so that
+  // keys read here may still be in that XCD's L2 when the pass kernel reads them again.
+  const uint32_t groups = (tiles + kUpTiles - 1) / kUpTiles, gper = groups / 8u, grem = groups % 8u;
+  const uint32_t gx = blockIdx.x % 8u;
+  const uint32_t group = blockIdx.x < groups ? gx * gper + min(gx, grem) + blockIdx.x / 8u : blockIdx.x;
+  const uint32_t tile0 = group * (uint32_t)kUpTiles;
+  if (tile0 * (uint32_t)kTile >= n) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int i = t; i < kUpTiles * kRadix; i += kSortBlock) (&s_h[0][0])[i] = 0;
   __syncthreads();
@@ -223,7 +228,15 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (t == 0) {
     if (kScanned) {
-      s_tile = blockIdx.x;
+      // Workgroups go round robin to the 8 XCDs (one L2 each). Tile t's run of a digit ends in
+      // the cache line where tile t + 1's run begins; giving each XCD one contiguous run of tiles
+      // (among the tiles that hold keys: launches sized for a capacity end with empty ones) lets
+      // most such shared lines be completed in one L2 instead of leaving two partial lines, and
+      // each XCD walks its run from the end, which its upsweep workgroups read last.
+      const uint32_t nk = n_dev ? *n_dev : n_host;
+      const uint32_t g = (uint32_t)(((uint64_t)nk + kTile - 1) / kTile), per = g / 8u, rem = g % 8u;
+      const uint32_t x = blockIdx.x % 8u, cnt = per + (x < rem ? 1u : 0u);
+      s_tile = blockIdx.x < g ? x * per + min(x, rem) + (cnt - 1u - blockIdx.x / 8u) : blockIdx.x;
       s_epoch = 0;
     } else {
       s_tile = atomicAdd(counters + pass, 1u);
