@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
     ap.add_argument("--batch-jobs", type=int, default=256, help="jobs of the batched workload")
     ap.add_argument("--no-plain", action="store_true", help="batched workload: skip the job-by-job plain pass")
+    ap.add_argument("--batch-paced-only", action="store_true",
+                    help="batched workload: only the host-paced loop (no graph replays; for kernel profiles), "
+                         "at --batch-wave-rays (default 2^27) with --batch-tuning or the default shapes")
     ap.add_argument("--batch-wave-rays", type=int, default=0,
                     help="rays per wave of the batched loop (0 = the batch autotuner's choice)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
@@ -436,6 +439,23 @@ def batched_vs_plain(ctx, args, plain=True):
     b = swr.Batch(ctx)
     for (s, _), rd in zip(jobs, rds):
         b.add(scs[s], rd)
+    if args.batch_paced_only:
+        if args.batch_tuning:
+            ctx.set_tuning(*[int(v) for v in args.batch_tuning.split(",")])
+        wave_rays = args.batch_wave_rays or (1 << 27)
+        paced = b.run(wave_rays, paced=True)
+        used = ctx.tuning()
+        b.close()
+        ctx.set_tuning(**saved)
+        for sc in scs:
+            sc.close()
+        return {"workload": f"{len(jobs)} jobs over 8 scenes (10k-2M tris), 128^2-1024^2 px, 1-64 spp, depth 6",
+                "rays": int(paced["closest_rays"] + paced["shadow_rays"]), "batched_wave_rays": wave_rays,
+                "batched_mrays_s": round((paced["closest_rays"] + paced["shadow_rays"]) / (paced["ms_total"] * 1e-3)
+                                         / 1e6, 2),
+                "batched_ms": round(paced["ms_total"], 2), "paced_only": True,
+                "paced_stage_ms": {k[3:]: round(paced[k], 2) for k in paced if k.startswith("ms_")},
+                "tuning": used}
     t0 = time.time()
     if args.batch_tuning:
         ctx.set_tuning(*[int(v) for v in args.batch_tuning.split(",")])
