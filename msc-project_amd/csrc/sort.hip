@@ -147,11 +147,13 @@ __global__ __launch_bounds__(kSortBlock) void upsweep_kernel(const uint32_t* __r
   __shared__ uint32_t s_h[kUpTiles][kRadix];
   const uint32_t n = n_dev ? *n_dev : n_host;
   const uint32_t tiles = (n + kTile - 1) / kTile;
-  // The same contiguous run of tiles per XCD as the scanned pass kernel (see there), so that
-  // keys read here may still be in that XCD's L2 when the pass kernel reads them again.
+  // With the count known at launch, the same contiguous run of tiles per XCD as the scanned pass
+  // kernel (see there), so that keys read here may still be in that XCD's L2 when the pass kernel
+  // reads them again.
   const uint32_t groups = (tiles + kUpTiles - 1) / kUpTiles, gper = groups / 8u, grem = groups % 8u;
   const uint32_t gx = blockIdx.x % 8u;
-  const uint32_t group = blockIdx.x < groups ? gx * gper + min(gx, grem) + blockIdx.x / 8u : blockIdx.x;
+  const uint32_t group =
+      !n_dev && blockIdx.x < groups ? gx * gper + min(gx, grem) + blockIdx.x / 8u : blockIdx.x;
   const uint32_t tile0 = group * (uint32_t)kUpTiles;
   if (tile0 * (uint32_t)kTile >= n) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -230,13 +232,13 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
     if (kScanned) {
       // Workgroups go round robin to the 8 XCDs (one L2 each). Tile t's run of a digit ends in
       // the cache line where tile t + 1's run begins; giving each XCD one contiguous run of tiles
-      // (among the tiles that hold keys: launches sized for a capacity end with empty ones) lets
-      // most such shared lines be completed in one L2 instead of leaving two partial lines, and
-      // each XCD walks its run from the end, which its upsweep workgroups read last.
-      const uint32_t nk = n_dev ? *n_dev : n_host;
-      const uint32_t g = (uint32_t)(((uint64_t)nk + kTile - 1) / kTile), per = g / 8u, rem = g % 8u;
+      // lets most such shared lines be completed in one L2 instead of leaving two partial lines,
+      // and each XCD walks its run from the end, which its upsweep workgroups read last. Only when
+      // the count is known at launch: with the count in device memory (sorts sized for a wave's
+      // capacity) the extra dependent load before the tile is known cost more than the runs gain.
+      const uint32_t g = (uint32_t)(((uint64_t)n_host + kTile - 1) / kTile), per = g / 8u, rem = g % 8u;
       const uint32_t x = blockIdx.x % 8u, cnt = per + (x < rem ? 1u : 0u);
-      s_tile = blockIdx.x < g ? x * per + min(x, rem) + (cnt - 1u - blockIdx.x / 8u) : blockIdx.x;
+      s_tile = !n_dev && blockIdx.x < g ? x * per + min(x, rem) + (cnt - 1u - blockIdx.x / 8u) : blockIdx.x;
       s_epoch = 0;
     } else {
       s_tile = atomicAdd(counters + pass, 1u);
