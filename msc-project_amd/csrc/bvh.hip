@@ -108,11 +108,22 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
 }
 
 // Leaf boxes, then unions up the tree: the second child to arrive at a node computes its box.
-__global__ void refit_kernel(const float4* __restrict__ pos, const uint4* __restrict__ tri,
-                             const uint32_t* __restrict__ perm, int n, const uint2* __restrict__ children,
-                             const uint32_t* __restrict__ parent_internal, const uint32_t* __restrict__ parent_leaf,
-                             uint32_t* __restrict__ visits, float4* __restrict__ leaf_lo, float4* __restrict__ leaf_hi,
-                             float4* __restrict__ node_lo, float4* __restrict__ node_hi) {
+// The thread that writes a node's box is the one that then arrives at its parent, so its arrival
+// (an acquire-release atomic on the parent's counter) publishes that box to the parent's second
+// arrival. A node whose leaf range lies inside this workgroup's 1024 leaves has both arrivals in
+// the workgroup, so its counter is a workgroup-scope atomic; only nodes whose range crosses a
+// workgroup boundary (the top of the tree) need device scope, whose release and acquire write
+// back and invalidate the XCD's L2 (12.4 ms for 2.1 M triangles with device scope throughout,
+// 4.4 ms with 256-leaf workgroups, 1.9 ms with 1024).
+__global__ __launch_bounds__(1024) void refit_kernel(const float4* __restrict__ pos, const uint4* __restrict__ tri,
+                                                     const uint32_t* __restrict__ perm, int n,
+                                                     const uint2* __restrict__ children,
+                                                     const uint2* __restrict__ range,
+                                                     const uint32_t* __restrict__ parent_internal,
+                                                     const uint32_t* __restrict__ parent_leaf,
+                                                     uint32_t* __restrict__ visits, float4* __restrict__ leaf_lo,
+                                                     float4* __restrict__ leaf_hi, float4* __restrict__ node_lo,
+                                                     float4* __restrict__ node_hi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float3 a, b, c;
@@ -120,11 +131,16 @@ __global__ void refit_kernel(const float4* __restrict__ pos, const uint4* __rest
   leaf_lo[i] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)), 0.f);
   leaf_hi[i] = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)), 0.f);
   if (n == 1) return;
+  const uint32_t wg_first = blockIdx.x * blockDim.x, wg_last = wg_first + blockDim.x - 1u;
   uint32_t node = parent_leaf[i];
   for (;;) {
-    __threadfence();
-    if (atomicAdd(&visits[node], 1u) == 0u) return;  // the sibling finishes this node
-    __threadfence();
+    const uint2 rg = range[node];
+    const bool local = rg.x >= wg_first && rg.y <= wg_last;
+    const uint32_t prior = local ? __hip_atomic_fetch_add(&visits[node], 1u, __ATOMIC_ACQ_REL,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP)
+                                 : __hip_atomic_fetch_add(&visits[node], 1u, __ATOMIC_ACQ_REL,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+    if (prior == 0u) return;  // the sibling finishes this node
     const uint2 ch = children[node];
     float4 l0, h0, l1, h1;
     if (ch.x & kBvh2Leaf) {
@@ -368,8 +384,9 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
                                                   static_cast<uint2*>(range.ptr), static_cast<uint32_t*>(par_i.ptr),
                                                   static_cast<uint32_t*>(par_l.ptr));
     fill_u32_kernel<<<blocks(ni), 256, 0, s>>>(static_cast<uint32_t*>(visits.ptr), (uint32_t)ni, 0u);
-    refit_kernel<<<blocks(n), 256, 0, s>>>(pos, tri, sv, (int)n, static_cast<uint2*>(children.ptr),
-                                           static_cast<uint32_t*>(par_i.ptr), static_cast<uint32_t*>(par_l.ptr),
+    refit_kernel<<<blocks(n, 1024), 1024, 0, s>>>(pos, tri, sv, (int)n, static_cast<uint2*>(children.ptr),
+                                           static_cast<uint2*>(range.ptr), static_cast<uint32_t*>(par_i.ptr),
+                                           static_cast<uint32_t*>(par_l.ptr),
                                            static_cast<uint32_t*>(visits.ptr), static_cast<float4*>(leaf_lo.ptr),
                                            static_cast<float4*>(leaf_hi.ptr), static_cast<float4*>(node_lo.ptr),
                                            static_cast<float4*>(node_hi.ptr));
