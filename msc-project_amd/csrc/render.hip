@@ -154,7 +154,11 @@ __global__ void raygen_rays_kernel(FrameParams p, uint32_t sid0, uint32_t k, flo
 
 
 // Closest hit of every queued ray, visited in ray-key order (perm). The hit (t, leaf) goes into
-// the ray's record, the hit key (the scene's hit_base + global triangle id + 1; a miss hit_base) into hit_keys. With
+// the ray's record, the hit key (the scene's hit_base + the triangle's position in the BVH's
+// leaf array + 1; a miss hit_base) into hit_keys. The leaf array is in Morton order of the
+// triangle centroids, so a key truncated to its top bits still groups hits that lie close
+// together (shading reads their records together, and their shadow rays, traced in hit order,
+// start close together). With
 // kRingFilter, only rays whose scene uses this launch's ring size are traced.
 // The scene of ray j is the top scene_bits bits of its sorted key (sorted_keys[j] >> scene_shift):
 // the ray record's scene word is overwritten with the hit t by whichever launch traces it.
@@ -237,7 +241,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
     const HitResult h = traverse<kTraceBlock, kShort, false, kUniform, kUniform, kQ>(
         nodes, scenes[s0].leaf_tris, r, SWR_MASK_ALL, s_ring + threadIdx.x, spill + (size_t)j * spill_stride);
     reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
-    hit_keys[i] = (scenes[s0].hit_base + h.id + 1u) >> hit_shift;
+    hit_keys[i] = (scenes[s0].hit_base + h.leaf + 1u) >> hit_shift;
   };
   if (kUniform) {
     if (pending) body(tile_scene);
