@@ -8,8 +8,9 @@
 //   2. stable radix sort of the codes (sort.hip), ties broken by triangle index;
 //   3. one thread per internal node finds its key range and split -> binary radix tree;
 //   4. leaves walk up with per-node arrival counters, the second arrival unions the boxes;
-//   5. level by level, each BVH4 node adopts up to four BVH2 descendants (largest box area
-//      expanded first); subtrees of <= leaf_max triangles become leaves, which are contiguous
+//   5. level by level, each BVH4 node adopts up to four BVH2 descendants (the one with the
+//      largest box area x triangle count, its SAH cost, expanded first); subtrees of <= leaf_max
+//      triangles become leaves, which are contiguous
 //      runs of the sorted triangle order;
 //   6. the leaf triangles are written in sorted order as (v0|id, e1|mask, e2).
 #include <algorithm>
@@ -195,13 +196,15 @@ __global__ void collapse_kernel(Bvh2View v, const uint2* __restrict__ frontier, 
   const uint2 ch = v.children[item.y];
   ref[0] = ch.x;
   ref[1] = ch.y;
-  // Expand the largest-area internal child that is too big to be a leaf, up to four children.
+  // Expand the internal child with the largest SAH cost that is too big to be a leaf, up to four
+  // children.
   while (cnt < 4) {
     int best = -1;
     float best_area = -1.f;
     for (int c = 0; c < cnt; ++c) {
       if ((ref[c] & kBvh2Leaf) || ref_size(v, ref[c]) <= leaf_max) continue;
-      const float a = half_area(v.node_lo[ref[c]], v.node_hi[ref[c]]);
+      // surface-area-heuristic cost of the child: box area times the triangles under it
+      const float a = half_area(v.node_lo[ref[c]], v.node_hi[ref[c]]) * (float)ref_size(v, ref[c]);
       if (a > best_area) {
         best_area = a;
         best = c;
