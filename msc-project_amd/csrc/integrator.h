@@ -8,6 +8,7 @@
 
 #include <cstdint>
 
+#include "bands.h"
 #include "device_util.h"
 #include "scene.h"
 

@@ -797,12 +797,12 @@ FrameParams make_params(const swr_render_desc& d, const DeviceScene* s) {
 
 void set_band(FrameParams* p, uint32_t x0, uint32_t x1) {
   if (x0 >= x1 || x1 > p->width) fail(SWR_E_INVALID, "render: column band out of range");
-  const uint32_t halo = p->filter == SWR_FILTER_BOX ? 0u : 1u;  // the tent reads one column aside
+  const BandLayout b = band_layout(p->width, p->height, p->spp, p->filter == SWR_FILTER_BOX, x0, x1);
   p->out_x0 = x0;
   p->out_x1 = x1;
-  p->band_x0 = x0 >= halo ? x0 - halo : 0u;
-  p->band_cols = std::min(p->width, x1 + halo) - p->band_x0;
-  p->sid_base = p->band_x0 * p->height * p->spp;
+  p->band_x0 = b.band_x0;
+  p->band_cols = b.band_cols;
+  p->sid_base = b.sid_base;
 }
 
 int bits_for(uint64_t max_value) {
@@ -1292,11 +1292,11 @@ uint32_t band_columns(swr_context* ctx, const LoopBuffers& b, const FrameParams&
   const uint64_t others = ctx->in_use - std::min(ctx->in_use, held);
   const uint64_t fb = 12ull * p.width * p.height + 256;
   if (ctx->budget <= others + fb) fail(SWR_E_BUDGET, "render: the framebuffer does not fit the memory budget");
-  const uint64_t cols = (ctx->budget - others - fb) / (per_sample * per_col + 512);
-  const uint64_t halo = p.filter == SWR_FILTER_BOX ? 0 : 2;
-  if (cols >= p.width) return p.width;
-  if (cols <= halo) fail(SWR_E_BUDGET, "render: one column of samples does not fit the memory budget");
-  return (uint32_t)(cols - halo);
+  // room in samples (each allocation is rounded up to 256 bytes: 512 spare bytes per column)
+  const uint64_t room_cols = (ctx->budget - others - fb) / (per_sample * per_col + 512);
+  const uint64_t cols = band_width_for(p.width, per_col, p.filter == SWR_FILTER_BOX, room_cols * per_col);
+  if (!cols) fail(SWR_E_BUDGET, "render: one column of samples does not fit the memory budget");
+  return (uint32_t)cols;
 }
 
 void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_desc& desc, float* pixels_host,
@@ -1314,9 +1314,7 @@ void render_frame(swr_context* ctx, const swr_scene* scene, const swr_render_des
   const uint32_t cols = band_columns(ctx, b, full, samples_host != nullptr);
   uint64_t band_max = total;
   if (cols < full.width) {
-    // an interior band is the largest: its columns and, for the tent, a neighbour column each side
-    const uint32_t halo = full.filter == SWR_FILTER_BOX ? 0u : 2u;
-    band_max = std::min<uint64_t>(total, (uint64_t)(cols + halo) * full.height * full.spp);
+    band_max = band_max_samples(full.width, (uint64_t)full.height * full.spp, full.filter == SWR_FILTER_BOX, cols);
     release(ctx, b.radiance);  // make room for the smaller arenas
     release(ctx, b.sample_xy);
   }

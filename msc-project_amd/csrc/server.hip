@@ -136,21 +136,17 @@ hipEvent_t take_event(swr_server* s) {
 // and the tent filter's neighbour columns; 0 when not even one column fits.
 uint32_t plan_band_cols(const swr_render_desc& d, uint64_t arena_samples) {
   const uint64_t per_col = (uint64_t)d.height * spp_of(d);
+  const bool box = d.filter == SWR_FILTER_BOX;
   if ((uint64_t)d.width * per_col <= arena_samples) return (uint32_t)d.width;
-  const uint64_t halo = d.filter == SWR_FILTER_BOX ? 0 : 2;
-  for (uint64_t room : {arena_samples / 2, arena_samples}) {
-    const uint64_t cols = room / per_col;
-    if (cols > halo) return (uint32_t)std::min<uint64_t>(cols - halo, (uint64_t)d.width);
-  }
-  return 0;
+  const uint64_t half = band_width_for((uint64_t)d.width, per_col, box, arena_samples / 2);
+  return (uint32_t)(half ? half : band_width_for((uint64_t)d.width, per_col, box, arena_samples));
 }
 
 // Samples one band job of the request needs in the arena (an interior band is the largest).
 uint64_t band_job_samples(const Request& r) {
   const uint64_t per_col = (uint64_t)r.desc.height * spp_of(r.desc);
   if (r.bands == 1) return (uint64_t)r.desc.width * per_col;
-  const uint32_t halo = r.desc.filter == SWR_FILTER_BOX ? 0u : 2u;
-  return (uint64_t)std::min<uint32_t>((uint32_t)r.desc.width, r.band_cols + halo) * per_col;
+  return band_max_samples((uint64_t)r.desc.width, per_col, r.desc.filter == SWR_FILTER_BOX, r.band_cols);
 }
 
 // Starts the next band job (iteration r->started / bands, band r->started % bands) of an
