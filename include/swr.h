@@ -148,6 +148,9 @@ typedef struct {
 
 /* ---- context (Pathtracer ctor/dtor, Pathtracer.cpp:369-392) ------------------------------ */
 int swr_create(int device, swr_context** out);
+/* Frees everything the context owns. Scenes and batches still alive lose their device memory and
+ * their context: their handles stay valid for swr_scene_destroy / swr_batch_destroy, every other
+ * call on them fails with SWR_E_INVALID. Destroy servers before their context. */
 void swr_destroy(swr_context* ctx);
 const char* swr_last_error(const swr_context* ctx);
 /* Device-memory budget for this context in bytes (0 = unlimited). Allocations beyond it fail
@@ -292,7 +295,11 @@ void swr_batch_destroy(swr_batch* batch);
 /* Adds a job (at most 65536; scenes: at most 256 distinct). *job receives its index. */
 int swr_batch_add(swr_batch* batch, const swr_scene* scene, const swr_render_desc* desc, int32_t* job);
 /* Traces every job to completion with waves of wave_rays rays (0 = 2^23, at most 2^28). stats: the whole
- * batch (ms_total = first raygen to the last resolved framebuffer). flags: 0, or
+ * batch (ms_total = first raygen to the last resolved framebuffer). Out of core: when the
+ * context budget (swr_set_budget) cannot hold every job's state (24 B per sample, 12 B per
+ * pixel), the jobs run in groups of consecutive jobs that fit, one loop per group, each group's
+ * images copied to host memory as it finishes (same images; sample records are then not kept);
+ * SWR_E_BUDGET if one job cannot fit alone. flags: 0, or
  * SWR_BATCH_PACED to launch each wave from the host with sizes read back after the previous one
  * (no graphs; fills the per-stage times; for profiling and comparison). */
 #define SWR_BATCH_PACED 1u

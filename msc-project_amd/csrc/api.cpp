@@ -111,6 +111,8 @@ void swr_destroy(swr_context* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (swr_scene* s : ctx->scenes) swr::detach_scene(s);
   ctx->scenes.clear();
+  for (swr_batch* b : ctx->batches) swr::detach_batch(b);
+  ctx->batches.clear();
   swr::destroy_loop_buffers(ctx);
   for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
   ctx->allocs.clear();

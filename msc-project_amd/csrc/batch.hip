@@ -6,7 +6,9 @@
 // space; every wave tops up with camera rays from wherever the cursor is, so waves stay full
 // across job boundaries and small jobs share waves instead of each paying a loop of its own.
 // Job state lives in arenas laid out back to back (samples: 16 B radiance + 8 B film position;
-// pixels: 12 B), all resident at once.
+// pixels: 12 B), all resident at once -- or, when a context budget cannot hold every job's state,
+// in groups of consecutive jobs that fit, one persistent loop per group, each group's images
+// kept in host memory (the out-of-core form of a batch).
 #include <algorithm>
 #include <memory>
 #include <vector>
@@ -25,6 +27,8 @@ struct swr_batch {
   swr::LoopBuffers bufs;
   std::unique_ptr<swr::WaveLoop> loop;   // the last run's loop (its arenas hold the results)
   bool done = false;
+  bool grouped = false;                  // the last run went in groups: images in `images`
+  std::vector<std::vector<float>> images;
 };
 
 namespace swr {
@@ -41,7 +45,7 @@ uint32_t wave_capacity(uint32_t wave_rays, uint64_t total) {
 // runs the persistent loop; returns the device milliseconds from the first raygen to the last
 // resolved framebuffer.
 double run_jobs(swr_batch* b, const std::vector<swr_batch::Job>& jobs, uint32_t wave_rays, bool keep,
-                swr_stats* stats, bool paced = false) {
+                swr_stats* stats, bool paced = false, const std::vector<size_t>* ids = nullptr) {
   swr_context* ctx = b->ctx;
   SWR_HIP(hipSetDevice(ctx->device));
   uint64_t samples = 0, pixels = 0;
@@ -105,15 +109,108 @@ double run_jobs(swr_batch* b, const std::vector<swr_batch::Job>& jobs, uint32_t 
     *stats = s;
   }
   if (keep) {
-    for (size_t i = 0; i < jobs.size(); ++i) b->jobs[i].p = params[i];
+    for (size_t i = 0; i < jobs.size(); ++i) b->jobs[ids ? (*ids)[i] : i].p = params[i];
     b->loop = std::move(loop);
   }
   return ms;
 }
 
+// Per-job state in bytes: 24 per sample, 12 per pixel.
+uint64_t job_state_bytes(const swr_render_desc& d) {
+  return 24ull * WaveLoop::job_samples(d) + 12ull * WaveLoop::job_pixels(d);
+}
+
+// Groups of consecutive jobs whose state fits `room` bytes (one group of every job when room is
+// 0: no budget). A job that cannot fit alone is refused.
+std::vector<std::vector<size_t>> plan_groups(const std::vector<swr_batch::Job>& jobs, uint64_t room) {
+  std::vector<std::vector<size_t>> groups(1);
+  uint64_t used = 0;
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    // each arena allocation rounds up to 256 bytes
+    const uint64_t need = job_state_bytes(jobs[i].desc);
+    if (room && need + 768 > room)
+      fail(SWR_E_BUDGET, "batch: job " + std::to_string(i) + " needs " + std::to_string(need) +
+                             " B of state, more than the budget leaves (" + std::to_string(room) + " B)");
+    if (room && used + need + 768 > room && !groups.back().empty()) {
+      groups.emplace_back();
+      used = 0;
+    }
+    groups.back().push_back(i);
+    used += need;
+  }
+  return groups;
+}
+
+// Runs the batch: all jobs in one loop when their state fits the context budget (or there is
+// none), else in groups that fit, each group's images copied to host memory as it finishes.
+void run_batch(swr_batch* b, uint32_t wave_rays, swr_stats* stats, bool paced) {
+  swr_context* ctx = b->ctx;
+  b->images.clear();
+  b->grouped = false;
+  uint64_t total = 0, samples = 0;
+  for (const auto& j : b->jobs) total += job_state_bytes(j.desc), samples += WaveLoop::job_samples(j.desc);
+  // What the budget leaves for the arenas once the loop's own buffers (queues, tables, sized for
+  // the whole batch so that no group grows them) are allocated.
+  auto arena_room = [&]() -> uint64_t {
+    const LoopBuffers& f = b->bufs;
+    const uint64_t held = f.radiance.bytes + f.sample_xy.bytes + f.pixels.bytes + f.staging.bytes;
+    const uint64_t others = ctx->in_use - std::min(ctx->in_use, held);
+    return ctx->budget > others ? ctx->budget - others : 0;
+  };
+  if (!ctx->budget || total + 768 <= arena_room()) {
+    run_jobs(b, b->jobs, wave_rays, true, stats, paced);
+    return;
+  }
+  b->loop.reset();
+  release(ctx, b->bufs.radiance);
+  release(ctx, b->bufs.sample_xy);
+  release(ctx, b->bufs.pixels);
+  { WaveLoop sizing(ctx, b->bufs, b->scenes, (uint32_t)b->jobs.size(), wave_capacity(wave_rays, samples)); }
+  const uint64_t room = arena_room();
+  if (total + 768 <= room) {
+    run_jobs(b, b->jobs, wave_rays, true, stats, paced);
+    return;
+  }
+  // Out of core: groups of consecutive jobs whose arenas fit what is left.
+  const std::vector<std::vector<size_t>> groups = plan_groups(b->jobs, room);
+  b->images.assign(b->jobs.size(), {});
+  swr_stats acc{};
+  for (const auto& g : groups) {
+    std::vector<swr_batch::Job> js;
+    for (size_t i : g) js.push_back(b->jobs[i]);
+    swr_stats st{};
+    // fresh arenas per group: growing one while the others still hold the last group's sizes
+    // could pass the budget
+    b->loop.reset();
+    release(ctx, b->bufs.radiance);
+    release(ctx, b->bufs.sample_xy);
+    release(ctx, b->bufs.pixels);
+    run_jobs(b, js, wave_rays, true, &st, paced, &g);
+    for (size_t i : g) {
+      const FrameParams& p = b->jobs[i].p;
+      b->images[i].resize(3ull * p.width * p.height);
+      b->loop->copy_out(p, b->images[i].data(), nullptr);
+    }
+    acc.camera_rays += st.camera_rays;
+    acc.closest_rays += st.closest_rays;
+    acc.shadow_rays += st.shadow_rays;
+    acc.waves += st.waves;
+    acc.max_wave = std::max(acc.max_wave, st.max_wave);
+    acc.ms_total += st.ms_total;
+    acc.ms_raygen += st.ms_raygen;
+    acc.ms_sort_rays += st.ms_sort_rays;
+    acc.ms_trace += st.ms_trace;
+    acc.ms_sort_hits += st.ms_sort_hits;
+    acc.ms_shade += st.ms_shade;
+    acc.ms_shadow += st.ms_shadow;
+  }
+  b->grouped = true;
+  if (stats) *stats = acc;
+}
+
 template <class F>
 int guarded_batch(swr_batch* b, F&& f) {
-  if (!b) return SWR_E_INVALID;
+  if (!b || !b->ctx) return SWR_E_INVALID;  // no batch, or its context was destroyed
   try {
     f();
     return SWR_OK;
@@ -129,6 +226,18 @@ int guarded_batch(swr_batch* b, F&& f) {
 }
 
 }  // namespace
+
+void detach_batch(swr_batch* b) {
+  if (!b->ctx) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  b->loop.reset();
+  b->bufs.release(b->ctx);
+  b->images.clear();
+  b->done = false;
+  b->ctx = nullptr;
+}
+
 }  // namespace swr
 
 using swr::Error;
@@ -139,15 +248,17 @@ int swr_batch_create(swr_context* ctx, swr_batch** out) {
   if (!ctx || !out) return SWR_E_INVALID;
   auto* b = new swr_batch();
   b->ctx = ctx;
+  ctx->batches.insert(b);
   *out = b;
   return SWR_OK;
 }
 
 void swr_batch_destroy(swr_batch* b) {
   if (!b) return;
-  (void)hipStreamSynchronize(b->ctx->stream);
-  b->loop.reset();
-  b->bufs.release(b->ctx);
+  if (b->ctx) {
+    b->ctx->batches.erase(b);
+    swr::detach_batch(b);
+  }
   delete b;
 }
 
@@ -176,7 +287,7 @@ int swr_batch_run(swr_batch* b, uint32_t wave_rays, uint32_t flags, swr_stats* s
     if (b->jobs.empty()) swr::fail(SWR_E_INVALID, "batch: no jobs");
     if (flags & ~SWR_BATCH_PACED) swr::fail(SWR_E_INVALID, "batch: unknown flags");
     b->done = false;
-    swr::run_jobs(b, b->jobs, wave_rays, true, stats, (flags & SWR_BATCH_PACED) != 0);
+    swr::run_batch(b, wave_rays, stats, (flags & SWR_BATCH_PACED) != 0);
     b->done = true;
   });
 }
@@ -185,6 +296,11 @@ int swr_batch_result(swr_batch* b, int32_t job, float* pixels, float* samples) {
   return swr::guarded_batch(b, [&] {
     if (!b->done || !b->loop) swr::fail(SWR_E_INVALID, "batch: not run");
     if (job < 0 || (size_t)job >= b->jobs.size()) swr::fail(SWR_E_INVALID, "batch: job index out of range");
+    if (b->grouped) {
+      if (samples) swr::fail(SWR_E_INVALID, "batch: sample records of a batch run in groups are not kept");
+      if (pixels) std::copy(b->images[job].begin(), b->images[job].end(), pixels);
+      return;
+    }
     b->loop->copy_out(b->jobs[job].p, pixels, samples);
   });
 }

@@ -62,6 +62,7 @@ struct SortWorkspace {
 }  // namespace swr
 
 struct swr_scene;
+struct swr_batch;
 
 namespace swr {
 struct LoopBuffers;
@@ -89,6 +90,7 @@ struct swr_context {
   swr::LoopBuffers* loop = nullptr;          // wavefront buffers of the plain render (render.hip)
   swr::DeviceBuffer trace_spill;             // traversal stack spill of the stand-alone queries
   std::unordered_set<swr_scene*> scenes;     // live scenes; destroying the context detaches them
+  std::unordered_set<swr_batch*> batches;    // live batches; likewise
   int num_cus = 0;
   uint32_t* sort_scratch = nullptr;          // device address of sort.hip's scratch array on `device`
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // brackets of the last timed entry point
@@ -113,6 +115,9 @@ struct Timer {
 // Called by swr_destroy for every live scene: the scene forgets its (now freed) device memory
 // and context; its handle stays valid for swr_scene_destroy.
 void detach_scene(swr_scene* s);
+// Called by swr_destroy for every live batch: the batch frees its device buffers and forgets its
+// context; its handle stays valid for swr_batch_destroy (every other call then fails).
+void detach_batch(swr_batch* b);
 
 // Allocation accounted against the context budget.
 void* dev_alloc(swr_context* ctx, size_t bytes);

@@ -98,3 +98,58 @@ def test_served_request_larger_than_the_state_budget(ctx, room, filt):
         acc = p if acc is None else acc + p
     np.testing.assert_array_equal(ref, acc * np.float32(1.0 / iters))
     sc.close()
+
+
+def test_batch_under_a_budget_runs_in_groups(room):
+    """A batch whose jobs' state exceeds the budget runs as groups of consecutive jobs, one loop
+    per group: every image is the same as in the unbudgeted run, and so are the ray counts."""
+    ctx = swr.Context(0)
+    try:
+        sc = swr.Scene(ctx, room)
+        rds = [_desc(room, width=40 + 17 * j, height=30 + 9 * j, base=1 + j % 3, seed=900 + j, filter=j % 2)
+               for j in range(7)]
+        b = swr.Batch(ctx)
+        for rd in rds:
+            b.add(sc, rd)
+        st = b.run(4096)
+        ref = [b.result(j)[0] for j in range(len(rds))]
+        state = sum(24 * rd.width * rd.height * rd.base * rd.base + 12 * rd.width * rd.height for rd in rds)
+        held = ctx.bytes_in_use()  # includes the arenas of the run above (the whole state)
+        biggest = max(24 * rd.width * rd.height * rd.base * rd.base + 12 * rd.width * rd.height for rd in rds)
+        budget = held - state + max(state // 3, biggest + 8192)  # room for about a third of the jobs
+        ctx.set_budget(budget)
+        st2 = b.run(4096)
+        assert ctx.bytes_in_use() <= budget
+        for j in range(len(rds)):
+            np.testing.assert_array_equal(b.result(j)[0], ref[j], err_msg=f"job {j}")
+        assert (st2["camera_rays"], st2["closest_rays"], st2["shadow_rays"]) == \
+               (st["camera_rays"], st["closest_rays"], st["shadow_rays"])
+        with pytest.raises(swr.SwrError) as e:
+            b.result(0, samples=True)  # per-sample records of a grouped run are not kept
+        assert e.value.code == swr.SWR_E_INVALID
+        # a job that cannot fit alone is refused
+        ctx.set_budget(held - state + biggest // 2)
+        with pytest.raises(swr.SwrError) as e:
+            b.run(4096)
+        assert e.value.code == swr.SWR_E_BUDGET
+        ctx.set_budget(0)
+        assert len(b.run(4096)) > 0  # and runs whole again without a budget
+        b.close()
+        sc.close()
+    finally:
+        ctx.close()
+
+
+def test_batch_and_scene_outlive_their_context(room):
+    """Destroying a context with a batch and a scene still alive detaches them: later calls fail
+    cleanly, and destroying the handles afterwards is safe."""
+    ctx = swr.Context(0)
+    sc = swr.Scene(ctx, room)
+    b = swr.Batch(ctx)
+    b.add(sc, _desc(room, width=16, height=12, base=1))
+    b.run()
+    ctx.close()
+    lib = swr.load_library()
+    assert lib.swr_batch_run(b.h, 0, 0, None) == swr.SWR_E_INVALID
+    b.close()
+    sc.close()
