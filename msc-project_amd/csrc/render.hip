@@ -503,8 +503,12 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const DeviceScene* 
     if (pending && (!kRingFilter || scene_ring[s0] == (uint32_t)ring)) {
       const RayState r = make_ray(make_float4(o.x, o.y, o.z, 0.001f), d);
       const void* nodes = kQuantNodes ? (const void*)scenes[s0].qnodes : (const void*)scenes[s0].nodes;
-      finish(traverse<kTraceBlock, kShort, true, false, true, kQuantNodes>(
-          nodes, scenes[s0].leaf_tris, r, SWR_MASK_SHADOW, s_ring + threadIdx.x, spill + (size_t)i * spill_stride));
+      if constexpr (kPostponeShadow)
+        finish(traverse_postponed<kTraceBlock, kShort, true, false, true, kQuantNodes>(
+            nodes, scenes[s0].leaf_tris, r, SWR_MASK_SHADOW, s_ring + threadIdx.x, spill + (size_t)i * spill_stride));
+      else
+        finish(traverse<kTraceBlock, kShort, true, false, true, kQuantNodes>(
+            nodes, scenes[s0].leaf_tris, r, SWR_MASK_SHADOW, s_ring + threadIdx.x, spill + (size_t)i * spill_stride));
     }
   } else if (pending && (!kRingFilter || scene_ring[scene] == (uint32_t)ring)) {
     const RayState r = make_ray(make_float4(o.x, o.y, o.z, 0.001f), d);

@@ -280,6 +280,51 @@ struct ShortStack {
   }
 };
 
+// Tests the triangles of leaf code `node` against the ray and updates best (the closest hit, at
+// equal t the smaller triangle id). kAny: returns true at the first hit (best holds it).
+// kPairTris: two triangles per step, loaded together; with an odd count the second load repeats
+// the first triangle's address and its test is skipped. Tests run in leaf order, so the first hit
+// of an any-hit query and the tie order of a closest-hit query are those of one at a time.
+template <bool kAny, bool kPairTris>
+__device__ inline bool leaf_tests(const RayState& r, const SWR_GLOBAL gf4* gtris, uint32_t node, uint32_t mask,
+                                  HitResult* best) {
+  const uint32_t first = leaf_first(node), count = leaf_count(node);
+  for (uint32_t k = 0; k < count; k += kPairTris ? 2 : 1) {
+    const bool two = kPairTris && k + 1 < count;
+    const SWR_GLOBAL gf4* ta = gtris + 3 * (size_t)(first + k);
+    const SWR_GLOBAL gf4* tb = two ? ta + 3 : ta;
+    const float4 va = gload(ta), ea = gload(ta + 1), fa = gload(ta + 2);
+    float4 vb = va, eb = ea, fb = fa;
+    if (kPairTris) vb = gload(tb), eb = gload(tb + 1), fb = gload(tb + 2);
+#pragma unroll
+    for (int s = 0; s < (kPairTris ? 2 : 1); ++s) {
+      if (s == 1 && !two) break;
+      const float4 v0 = s ? vb : va, e1 = s ? eb : ea, e2 = s ? fb : fa;
+      float t, u, v;
+      // The mask is checked after the test, so that the triangle's rows load together.
+      if (tri_test(r, v0, e1, e2, &t, &u, &v) && (__float_as_uint(e1.w) & mask) != 0u) {
+        const uint32_t id = __float_as_uint(v0.w);
+        if (kAny) {
+          best->t = t;
+          best->u = u;
+          best->v = v;
+          best->id = id;
+          best->leaf = first + k + s;
+          return true;
+        }
+        if (best->id == kMiss || t < best->t || (t == best->t && id < best->id)) {
+          best->t = t;
+          best->u = u;
+          best->v = v;
+          best->id = id;
+          best->leaf = first + k + s;
+        }
+      }
+    }
+  }
+  return false;
+}
+
 // Traverses one ray. kAny: returns on the first hit (id of that triangle), else the closest.
 // lds_col: this lane's LDS ring column (kShort entries, stride kBlock); spill_col: this lane's
 // HBM spill column (stack_need(depth) entries; unused when the ring is deep enough).
@@ -350,48 +395,98 @@ __device__ inline HitResult traverse(const void* __restrict__ nodes, const float
           continue;
         }
       }
-    } else {
-      // With kPairTris, two triangles per step, loaded together; with an odd count the second load
-      // repeats the first triangle's address and its test is skipped. Tests run in leaf order, so
-      // the first hit of an any-hit query and the tie order of a closest-hit query are those of one
-      // at a time.
-      const uint32_t first = leaf_first(node), count = leaf_count(node);
-      for (uint32_t k = 0; k < count; k += kPairTris ? 2 : 1) {
-        const bool two = kPairTris && k + 1 < count;
-        const SWR_GLOBAL gf4* ta = gtris + 3 * (size_t)(first + k);
-        const SWR_GLOBAL gf4* tb = two ? ta + 3 : ta;
-        const float4 va = gload(ta), ea = gload(ta + 1), fa = gload(ta + 2);
-        float4 vb = va, eb = ea, fb = fa;
-        if (kPairTris) vb = gload(tb), eb = gload(tb + 1), fb = gload(tb + 2);
-#pragma unroll
-        for (int s = 0; s < (kPairTris ? 2 : 1); ++s) {
-          if (s == 1 && !two) break;
-          const float4 v0 = s ? vb : va, e1 = s ? eb : ea, e2 = s ? fb : fa;
-          float t, u, v;
-          // The mask is checked after the test, so that the triangle's rows load together.
-          if (tri_test(r, v0, e1, e2, &t, &u, &v) && (__float_as_uint(e1.w) & mask) != 0u) {
-            const uint32_t id = __float_as_uint(v0.w);
-            if (kAny) {
-              best.t = t;
-              best.u = u;
-              best.v = v;
-              best.id = id;
-              best.leaf = first + k + s;
-              return best;
-            }
-            if (best.id == kMiss || t < best.t || (t == best.t && id < best.id)) {
-              best.t = t;
-              best.u = u;
-              best.v = v;
-              best.id = id;
-              best.leaf = first + k + s;
-            }
-          }
-        }
-      }
+    } else if (leaf_tests<kAny, kPairTris>(r, gtris, node, mask, &best)) {
+      return best;
     }
     if (st.empty()) break;
     node = st.pop();
+  }
+  return best;
+}
+
+// traverse() with leaves postponed (Aila and Laine, "Understanding the efficiency of ray
+// traversal on GPUs", HPG 2009, "while-while" with postponing): a lane that reaches a leaf keeps
+// it aside and goes on through internal nodes until every lane of the wave still walking internal
+// nodes holds a leaf; then the wave tests its held leaves together. Internal-node steps and leaf
+// steps then run with more of the wave's lanes on the same branch. Same results as traverse():
+// the closest hit (by t, then triangle id) does not depend on the visiting order, nor does
+// whether anything is hit. Measured on the 1080p terrain frame: shadow rays (any hit) 2% faster,
+// and 9% on the 256-job batch; closest-hit rays 2% slower (a held leaf cannot shorten the walk
+// that continues past it), so the closest-hit kernels keep traverse().
+template <int kBlock, int kShort, bool kAny, bool kPairTris = false, bool kNearFar = false, bool kQuant = false>
+__device__ inline HitResult traverse_postponed(const void* __restrict__ nodes, const float4* __restrict__ leaf_tris,
+                                               const RayState& r, uint32_t mask, uint32_t* lds_col,
+                                               uint32_t* spill_col) {
+  HitResult best;
+  best.t = r.tmax;
+  best.u = best.v = __int_as_float(0x7FC00000);
+  best.id = kMiss;
+  best.leaf = kMiss;
+  const float inf = __int_as_float(0x7F800000);
+  ShortStack<kBlock, kShort> st(lds_col, spill_col);
+  const SWR_GLOBAL char* gnodes = (const SWR_GLOBAL char*)nodes;
+  const SWR_GLOBAL gf4* gtris = global_f4(leaf_tris);
+  const PlaneRows pr = plane_rows(r);
+  constexpr uint32_t kDone = 0x7FFFFFFFu;  // the stack is empty (no node has this index)
+  uint32_t node = 0;                       // root
+  uint32_t leaf = 0;                       // the held leaf code, 0 for none
+  for (;;) {
+    while (node != kDone && !(node & kLeafBit)) {
+      uint4 ch;
+      float t0, t1, t2, t3;
+      if (kQuant)
+        node_entries_q(r, gnodes, node * 64u, best.t, &ch, &t0, &t1, &t2, &t3);
+      else
+        node_entries<kNearFar>(r, pr, gnodes, node * 128u, best.t, &ch, &t0, &t1, &t2, &t3);
+      if (kAny) {
+        const bool h0 = t0 != inf, h1 = t1 != inf, h2 = t2 != inf, h3 = t3 != inf;
+        if (h1 && h0) st.push(ch.y);
+        if (h2 && (h0 || h1)) st.push(ch.z);
+        if (h3 && (h0 || h1 || h2)) st.push(ch.w);
+        if (h0 || h1 || h2 || h3)
+          node = h0 ? ch.x : (h1 ? ch.y : (h2 ? ch.z : ch.w));
+        else
+          node = st.empty() ? kDone : st.pop();
+      } else {
+        uint32_t k0 = (__float_as_uint(t0) & ~3u) | 0u;
+        uint32_t k1 = (__float_as_uint(t1) & ~3u) | 1u;
+        uint32_t k2 = (__float_as_uint(t2) & ~3u) | 2u;
+        uint32_t k3 = (__float_as_uint(t3) & ~3u) | 3u;
+        cswap(k0, k1);
+        cswap(k2, k3);
+        cswap(k0, k2);
+        cswap(k1, k3);
+        cswap(k1, k2);
+        const uint32_t miss_key = 0x7F800000u;
+        const uint32_t cs[4] = {ch.x, ch.y, ch.z, ch.w};
+        const bool h1 = (k1 & ~3u) < miss_key, h2 = (k2 & ~3u) < miss_key, h3 = (k3 & ~3u) < miss_key;
+        st.reserve((int)h1 + (int)h2 + (int)h3);
+        if (h3) st.push_reserved(cs[k3 & 3u]);
+        if (h2) st.push_reserved(cs[k2 & 3u]);
+        if (h1) st.push_reserved(cs[k1 & 3u]);
+        if ((k0 & ~3u) < miss_key)
+          node = cs[k0 & 3u];
+        else
+          node = st.empty() ? kDone : st.pop();
+      }
+      // The first leaf reached is held and the walk goes on with the next stack entry.
+      if ((node & kLeafBit) && node != kDone && leaf == 0u) {
+        leaf = node;
+        node = st.empty() ? kDone : st.pop();
+      }
+      // Every lane still in this loop holds a leaf: test them together.
+      if (!__ballot(leaf == 0u)) break;
+    }
+    // The held leaf, then any leaf the walk stopped at.
+    while (leaf != 0u) {
+      if (leaf_tests<kAny, kPairTris>(r, gtris, leaf, mask, &best)) return best;
+      leaf = 0u;
+      if (node != kDone && (node & kLeafBit)) {
+        leaf = node;
+        node = st.empty() ? kDone : st.pop();
+      }
+    }
+    if (node == kDone) break;
   }
   return best;
 }
@@ -563,6 +658,11 @@ constexpr int kTraceBlock = 128;
 #define SWR_QUANT_NODES 1
 #endif
 constexpr bool kQuantNodes = SWR_QUANT_NODES != 0;
+// The one-scene shadow-ray traversal holds leaves back (traverse_postponed).
+#ifndef SWR_POSTPONE_SHADOW
+#define SWR_POSTPONE_SHADOW 1
+#endif
+constexpr bool kPostponeShadow = SWR_POSTPONE_SHADOW != 0;
 #ifndef SWR_SCENE_TILES
 #define SWR_SCENE_TILES 1
 #endif
