@@ -288,8 +288,12 @@ inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 
 
 }  // namespace
 
-// One node per thread: the quantised copy (NodeQ) of a BVH4 node.
-__global__ void quantize_kernel(const Node4* __restrict__ nodes, uint32_t n, NodeQ* __restrict__ out) {
+// One node per thread: the quantised copy (NodeQ) of a BVH4 node. Unused slots get an empty plane
+// interval and the code of the null leaf (one degenerate triangle that no ray hits), so that the
+// traversal needs no per-slot test: such a slot's box is missed, and if rounding ever lets a ray
+// into it, the leaf test finds nothing.
+__global__ void quantize_kernel(const Node4* __restrict__ nodes, uint32_t n, uint32_t null_leaf,
+                                NodeQ* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Node4 nd = nodes[i];
@@ -304,15 +308,17 @@ __global__ void quantize_kernel(const Node4* __restrict__ nodes, uint32_t n, Nod
   q.px = ax.p;
   q.py = ay.p;
   q.pz = az.p;
-  q.exps = ax.e | ay.e << 8 | az.e << 16;
+  q.sx = __uint_as_float(ax.e << 23);
+  q.sy = __uint_as_float(ay.e << 23);
+  q.sz = __uint_as_float(az.e << 23);
   q.qlox = ax.qlo;
   q.qhix = ax.qhi;
   q.qloy = ay.qlo;
   q.qhiy = ay.qhi;
   q.qloz = az.qlo;
   q.qhiz = az.qhi;
-  q.pad0 = q.pad1 = 0u;
-  q.child = nd.child;
+  q.child = make_uint4(valid[0] ? nd.child.x : null_leaf, valid[1] ? nd.child.y : null_leaf,
+                       valid[2] ? nd.child.z : null_leaf, valid[3] ? nd.child.w : null_leaf);
   out[i] = q;
 }
 
@@ -320,6 +326,7 @@ void quantize_bvh(swr_context* ctx, BvhBuild* out) {
   const uint32_t n = std::max<uint32_t>(out->num_nodes, 1);
   ensure(ctx, out->qnodes, (size_t)n * sizeof(NodeQ));
   quantize_kernel<<<blocks(n), 256, 0, ctx->stream>>>(static_cast<const Node4*>(out->nodes.ptr), n,
+                                                       null_leaf_code(out->num_tris),
                                                        static_cast<NodeQ*>(out->qnodes.ptr));
   SWR_HIP(hipGetLastError());
 }
@@ -330,7 +337,10 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
   leaf_max = std::max(1, std::min(leaf_max, 16));
   const size_t nn = std::max<uint32_t>(n, 1);
   ensure(ctx, out->nodes, nn * sizeof(Node4));
-  ensure(ctx, out->leaf_tris, nn * 3 * sizeof(float4));
+  // The leaf triangles plus the null leaf at index n (zeros: a degenerate triangle, mask 0).
+  ensure(ctx, out->leaf_tris, (nn + 1) * 3 * sizeof(float4));
+  SWR_HIP(hipMemsetAsync(static_cast<float4*>(out->leaf_tris.ptr) + 3 * (size_t)n, 0, 3 * sizeof(float4), s));
+  out->num_tris = n;
   Node4* nodes = static_cast<Node4*>(out->nodes.ptr);
 
   if (n == 0) {  // empty scene: a root without children

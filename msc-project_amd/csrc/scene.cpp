@@ -438,7 +438,8 @@ int swr_scene_prim_to_geom(const swr_scene* s, uint32_t prim, uint32_t* geom_id,
 }
 
 // Validates the BVH on the host: every triangle in exactly one leaf, leaf boxes contain their
-// triangles, child node boxes lie inside the parent's slot box, the leaf array is a permutation.
+// triangles, child node boxes lie inside the parent's slot box, the leaf array is a permutation
+// followed by the null leaf, the quantised copy conservative with the null leaf in unused slots.
 int swr_scene_bvh_info(const swr_scene* s, uint32_t* nodes_out, uint32_t* depth_out, uint32_t* valid_out) {
   if (!s || !s->committed) return SWR_E_INVALID;
   swr_context* ctx = s->ctx;
@@ -451,6 +452,14 @@ int swr_scene_bvh_info(const swr_scene* s, uint32_t* nodes_out, uint32_t* depth_
     SWR_HIP(hipMemcpy(qnodes.data(), s->bvh.qnodes.ptr, nn * sizeof(swr::NodeQ), hipMemcpyDeviceToHost));
     if (nt) SWR_HIP(hipMemcpy(lt.data(), s->bvh.leaf_tris.ptr, lt.size() * sizeof(float4), hipMemcpyDeviceToHost));
     bool ok = true;
+    {  // the null leaf after the triangles: all zeros
+      float4 null_tri[3];
+      SWR_HIP(hipMemcpy(null_tri, static_cast<const float4*>(s->bvh.leaf_tris.ptr) + 3 * (size_t)nt, sizeof null_tri,
+                        hipMemcpyDeviceToHost));
+      const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (const float4& v : null_tri)
+        if (std::memcmp(&v, &zero, sizeof v) != 0) ok = false;
+    }
     std::vector<uint8_t> seen_leaf(nt, 0), seen_id(nt, 0), seen_node(nn, 0);
     for (uint32_t j = 0; j < nt; ++j) {
       uint32_t id;
@@ -476,20 +485,30 @@ int swr_scene_bvh_info(const swr_scene* s, uint32_t* nodes_out, uint32_t* depth_
       const float* ly = &nd.loy.x; const float* hy = &nd.hiy.x;
       const float* lz = &nd.loz.x; const float* hz = &nd.hiz.x;
       const uint32_t* ch = &nd.child.x;
-      // The quantised copy: same children, planes at least one quantum outside every child box.
+      // The quantised copy: same children (the null leaf in unused slots), planes at least one
+      // quantum outside every child box.
       const swr::NodeQ& qn = qnodes[it.node];
-      if (std::memcmp(&qn.child, &nd.child, sizeof nd.child) != 0) ok = false;
+      const uint32_t* qch = &qn.child.x;
+      for (int c = 0; c < 4; ++c)
+        if (qch[c] != (ch[c] == swr::kNoChild ? swr::null_leaf_code(nt) : ch[c])) ok = false;
       {
         const float* los[3] = {lx, ly, lz};
         const float* his[3] = {hx, hy, hz};
         const float ps[3] = {qn.px, qn.py, qn.pz};
         const uint32_t qlo[3] = {qn.qlox, qn.qloy, qn.qloz}, qhi[3] = {qn.qhix, qn.qhiy, qn.qhiz};
         for (int a = 0; a < 3; ++a) {
-          const uint32_t e = (qn.exps >> (8 * a)) & 0xFFu;
+          const float sq[3] = {qn.sx, qn.sy, qn.sz};
+          uint32_t e;
+          std::memcpy(&e, &sq[a], 4);
+          if (e & 0x807FFFFFu) ok = false;  // a power of two
+          e >>= 23;
           if (e == 0u || e == 255u) ok = false;
           const double q = std::ldexp(1.0, (int)e - 127);
           for (int c = 0; c < 4; ++c) {
-            if (ch[c] == swr::kNoChild) continue;
+            if (ch[c] == swr::kNoChild) {  // an empty interval
+              if (((qlo[a] >> (8 * c)) & 0xFFu) != 255u || ((qhi[a] >> (8 * c)) & 0xFFu) != 0u) ok = false;
+              continue;
+            }
             const double l = (double)ps[a] + (double)((qlo[a] >> (8 * c)) & 0xFFu) * q;
             const double h = (double)ps[a] + (double)((qhi[a] >> (8 * c)) & 0xFFu) * q;
             if (!(l <= (double)los[a][c] - q && h >= (double)his[a][c] + q)) ok = false;

@@ -31,15 +31,16 @@ struct alignas(16) Node4 {
 static_assert(sizeof(Node4) == 128, "BVH4 node must be 128 bytes");
 
 // The same node with quantised child boxes (64 B, half a cache line): per axis the node box
-// origin p and a power-of-two quantum s = 2^(e - 127), and per child its lo and hi planes as bytes
-// q with plane = p + q * s, rounded outward by at least one quantum (so that the rounding of the
-// traversal's slab arithmetic never culls a box). Rows: (px, py, pz, e_x | e_y << 8 | e_z << 16),
-// (qlo_x, qhi_x, qlo_y, qhi_y), (qlo_z, qhi_z, 0, 0), child; byte c of a q word is child c.
+// origin p and a power-of-two quantum s = 2^(e - 127) (stored as the float s), and per child its
+// lo and hi planes as bytes q with plane = p + q * s, rounded outward by at least one quantum (so
+// that the rounding of the traversal's slab arithmetic never culls a box). Rows: (px, py, pz, sz),
+// (qlo_x, qhi_x, qlo_y, qhi_y), (qlo_z, qhi_z, sx, sy), child; byte c of a q word is child c. An
+// unused slot has planes qlo 255, qhi 0 (an empty interval) and the null leaf as its child.
 struct alignas(16) NodeQ {
-  float px, py, pz;
-  uint32_t exps;
+  float px, py, pz, sz;
   uint32_t qlox, qhix, qloy, qhiy;
-  uint32_t qloz, qhiz, pad0, pad1;
+  uint32_t qloz, qhiz;
+  float sx, sy;
   uint4 child;
 };
 static_assert(sizeof(NodeQ) == 64, "quantised node must be 64 bytes");
@@ -124,7 +125,11 @@ struct DeviceScene {
 struct BvhBuild {
   DeviceBuffer nodes, qnodes, leaf_tris;
   uint32_t num_nodes = 0, max_depth = 0;
+  uint32_t num_tris = 0;  // leaf_tris holds num_tris triangles and the null leaf after them
 };
+// Code of the null leaf: one degenerate triangle (all zeros, geometry mask 0) after the scene's
+// num_tris leaf triangles, which no ray hits; unused slots of quantised nodes point at it.
+__host__ __device__ inline uint32_t null_leaf_code(uint32_t num_tris) { return leaf_code(num_tris, 1u); }
 void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const GeomRec* geoms, uint32_t num_tris,
                const float lo[3], const float hi[3], int leaf_max, BvhBuild* out);
 // Writes out->qnodes, the quantised copy of out->nodes.

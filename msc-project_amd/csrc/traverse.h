@@ -38,8 +38,9 @@ __device__ inline float safe_inv(float d) {
 
 __device__ inline RayState make_ray(float4 o, float4 d) {
   RayState r;
-  r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = o.w;
-  r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = d.w;
+  // tmin and tmax canonicalised once here, so that the min/max of every box test need not.
+  r.ox = o.x; r.oy = o.y; r.oz = o.z; r.tmin = __builtin_canonicalizef(o.w);
+  r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = __builtin_canonicalizef(d.w);
   r.ix = safe_inv(d.x);
   r.iy = safe_inv(d.y);
   r.iz = safe_inv(d.z);
@@ -203,10 +204,7 @@ __device__ inline void node_entries_q(const RayState& r, const SWR_GLOBAL char* 
   const gu4 qb = *(const SWR_GLOBAL gu4*)(base + (node_off + 32u));
   const gu4 c = *(const SWR_GLOBAL gu4*)(base + (node_off + 48u));
   *ch = make_uint4(c.x, c.y, c.z, c.w);
-  const uint32_t e = __float_as_uint(h.w);
-  const float ax = __uint_as_float((e & 0xFFu) << 23) * r.ix;
-  const float ay = __uint_as_float(((e >> 8) & 0xFFu) << 23) * r.iy;
-  const float az = __uint_as_float(((e >> 16) & 0xFFu) << 23) * r.iz;
+  const float ax = __uint_as_float(qb.z) * r.ix, ay = __uint_as_float(qb.w) * r.iy, az = h.w * r.iz;
   const float bx = (h.x - r.ox) * r.ix, by = (h.y - r.oy) * r.iy, bz = (h.z - r.oz) * r.iz;
   const bool sx = r.ix < 0.f, sy = r.iy < 0.f, sz = r.iz < 0.f;
   const uint32_t nx = sx ? qa.y : qa.x, fx = sx ? qa.x : qa.y;
@@ -220,10 +218,11 @@ __device__ inline void node_entries_q(const RayState& r, const SWR_GLOBAL char* 
     const float tf = fminf(fminf(qplane(fx, k, ax, bx), qplane(fy, k, ay, by)), fminf(qplane(fz, k, az, bz), tfar));
     s[k] = tn <= tf * kBoxSlackQ ? tn : inf;
   }
-  *t0 = ch->x == kNoChild ? inf : s[0];
-  *t1 = ch->y == kNoChild ? inf : s[1];
-  *t2 = ch->z == kNoChild ? inf : s[2];
-  *t3 = ch->w == kNoChild ? inf : s[3];
+  // Unused slots need no test: their empty interval is missed, or the null leaf is entered.
+  *t0 = s[0];
+  *t1 = s[1];
+  *t2 = s[2];
+  *t3 = s[3];
 }
 
 __device__ inline void cswap(uint32_t& a, uint32_t& b) {
