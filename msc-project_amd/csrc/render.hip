@@ -184,6 +184,9 @@ __device__ inline uint32_t lower_bound_u32(const uint32_t* __restrict__ a, uint3
   return lo;
 }
 
+template <bool kOne, int kTile>
+constexpr bool kUniformTrace = kOne || kTile == kTileOne;
+
 // Held to 8 waves per SIMD (64 VGPRs): the one-scene traversal fits without spilling (63), and
 // more waves hide more of the node-fetch latency; the per-lane multi-scene form, left for tiles at
 // scene boundaries, spills 24 bytes.
@@ -199,6 +202,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
                                                                   uint32_t hit_shift,
                                                                   uint32_t* __restrict__ spill, uint32_t spill_stride) {
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
+  __shared__ uint32_t s_child[kUniformTrace<kOne, kTile> ? 4 * kTraceBlock : 1];
   const uint32_t n = w->n;
   uint32_t tile = blockIdx.x;
   if (kTile == kTileMixed) {
@@ -238,8 +242,9 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
     const RayState r = make_ray(o, d);
     constexpr bool kQ = kUniform && kQuantNodes;
     const void* nodes = kQ ? (const void*)scenes[s0].qnodes : (const void*)scenes[s0].nodes;
-    const HitResult h = traverse<kTraceBlock, kShort, false, kUniform, kUniform, kQ>(
-        nodes, scenes[s0].leaf_tris, r, SWR_MASK_ALL, s_ring + threadIdx.x, spill + (size_t)j * spill_stride);
+    const HitResult h = traverse<kTraceBlock, kShort, false, kUniform, kUniform, kQ, kUniform && kLdsChildren>(
+        nodes, scenes[s0].leaf_tris, r, SWR_MASK_ALL, s_ring + threadIdx.x, spill + (size_t)j * spill_stride,
+        s_child + (kUniform && kLdsChildren ? threadIdx.x : 0u));
     reinterpret_cast<uint2*>(&q[i].id)[1] = make_uint2(__float_as_uint(h.t), h.leaf);
     hit_keys[i] = (scenes[s0].hit_base + h.leaf + 1u) >> hit_shift;
   };

@@ -334,9 +334,15 @@ __device__ inline bool leaf_tests(const RayState& r, const SWR_GLOBAL gf4* gtris
 // Popped nodes are fetched and their children tested against the closest hit so far; keeping
 // each entry's distance to skip far ones unfetched cost more in stack traffic than it saved.
 // kQuant: nodes is the quantised node array (NodeQ) instead of the full one (Node4).
-template <int kBlock, int kShort, bool kAny, bool kPairTris = false, bool kNearFar = false, bool kQuant = false>
+// kChildLds (closest hit only): child_col is this lane's column of four LDS words (stride
+// kBlock). A node with children to push stores its four child codes there, and the pushes read
+// them back by slot: an LDS read per push instead of a chain of selects over the four registers
+// (the traversal is bound by vector-instruction issue; LDS has room).
+template <int kBlock, int kShort, bool kAny, bool kPairTris = false, bool kNearFar = false, bool kQuant = false,
+          bool kChildLds = false>
 __device__ inline HitResult traverse(const void* __restrict__ nodes, const float4* __restrict__ leaf_tris,
-                                     const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col) {
+                                     const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col,
+                                     uint32_t* child_col = nullptr) {
   HitResult best;
   best.t = r.tmax;
   best.u = best.v = __int_as_float(0x7FC00000);
@@ -385,10 +391,22 @@ __device__ inline HitResult traverse(const void* __restrict__ nodes, const float
         // Push hit children far to near except the nearest (keys ascend, so a hit k3 implies
         // hits k0..k2), then descend into the nearest.
         const bool h1 = (k1 & ~3u) < miss_key, h2 = (k2 & ~3u) < miss_key, h3 = (k3 & ~3u) < miss_key;
-        st.reserve((int)h1 + (int)h2 + (int)h3);
-        if (h3) st.push_reserved(cs[k3 & 3u]);
-        if (h2) st.push_reserved(cs[k2 & 3u]);
-        if (h1) st.push_reserved(cs[k1 & 3u]);
+        if (h1) {
+          st.reserve((int)h1 + (int)h2 + (int)h3);
+          if (kChildLds) {
+            child_col[0] = ch.x;
+            child_col[kBlock] = ch.y;
+            child_col[2 * kBlock] = ch.z;
+            child_col[3 * kBlock] = ch.w;
+            if (h3) st.push_reserved(child_col[(k3 & 3u) * kBlock]);
+            if (h2) st.push_reserved(child_col[(k2 & 3u) * kBlock]);
+            st.push_reserved(child_col[(k1 & 3u) * kBlock]);
+          } else {
+            if (h3) st.push_reserved(cs[k3 & 3u]);
+            if (h2) st.push_reserved(cs[k2 & 3u]);
+            st.push_reserved(cs[k1 & 3u]);
+          }
+        }
         if ((k0 & ~3u) < miss_key) {
           node = cs[k0 & 3u];
           continue;
@@ -657,6 +675,11 @@ constexpr int kTraceBlock = 128;
 #define SWR_QUANT_NODES 1
 #endif
 constexpr bool kQuantNodes = SWR_QUANT_NODES != 0;
+// The one-scene closest-hit traversal reads pushed children back from LDS (traverse's child_col).
+#ifndef SWR_LDS_CHILDREN
+#define SWR_LDS_CHILDREN 1
+#endif
+constexpr bool kLdsChildren = SWR_LDS_CHILDREN != 0;
 // The one-scene shadow-ray traversal holds leaves back (traverse_postponed).
 #ifndef SWR_POSTPONE_SHADOW
 #define SWR_POSTPONE_SHADOW 1
