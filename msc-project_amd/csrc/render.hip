@@ -89,33 +89,44 @@ __device__ inline uint32_t find_range(const JobRange* __restrict__ ranges, uint3
 
 // Camera rays for the wave's k new paths into its last k queue slots; resets each sample's
 // radiance and records its film position (Camera.cpp:23-33).
+// Paths are generated in sample-id order (a pixel's samples together: coherent camera rays), but
+// per-sample state is stored sample-major (slot = (x * spp + k) * H + y), so a lane's own slot lies
+// a column height away from its neighbour's. A workgroup whose paths are one run of sample ids of
+// one job (nearly all of them) stages the film positions in LDS and writes the state in slot
+// order instead: lane e takes the e-th sample of the workgroup's pixels in (sample, pixel) order,
+// so that consecutive lanes write consecutive slots (measured on the terrain frame: raygen 3.85 ->
+// 3.26 ms; 256- and 1024-lane workgroups 3.33 and 3.35 ms).
+constexpr uint32_t kRaygenBlock = 512;
 template <bool kOne>
-__global__ __launch_bounds__(256) void raygen_kernel(const FrameParams* __restrict__ jobs,
-                                                     const JobRange* __restrict__ ranges, uint32_t range_mask,
-                                                     const WaveState* __restrict__ w, RayRec* __restrict__ q,
-                                                     uint32_t* __restrict__ keys, float2* __restrict__ sample_xy,
-                                                     float4* __restrict__ radiance) {
+__global__ __launch_bounds__(kRaygenBlock) void raygen_kernel(const FrameParams* __restrict__ jobs,
+                                                              const JobRange* __restrict__ ranges, uint32_t range_mask,
+                                                              const WaveState* __restrict__ w, RayRec* __restrict__ q,
+                                                              uint32_t* __restrict__ keys,
+                                                              float2* __restrict__ sample_xy,
+                                                              float4* __restrict__ radiance) {
+  __shared__ float2 s_xy[kRaygenBlock];
+  __shared__ uint32_t s_first[3];
   const uint32_t k = w->k;
   if (blockIdx.x * blockDim.x >= k) return;
   // Workgroups go round robin to the 8 XCDs (one L2 each); a sample's radiance and film position
-  // are one partial line of a slot row that the neighbouring pixels' workgroups complete. Each XCD
-  // therefore takes one contiguous run of the wave's active workgroups, so that most writes to a
-  // line meet in one L2. Only which workgroup computes which rays changes, not what is written
-  // where. (Measured on the terrain frame: raygen 3.92 -> 3.87 ms, frame -0.5%.)
+  // are part of a slot row that the neighbouring pixels' workgroups complete. Each XCD therefore
+  // takes one contiguous run of the wave's active workgroups, so that most writes to a line meet
+  // in one L2. Only which workgroup computes which rays changes, not what is written where.
   const uint32_t active = (k + blockDim.x - 1) / blockDim.x, per = active / 8u, rem = active % 8u;
   const uint32_t xcd = blockIdx.x % 8u;
   const uint32_t logical = xcd * per + min(xcd, rem) + blockIdx.x / 8u;
   const uint32_t t = logical * blockDim.x + threadIdx.x;
   bool pending = t < k;
-  uint32_t job = 0, sid_local = 0;
+  uint32_t job = 0, sid_local = 0, range = 0;
   if (pending) {
     const uint64_t g = w->raygen_base + t;
-    const JobRange rg = ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, g)];
+    range = find_range(ranges, range_mask, w->range_lo, w->num_ranges, g);
+    const JobRange rg = ranges[range];
     job = rg.job;
     sid_local = (uint32_t)(g - rg.path_begin);  // index within the job; its sample id adds sid_base
   }
   const uint32_t slot = w->n - k + t;
-  auto body = [&](uint32_t j0) {
+  auto body = [&](uint32_t j0, bool state) {
     const FrameParams& p = jobs[j0];
     const uint32_t sid = p.sid_base + sid_local;
     V3 o, d;
@@ -128,14 +139,47 @@ __global__ __launch_bounds__(256) void raygen_kernel(const FrameParams* __restri
     r.id = make_uint4(sid, j0 << 16, p.scene, kMiss);
     q[slot] = r;
     keys[slot] = queue_key(p, o, d);
-    const uint64_t s = p.sample_base + slot_of_sid(p, sid);
-    sample_xy[s] = make_float2(sx, sy);
-    radiance[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (state) {
+      const uint64_t s = p.sample_base + slot_of_sid(p, sid);
+      sample_xy[s] = make_float2(sx, sy);
+      radiance[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      s_xy[threadIdx.x] = make_float2(sx, sy);
+    }
   };
-  if (kOne) {
-    if (pending) body(0u);
+  // The workgroup's pending lanes are a prefix of it; they hold one run of sample ids of one job
+  // when they all lie in the first lane's range.
+  const uint32_t cnt = min(kRaygenBlock, k - (t - threadIdx.x));
+  if (threadIdx.x == 0) {
+    s_first[0] = range;
+    s_first[1] = sid_local;
+    s_first[2] = job;
+  }
+  __syncthreads();
+  if (__syncthreads_and(!pending || range == s_first[0])) {
+    const uint32_t j0 = kOne ? 0u : __builtin_amdgcn_readfirstlane(s_first[2]);
+    const FrameParams& p = jobs[j0];
+    const uint32_t spp = p.spp, sid0 = p.sid_base + s_first[1];
+    // Pixels (sample id / spp, column-major) the run touches, the first and last partly; lanes
+    // take the samples of those pixels in (sample, pixel) order and skip those outside the run.
+    // Only for a few pixels' worth of samples (else each lane writes its own).
+    const uint32_t pix0 = sid0 / spp;
+    const uint64_t npix = ((uint64_t)sid0 + cnt - 1u) / spp - pix0 + 1u;
+    const bool staged = npix * spp <= 4u * kRaygenBlock;
+    if (pending) body(j0, !staged);
+    if (!staged) return;  // the whole workgroup
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < (uint32_t)(npix * spp); e += kRaygenBlock) {
+      const uint32_t sid = (pix0 + e % (uint32_t)npix) * spp + e / (uint32_t)npix;
+      if (sid < sid0 || sid - sid0 >= cnt) continue;
+      const uint64_t s = p.sample_base + slot_of_sid(p, sid);
+      sample_xy[s] = s_xy[sid - sid0];
+      radiance[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  } else if (kOne) {
+    if (pending) body(0u, true);
   } else {
-    SWR_BY_UNIFORM(pending, job, j0, { body(j0); })
+    SWR_BY_UNIFORM(pending, job, j0, { body(j0, true); })
   }
 }
 
@@ -1008,9 +1052,9 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
   const bool one_job = max_jobs_ == 1 && scenes_.size() == 1, one_scene = scenes_.size() == 1;
   if (kb) {
     if (one_job)
-      raygen_kernel<true><<<checked_grid(blocks(kb), 256, "raygen"), 256, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
+      raygen_kernel<true><<<checked_grid(blocks(kb, kRaygenBlock), kRaygenBlock, "raygen"), kRaygenBlock, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
     else
-      raygen_kernel<false><<<checked_grid(blocks(kb), 256, "raygen"), 256, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
+      raygen_kernel<false><<<checked_grid(blocks(kb, kRaygenBlock), kRaygenBlock, "raygen"), kRaygenBlock, 0, st>>>(jobs, ranges, mask, w, queue[cur], keys[cur], sxy_, radiance_);
   }
   mark(1);
   const uint32_t shade_blocks = nb ? checked_grid(blocks(nb, one_job ? kShadeBlock : kShadeBlockMulti),
