@@ -217,8 +217,9 @@ def main():
             "traffic": None,
             "avg_launch_ms": round(agg["ms_trace"] / agg["waves"], 4),
             "rays_per_launch": agg["closest_rays"] // agg["waves"],
-            "note": "algorithmic 48 B/ray (ray in + hit out), node traffic excluded; traversal is bound by the "
-                    "latency of node fetches that miss L2, not by HBM bandwidth",
+            "note": "algorithmic 48 B/ray (ray in + hit out), node traffic excluded; traversal is bound by "
+                    "vector-instruction issue at 8 waves per SIMD over chains of node fetches that miss L2 "
+                    "half the time, not by HBM bandwidth",
             "mrays_s": round(agg["closest_rays"] / (agg["ms_trace"] * 1e-3) / 1e6, 1),
             "l2_hit": None,
         },

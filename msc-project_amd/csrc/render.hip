@@ -1476,7 +1476,7 @@ int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc
     auto time_it = [&](const swr::Tuning& t) {
       ctx->tuning = t;
       double best = 1e30;
-      for (int rep = 0; rep < 2; ++rep) {
+      for (int rep = 0; rep < 3; ++rep) {
         swr_stats st{};
         swr::render_frame(ctx, scene, d, nullptr, nullptr, &st);
         best = std::min(best, st.ms_total);
@@ -1486,33 +1486,36 @@ int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc
     swr::Tuning best = ctx->tuning;
     swr::render_frame(ctx, scene, d, nullptr, nullptr, nullptr);  // allocations outside the timings
     double best_ms = time_it(best);
+    // A candidate replaces the best so far only when it is faster by more than run-to-run noise
+    // on the reduced frame (1%); key widths are tried from the fewest sort passes up, so that
+    // ties keep the cheaper sort.
+    auto consider = [&](const swr::Tuning& t) {
+      const double ms = time_it(t);
+      if (ms < best_ms * 0.99) best_ms = ms, best = t;
+    };
     for (int alg : {SWR_SORT_ONESWEEP, SWR_SORT_REDUCE_SCAN})
       for (int kpt : {8, 12, 16, 20, 24}) {
         swr::Tuning t = best;
         t.sort_algorithm = alg;
         t.sort_kpt = kpt;
-        const double ms = time_it(t);
-        if (ms < best_ms) best_ms = ms, best = t;
+        consider(t);
       }
-    for (int rb : {32, 24, 20, 16, 12, 8}) {
+    for (int rb : {8, 12, 16, 20, 24, 32}) {
       swr::Tuning t = best;
       t.ray_key_bits = rb;
-      const double ms = time_it(t);
-      if (ms < best_ms) best_ms = ms, best = t;
+      consider(t);
     }
     const int full = swr::bits_for((uint64_t)scene->dev.num_tris);
-    for (int hb : {0, full - 6, full - 12, 8}) {
+    for (int hb : {8, full - 12, full - 6, 0}) {
       if (hb < 0 || (hb > 0 && hb < 4) || (hb == 8 && full - 12 <= 8)) continue;
       swr::Tuning t = best;
       t.hit_key_bits = hb;
-      const double ms = time_it(t);
-      if (ms < best_ms) best_ms = ms, best = t;
+      consider(t);
     }
     for (int ring : {8, 16, 32, 64}) {
       swr::Tuning t = best;
       t.trace_stack = ring;
-      const double ms = time_it(t);
-      if (ms < best_ms) best_ms = ms, best = t;
+      consider(t);
     }
     ctx->tuning = best;
     if (out) swr_get_tuning(ctx, out);
