@@ -321,18 +321,26 @@ def kernels_bench(ctx, scene, desc, args):
         return {"ms": round(m, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
                 "bits": bits, "bit_exact": ok}
 
-    # the sort's launch shapes tuned for this size: both pass structures at every tile size
+    # the sort's launch shapes tuned for this size: the LSD pass structures at every tile size and
+    # the bucket sort (whose tiles are fixed), each key type taking the fastest
+    names = ["onesweep", "reduce-then-scan", "bucket"]
     saved = ctx.tuning()
-    sweep = {}
-    for alg in (0, 1):
-        for kpt in (8, 12, 16, 20, 24):
-            ctx.set_tuning(**dict(saved, sort_algorithm=alg, sort_keys_per_thread=kpt))
-            sweep[(alg, kpt)] = time_sort(keys, 32)
-    best = min(sweep, key=lambda c: sweep[c]["ms"])
-    ctx.set_tuning(**dict(saved, sort_algorithm=best[0], sort_keys_per_thread=best[1]))
-    res["ray_sort"] = dict(sweep[best], algorithm=["onesweep", "reduce-then-scan"][best[0]],
-                           keys_per_thread=best[1],
-                           sweep_ms={f"{['onesweep', 'scan'][a]}/{k}": v["ms"] for (a, k), v in sweep.items()})
+
+    def sweep_sort(host_keys, bits):
+        sweep = {}
+        for alg in (0, 1):
+            for kpt in (8, 12, 16, 20, 24):
+                ctx.set_tuning(**dict(saved, sort_algorithm=alg, sort_keys_per_thread=kpt))
+                sweep[(alg, kpt)] = time_sort(host_keys, bits)
+        ctx.set_tuning(**dict(saved, sort_algorithm=2))
+        sweep[(2, 0)] = time_sort(host_keys, bits)
+        best = min(sweep, key=lambda c: sweep[c]["ms"])
+        return dict(sweep[best], algorithm=names[best[0]], keys_per_thread=best[1] or None,
+                    sweep_ms={(f"{['onesweep', 'scan'][a]}/{k}" if a < 2 else "bucket"): v["ms"]
+                              for (a, k), v in sweep.items()},
+                    all_bit_exact=all(v["bit_exact"] for v in sweep.values()))
+
+    res["ray_sort"] = sweep_sort(keys, 32)
     del keys
     rng = np.random.default_rng(0x5EED0007)
     mesh = rng.integers(0, 64, n, dtype=np.uint32)
@@ -340,7 +348,7 @@ def kernels_bench(ctx, scene, desc, args):
     d_mesh, d_face = ctx.upload(mesh), ctx.upload(face)
     ctx.mesh_face_keys(d_mesh, d_face, n, d_keys)
     hkeys = d_keys.download(np.uint32, (n,))
-    res["hit_sort"] = time_sort(hkeys, 28)
+    res["hit_sort"] = sweep_sort(hkeys, 28)
     res["hit_sort"]["keys_match"] = bool(np.array_equal(hkeys, ((mesh + 1) << 21) | face))
     ctx.set_tuning(**saved)
     for a in (d_mesh, d_face, d_keys, d_vals):

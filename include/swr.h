@@ -128,10 +128,14 @@ typedef struct {
 } swr_stats;
 
 /* Pass structures of the radix sort: onesweep (one kernel per digit, decoupled look-back over
- * earlier tiles) or reduce-then-scan (per digit: tile counts, a scan over tiles, the scatter;
- * no tile waits for another). */
+ * earlier tiles), reduce-then-scan (per digit: tile counts, a scan over tiles, the scatter;
+ * no tile waits for another), or the bucket sort (one stable partition of the key range into
+ * 2048 buckets, then each bucket sorted on its remaining bits in LDS: two trips through device
+ * memory instead of one per 8-bit digit; used for sorts of more than 16 key bits whose count is
+ * known at launch, reduce-then-scan otherwise). */
 #define SWR_SORT_ONESWEEP 0
 #define SWR_SORT_REDUCE_SCAN 1
+#define SWR_SORT_BUCKET 2
 
 /* Launch shapes of the loop. The reference exposes the equivalent knobs as settings
  * (bucket size, shading size, bin exponent; Settings.h:16-24); these are chosen per context by
@@ -143,7 +147,7 @@ typedef struct {
   int32_t trace_stack_entries;   /* LDS ring of the traversal stack per ray: 8, 16, 32 or 64 entries, deeper
                                     entries spill to device memory (0 = per scene: the ring
                                     swr_batch_autotune chose for it, else sized from its BVH depth) */
-  int32_t sort_algorithm;        /* SWR_SORT_ONESWEEP or SWR_SORT_REDUCE_SCAN */
+  int32_t sort_algorithm;        /* SWR_SORT_ONESWEEP, SWR_SORT_REDUCE_SCAN or SWR_SORT_BUCKET */
 } swr_tuning;
 
 /* ---- context (Pathtracer ctor/dtor, Pathtracer.cpp:369-392) ------------------------------ */
@@ -178,13 +182,14 @@ int swr_memcpy_h2d(swr_context* ctx, void* dst_dev, const void* src, size_t byte
 int swr_memcpy_d2h(swr_context* ctx, void* dst, const void* src_dev, size_t bytes);
 
 /* ---- sort and key primitives (stages of Pathtracer::process) ------------------------------ */
-/* Stable LSD radix sort of (key, value) pairs in place, 8-bit digits, one pass per digit, each
- * pass onesweep with decoupled look-back or reduce-then-scan, as the context tuning's
- * sort_algorithm says (onesweep on a new context; swr_autotune may choose either).
+/* Stable radix sort of (key, value) pairs in place, as the context tuning's sort_algorithm says
+ * (onesweep on a new context; swr_autotune may choose another): LSD passes of 8-bit digits, each
+ * onesweep with decoupled look-back or reduce-then-scan, or the bucket sort (a partition of the
+ * key range into 2048 buckets, then every bucket sorted in LDS) for keys of more than 16 bits.
  * Replaces the ray sort (RaySort.cpp:7-118, Pathtracer.cpp:321-327) and the hit sort
  * (tbb::parallel_sort with CompareHit, Pathtracer.cpp:335-339). key_bits (1..32) is the key
- * width: keys must lie below 2^key_bits, and the passes order them by their ceil(key_bits/8) low
- * bytes. With iota_values = 1 the input values are not read and the output
+ * width: keys must lie below 2^key_bits, and the sort orders them by their low key_bits bits.
+ * With iota_values = 1 the input values are not read and the output
  * values are the stable permutation (original index of each sorted key). n <= 2^30. */
 int swr_sort_pairs_u32(swr_context* ctx, uint32_t* keys_dev, uint32_t* vals_dev, size_t n, int key_bits,
                        int iota_values);

@@ -163,10 +163,11 @@ int swr_set_tuning(swr_context* ctx, const swr_tuning* t) {
       t->hit_key_bits < 0 || t->hit_key_bits > 32 ||
       !(t->trace_stack_entries == 0 || t->trace_stack_entries == 8 || t->trace_stack_entries == 16 ||
         t->trace_stack_entries == 32 || t->trace_stack_entries == 64) ||
-      !(t->sort_algorithm == SWR_SORT_ONESWEEP || t->sort_algorithm == SWR_SORT_REDUCE_SCAN)) {
+      !(t->sort_algorithm == SWR_SORT_ONESWEEP || t->sort_algorithm == SWR_SORT_REDUCE_SCAN ||
+        t->sort_algorithm == SWR_SORT_BUCKET)) {
     ctx->err =
         "set_tuning: keys per thread must be 8/12/16/20/24, ray key bits 3..32, hit key bits 0..32, "
-        "trace stack entries 0/8/16/32/64, sort algorithm 0/1";
+        "trace stack entries 0/8/16/32/64, sort algorithm 0/1/2";
     return SWR_E_INVALID;
   }
   ctx->tuning.sort_kpt = k;

@@ -390,6 +390,649 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
   }
 }
 
+// ---- bucket sort: one partition pass over the key range, then every bucket sorted on chip ----
+// Four 8-bit LSD passes move each pair through HBM four times. The bucket sort moves it twice:
+//   1. the range [kmin, kmax] of the keys (one read of the keys);
+//   2. a stable partition into 2048 buckets by the top 11 bits of key - kmin over that range
+//      (upsweep of per-tile bucket counts, a scan over tiles, then the scatter: the scanned pass
+//      structure above with an 11-bit digit and 16 K-key tiles);
+//   3. each bucket (about n / 2048 pairs on keys spread over their range) loaded into LDS, sorted
+//      stably on its remaining bits by LSD passes of at most 8 bits inside the workgroup, and
+//      written once into the caller's buffers.
+// A bucket larger than the LDS holds is sorted by its workgroup through device memory instead
+// (LSD passes over its range of both buffers), which is correct for any key distribution and
+// slow only for heavily clustered keys.
+constexpr int kBkBits = 11;
+constexpr int kBkBins = 1 << kBkBits;
+constexpr int kBkBlock = 512;
+constexpr int kBkWaves = kBkBlock / 64;
+#ifndef SWR_BK_KPT
+#define SWR_BK_KPT 16
+#endif
+constexpr int kBkKpt = SWR_BK_KPT;  // keys per lane of a partition tile
+constexpr int kBkTile = kBkBlock * kBkKpt;
+constexpr int kBkRangeBlock = 1024;
+constexpr int kBkRangeGroups = 256;  // at most: the consumers reduce one (min, max) pair per group
+constexpr int kBkLocalBlock = 1024;  // lanes of the local sort kernels
+constexpr int kBkLocalWaves = kBkLocalBlock / 64;
+constexpr int kBkLocalKpt = 12;
+constexpr int kBkCap = kBkLocalBlock * kBkLocalKpt;  // largest bucket sorted in LDS (12288 pairs)
+
+// Per-workgroup (min, max) of the keys: mm[2 g], mm[2 g + 1]. Also clears the count of buckets
+// beyond the LDS that the local kernel appends to.
+__global__ __launch_bounds__(kBkRangeBlock) void bk_range_kernel(const uint32_t* __restrict__ keys, uint32_t n,
+                                                                 uint32_t* __restrict__ mm, uint32_t* __restrict__ big) {
+  __shared__ uint32_t s_lo[kBkRangeBlock / 64], s_hi[kBkRangeBlock / 64];
+  const uint32_t t = threadIdx.x, stride = gridDim.x * kBkRangeBlock;
+  if (blockIdx.x == 0 && t == 0) big[0] = 0;
+  uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+  const uint32_t n4 = (reinterpret_cast<uintptr_t>(keys) & 15u) == 0 ? n / 4 : 0;
+  const uint4* k4 = reinterpret_cast<const uint4*>(keys);
+  uint32_t i = blockIdx.x * kBkRangeBlock + t;
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    const uint4 a = k4[i], b = k4[i + stride], c = k4[i + 2 * stride], d = k4[i + 3 * stride];
+    lo = min(lo, min(min(min(a.x, a.y), min(a.z, a.w)), min(min(b.x, b.y), min(b.z, b.w))));
+    lo = min(lo, min(min(min(c.x, c.y), min(c.z, c.w)), min(min(d.x, d.y), min(d.z, d.w))));
+    hi = max(hi, max(max(max(a.x, a.y), max(a.z, a.w)), max(max(b.x, b.y), max(b.z, b.w))));
+    hi = max(hi, max(max(max(c.x, c.y), max(c.z, c.w)), max(max(d.x, d.y), max(d.z, d.w))));
+  }
+  for (; i < n4; i += stride) {
+    const uint4 a = k4[i];
+    lo = min(lo, min(min(a.x, a.y), min(a.z, a.w)));
+    hi = max(hi, max(max(a.x, a.y), max(a.z, a.w)));
+  }
+  for (uint32_t j = 4 * n4 + blockIdx.x * kBkRangeBlock + t; j < n; j += stride) {
+    lo = min(lo, keys[j]);
+    hi = max(hi, keys[j]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, off, 64));
+  }
+  if ((t & 63) == 0) {
+    s_lo[t >> 6] = lo;
+    s_hi[t >> 6] = hi;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < kBkRangeBlock / 64; ++w) {
+      lo = min(lo, s_lo[w]);
+      hi = max(hi, s_hi[w]);
+    }
+    mm[2 * blockIdx.x] = lo;
+    mm[2 * blockIdx.x + 1] = hi;
+  }
+}
+
+// The partition digit of a key is (key - kmin) >> shift, with shift chosen so that the digit of
+// kmax is below 2048: the top 11 bits of the range. Buckets are ordered like their keys.
+struct BkRange {
+  uint32_t kmin, shift;
+};
+// Reduces the range kernel's per-group pairs; every thread of the kBlock-lane workgroup calls it
+// (s_red: 2 kBlock / 64 words).
+template <int kBlock = kBkBlock>
+__device__ inline BkRange bk_key_range(const uint32_t* __restrict__ mm, int groups, uint32_t* s_red) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  constexpr int kW = kBlock / 64;
+  uint32_t lo = t < groups ? mm[2 * t] : 0xFFFFFFFFu;
+  uint32_t hi = t < groups ? mm[2 * t + 1] : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, off, 64));
+  }
+  if (lane == 0) {
+    s_red[w] = lo;
+    s_red[kW + w] = hi;
+  }
+  __syncthreads();
+  lo = s_red[0];
+  hi = s_red[kW];
+#pragma unroll
+  for (int i = 1; i < kW; ++i) {
+    lo = min(lo, s_red[i]);
+    hi = max(hi, s_red[kW + i]);
+  }
+  __syncthreads();
+  const uint32_t span = hi - lo;
+  const uint32_t bits = span ? 32u - (uint32_t)__clz((int)span) : 0u;
+  return BkRange{lo, bits > (uint32_t)kBkBits ? bits - (uint32_t)kBkBits : 0u};
+}
+
+// Tile t of a partition launch with a grid of exactly `tiles` workgroups: each XCD takes one
+// contiguous run of tiles and walks it from its end (see the scanned pass kernel above).
+__device__ inline uint32_t bk_tile_of_block(uint32_t tiles) {
+  const uint32_t per = tiles / 8u, rem = tiles % 8u, x = blockIdx.x % 8u, cnt = per + (x < rem ? 1u : 0u);
+  return x * per + min(x, rem) + (cnt - 1u - blockIdx.x / 8u);
+}
+
+// Bucket counts of one partition tile, tile-major: counts[tile * 2048 + bucket].
+template <int kKpt>
+__global__ __launch_bounds__(kBkBlock) void bk_upsweep_kernel(const uint32_t* __restrict__ keys, uint32_t n,
+                                                              const uint32_t* __restrict__ mm, int groups,
+                                                              uint32_t* __restrict__ counts) {
+  constexpr int kTile = kBkBlock * kKpt;
+  __shared__ uint32_t s_h[kBkBins];
+  __shared__ uint32_t s_red[2 * kBkWaves];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint32_t tiles = (n + kTile - 1) / kTile;
+  const uint32_t tile = bk_tile_of_block(tiles);
+  for (int i = t; i < kBkBins; i += kBkBlock) s_h[i] = 0;
+  const BkRange r = bk_key_range(mm, groups, s_red);  // synchronises the workgroup
+  const uint32_t wbase = tile * (uint32_t)kTile + (uint32_t)w * 64u * kKpt;
+  uint32_t k[kKpt];
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    k[i] = p < n ? keys[p] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) hist_add(s_h, (k[i] - r.kmin) >> r.shift, wbase + i * 64 + lane < n);
+  __syncthreads();
+  uint32_t* row = counts + (size_t)tile * kBkBins;
+  for (int i = t; i < kBkBins; i += kBkBlock) row[i] = s_h[i];
+}
+
+// Exclusive prefix over tiles of every bucket's counts (in place) and each bucket's total.
+// A workgroup takes 16 buckets; lane = (tile group g = lane / 16, bucket = lane % 16), so a wave
+// reads four 64-byte row pieces per load.
+constexpr int kBkScanBlock = 1024;
+constexpr int kBkScanCols = 16;
+__global__ __launch_bounds__(kBkScanBlock) void bk_scan_kernel(uint32_t* __restrict__ counts, uint32_t tiles,
+                                                               uint32_t* __restrict__ totals) {
+  constexpr int kGroups = kBkScanBlock / kBkScanCols;
+  __shared__ uint32_t s_part[kGroups][kBkScanCols];
+  const int t = threadIdx.x, c = t % kBkScanCols, g = t / kBkScanCols;
+  const uint32_t b = blockIdx.x * kBkScanCols + c;
+  const uint32_t per = (tiles + kGroups - 1) / kGroups, tb = min(tiles, g * per), te = min(tiles, tb + per);
+  uint32_t sum = 0;
+#pragma unroll 8
+  for (uint32_t i = tb; i < te; ++i) sum += counts[(size_t)i * kBkBins + b];
+  s_part[g][c] = sum;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int gg = 0; gg < g; ++gg) pre += s_part[gg][c];
+  if (g == kGroups - 1) totals[b] = pre + sum;
+#pragma unroll 8
+  for (uint32_t i = tb; i < te; ++i) {
+    uint32_t* p = counts + (size_t)i * kBkBins + b;
+    const uint32_t v = *p;
+    *p = pre;
+    pre += v;
+  }
+}
+
+// The partition pass: a tile of 512 * kKpt keys ranked stably by bucket inside each wave (an LDS
+// peer word over the bucket's low 8 bits, refined by three ballots over its top 3 bits), staged in
+// LDS in bucket order and written as runs of equal buckets to bucket base + earlier tiles' count
+// + position in the tile's run. Per-wave bucket counts are 16-bit, two to a word.
+template <bool kIota, int kKpt>
+__global__ __launch_bounds__(kBkBlock) void bk_partition_kernel(const uint32_t* __restrict__ keys_in,
+                                                                const uint32_t* __restrict__ vals_in,
+                                                                uint32_t* __restrict__ keys_out,
+                                                                uint32_t* __restrict__ vals_out, uint32_t n,
+                                                                const uint32_t* __restrict__ mm, int groups,
+                                                                const uint32_t* __restrict__ tile_prefix,
+                                                                const uint32_t* __restrict__ totals) {
+  constexpr int kTile = kBkBlock * kKpt;
+  constexpr int kWords = kBkBins / 2;
+  static_assert(kTile < 65536, "16-bit per-wave bucket counts");
+  // Per-wave bucket counts (two to a word, then a word for lanes past the end of the tile) while
+  // ranking; once every key's place in the tile is known, the same LDS stages the tile.
+  constexpr int kCountWords = kBkWaves * (kWords + 1);
+  __shared__ uint32_t s_mem[kTile > kCountWords ? kTile : kCountWords];
+  uint32_t(*s_wave)[kWords + 1] = reinterpret_cast<uint32_t(*)[kWords + 1]>(s_mem);
+  uint32_t* s_buf = s_mem;
+  // Peer words while ranking (by a bucket's low 8 bits; word 256 for lanes past the end);
+  // afterwards (all zero again) the tile's global bucket offsets.
+  __shared__ uint64_t s_peer[kBkWaves][kRadix + 1];
+  __shared__ uint32_t s_red[2 * kBkWaves];
+  static_assert(sizeof(uint64_t) * kBkWaves * kRadix >= sizeof(uint32_t) * kBkBins, "offsets fit the peer words");
+  uint32_t* s_global = reinterpret_cast<uint32_t*>(&s_peer[0][0]);
+
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint32_t tiles = (n + kTile - 1) / kTile;
+  const uint32_t tile = bk_tile_of_block(tiles);
+  for (int i = t; i < kBkWaves * (kWords + 1); i += kBkBlock) (&s_wave[0][0])[i] = 0;
+  for (int i = t; i < kBkWaves * (kRadix + 1); i += kBkBlock) (&s_peer[0][0])[i] = 0;
+  const BkRange r = bk_key_range(mm, groups, s_red);  // synchronises the workgroup
+  const uint32_t tile_base = tile * (uint32_t)kTile;
+  const uint32_t valid = min((uint32_t)kTile, n - tile_base);
+  const uint32_t wbase = (uint32_t)w * 64u * kKpt;
+
+  uint32_t key[kKpt], rank[kKpt];
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    key[i] = p < valid ? keys_in[tile_base + p] : 0u;
+  }
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t my_bit = 1ull << lane;
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    // lanes past the end of the tile rank under bucket 2048, in peer word 256 and counter word
+    // 1024 of their own: no per-lane branches
+    const bool ok = wbase + i * 64 + lane < valid;
+    const uint32_t d = ok ? (key[i] - r.kmin) >> r.shift : (uint32_t)kBkBins;
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    uint64_t peers;
+    if (__ballot(d == d0) == ~0ull) {
+      peers = ~0ull;
+    } else {
+      uint64_t* word = &s_peer[w][ok ? (d & 255u) : (uint32_t)kRadix];
+      __hip_atomic_fetch_or(word, my_bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      peers = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      __hip_atomic_store(word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      const uint32_t hi = d >> 8;
+      const uint64_t b0 = __ballot(hi & 1u), b1 = __ballot(hi & 2u), b2 = __ballot(hi & 4u);
+      peers &= ((hi & 1u) ? b0 : ~b0) & ((hi & 2u) ? b1 : ~b1) & ((hi & 4u) ? b2 : ~b2);
+    }
+    const int leader = __ffsll((unsigned long long)peers) - 1;  // peers hold the lane itself
+    uint32_t old = 0;
+    if (lane == leader) {
+      const uint32_t sh = 16u * (d & 1u);
+      old = (atomicAdd(&s_wave[w][d >> 1], (uint32_t)__popcll(peers) << sh) >> sh) & 0xFFFFu;
+    }
+    rank[i] = (uint32_t)__shfl((int)old, leader, 64) + (uint32_t)__popcll(peers & lt_mask);
+  }
+  uint32_t val[kKpt];
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    if (kIota)
+      val[i] = tile_base + p;
+    else
+      val[i] = p < valid ? vals_in[tile_base + p] : 0u;
+  }
+  __syncthreads();
+
+  // Thread t owns buckets 4t .. 4t+3 (words 2t and 2t+1 of every wave's row): per-wave prefixes,
+  // the tile's bucket counts, then the bucket starts in the tile and the global run bases.
+  uint32_t run0 = 0, run1 = 0, run2 = 0, run3 = 0;
+#pragma unroll
+  for (int ww = 0; ww < kBkWaves; ++ww) {
+    const uint32_t c0 = s_wave[ww][2 * t], c1 = s_wave[ww][2 * t + 1];
+    s_wave[ww][2 * t] = run0 | (run1 << 16);
+    s_wave[ww][2 * t + 1] = run2 | (run3 << 16);
+    run0 += c0 & 0xFFFFu;
+    run1 += c0 >> 16;
+    run2 += c1 & 0xFFFFu;
+    run3 += c1 >> 16;
+  }
+  const uint4 gt = reinterpret_cast<const uint4*>(totals)[t];
+  const uint4 tp = reinterpret_cast<const uint4*>(tile_prefix + (size_t)tile * kBkBins)[t];
+  const uint32_t gpre = block_exclusive_scan<kBkBlock>(gt.x + gt.y + gt.z + gt.w, s_red);
+  const uint32_t tpre = block_exclusive_scan<kBkBlock>(run0 + run1 + run2 + run3, s_red);
+  const uint32_t st0 = tpre, st1 = st0 + run0, st2 = st1 + run1, st3 = st2 + run2;
+  const uint32_t gb0 = gpre, gb1 = gb0 + gt.x, gb2 = gb1 + gt.y, gb3 = gb2 + gt.z;
+  // wraps; a bucket's start in the tile is at most the position of its keys
+  s_global[4 * t + 0] = gb0 + tp.x - st0;
+  s_global[4 * t + 1] = gb1 + tp.y - st1;
+  s_global[4 * t + 2] = gb2 + tp.z - st2;
+  s_global[4 * t + 3] = gb3 + tp.w - st3;
+#pragma unroll
+  for (int ww = 0; ww < kBkWaves; ++ww) {
+    s_wave[ww][2 * t] += st0 | (st1 << 16);
+    s_wave[ww][2 * t + 1] += st2 | (st3 << 16);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t d = (key[i] - r.kmin) >> r.shift;
+    if (wbase + i * 64 + lane < valid) rank[i] += (s_wave[w][d >> 1] >> (16u * (d & 1u))) & 0xFFFFu;
+  }
+  __syncthreads();  // the counts are dead: their LDS now stages the tile
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i)
+    if (wbase + i * 64 + lane < valid) s_buf[rank[i]] = key[i];
+  __syncthreads();
+  uint32_t dest[kKpt];
+#pragma unroll
+  for (int j = 0; j < kKpt; ++j) {
+    const uint32_t p = j * kBkBlock + t;
+    if (p < valid) {
+      const uint32_t k = s_buf[p];
+      dest[j] = s_global[(k - r.kmin) >> r.shift] + p;
+      keys_out[dest[j]] = k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    if (p < valid) s_buf[rank[i]] = val[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kKpt; ++j) {
+    const uint32_t p = j * kBkBlock + t;
+    if (p < valid) vals_out[dest[j]] = s_buf[p];
+  }
+}
+
+// Digit of a key inside one bucket: the relative key r = (key - kmin) mod 2^shift, less the
+// bucket's smallest r, shifted down to the pass's bits.
+struct BkDigit {
+  uint32_t kmin, rmask, lo, sh, dmask;
+  __device__ uint32_t operator()(uint32_t k) const { return ((((k - kmin) & rmask) - lo) >> sh) & dmask; }
+};
+
+// Lanes past the end of a chunk rank under this digit, with a peer word and a counter of their
+// own, so that the ranking loop needs no per-lane branches; its counts are never read.
+constexpr int kBkSink = kRadix;
+
+// LDS of the local sort kernels.
+struct BkLocalShared {
+  uint2 buf[kBkCap];  // (key, value) pairs staged in digit order
+  uint64_t peer[kBkLocalWaves][kRadix + 1];
+  uint32_t wave[kBkLocalWaves][kRadix + 1];
+  uint32_t start[kRadix];
+  uint32_t run[kRadix];
+  uint32_t red[2 * kBkLocalWaves];
+  uint32_t bucket[2];
+};
+
+// Stable 8-bit ranking of the keys key[0 .. ke) of every lane (positions wbase + i * 64 + lane of
+// the chunk, those from `valid` on in the sink) inside each wave, by the peer-word scheme of the
+// pass kernels; then per-digit wave prefixes and the chunk's digit starts, so that a key's place in
+// the chunk in digit order is sh.wave[w][digit] + rank. Thread t < 256 receives digit t's count in
+// the chunk as the result and its start in *start. (Issuing every key's peer-word operations
+// before any counter add, one wait per phase instead of per key, was slower: the kernel is bound
+// by LDS throughput, not by the wait.)
+template <int kKpt>
+__device__ inline uint32_t bk_rank8(const uint32_t (&key)[kKpt], const BkDigit& dg, uint32_t (&rank)[kKpt],
+                                    uint32_t ke, uint32_t wbase, uint32_t valid, BkLocalShared& sh,
+                                    uint32_t* start) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t my_bit = 1ull << lane;
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    uint32_t rk = 0;  // assigned to rank[i] once (see the partition kernel)
+    if ((uint32_t)i < ke) {
+      const uint32_t d = wbase + i * 64 + lane < valid ? dg(key[i]) : (uint32_t)kBkSink;
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+      uint64_t peers;
+      if (__ballot(d == d0) == ~0ull) {
+        peers = ~0ull;
+      } else {
+        uint64_t* word = &sh.peer[w][d];
+        __hip_atomic_fetch_or(word, my_bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        peers = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __hip_atomic_store(word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      }
+      const int leader = __ffsll((unsigned long long)peers) - 1;  // peers hold the lane itself
+      uint32_t old = 0;
+      if (lane == leader) old = atomicAdd(&sh.wave[w][d], (uint32_t)__popcll(peers));
+      rk = (uint32_t)__shfl((int)old, leader, 64) + (uint32_t)__popcll(peers & lt_mask);
+    }
+    rank[i] = rk;
+  }
+  __syncthreads();
+  uint32_t total = 0;
+  if (t < kRadix) {
+#pragma unroll
+    for (int ww = 0; ww < kBkLocalWaves; ++ww) {
+      const uint32_t c = sh.wave[ww][t];
+      sh.wave[ww][t] = total;
+      total += c;
+    }
+  }
+  const uint32_t st = block_exclusive_scan<kBkLocalBlock>(total, sh.red);
+  if (t < kRadix) {
+#pragma unroll
+    for (int ww = 0; ww < kBkLocalWaves; ++ww) sh.wave[ww][t] += st;
+  }
+  *start = st;
+  __syncthreads();
+  return total;
+}
+
+// Clears the per-wave digit counts (sink included) for the next ranking.
+__device__ inline void bk_clear_counts(BkLocalShared& sh) {
+  for (int i = threadIdx.x; i < kBkLocalWaves * (kRadix + 1); i += kBkLocalBlock) (&sh.wave[0][0])[i] = 0;
+}
+
+// Bucket b's place in the partition output (thread b / 4 writes it) and the key range; every
+// thread of the workgroup calls it.
+__device__ inline BkRange bk_bucket(uint32_t b, const uint32_t* __restrict__ totals, const uint32_t* __restrict__ mm,
+                                    int groups, BkLocalShared& sh) {
+  const int t = threadIdx.x;
+  const uint4 gt = t < kBkBins / 4 ? reinterpret_cast<const uint4*>(totals)[t] : make_uint4(0u, 0u, 0u, 0u);
+  const uint32_t pre = block_exclusive_scan<kBkLocalBlock>(gt.x + gt.y + gt.z + gt.w, sh.red);
+  if ((uint32_t)t == b / 4u) {
+    const uint32_t c[4] = {gt.x, gt.y, gt.z, gt.w};
+    uint32_t s = pre;
+    for (uint32_t k = 0; k < b % 4u; ++k) s += c[k];
+    sh.bucket[0] = s;
+    sh.bucket[1] = s + c[b % 4u];
+  }
+  return bk_key_range<kBkLocalBlock>(mm, groups, sh.red);  // synchronises the workgroup
+}
+
+// Smallest and largest of a (local sort) workgroup's lo / hi.
+__device__ inline void bk_block_minmax(uint32_t& lo, uint32_t& hi, uint32_t* s_red) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, off, 64));
+  }
+  if (lane == 0) {
+    s_red[w] = lo;
+    s_red[kBkLocalWaves + w] = hi;
+  }
+  __syncthreads();
+  lo = s_red[0];
+  hi = s_red[kBkLocalWaves];
+#pragma unroll
+  for (int i = 1; i < kBkLocalWaves; ++i) {
+    lo = min(lo, s_red[i]);
+    hi = max(hi, s_red[kBkLocalWaves + i]);
+  }
+  __syncthreads();
+}
+
+// Pass count and digit function of a bucket whose relative keys span [lo, hi]: the fewest LSD
+// passes of at most 8 bits, all of one width.
+__device__ inline BkDigit bk_digit_plan(const BkRange& r, uint32_t rmask, uint32_t lo, uint32_t hi, uint32_t* passes,
+                                        uint32_t* width) {
+  const uint32_t span = hi - lo;
+  const uint32_t bits = span ? 32u - (uint32_t)__clz((int)span) : 0u;
+  *passes = (bits + 7u) / 8u;
+  *width = *passes ? (bits + *passes - 1u) / *passes : 0u;
+  return BkDigit{r.kmin, rmask, lo, 0u, (1u << *width) - 1u};
+}
+
+// The buckets beyond the LDS, listed by the local kernel (big[0] buckets in big[1 ..]): each
+// sorted by one workgroup through the bucket's range of both buffer pairs, by LSD passes, each a
+// histogram of the whole bucket, then chunks of kBkCap pairs in order, each ranked in LDS and
+// written at the running digit offsets. An odd pass count ends in (keys_out, vals_out) by itself;
+// an even one first copies the bucket there and runs its passes back from it. Correct for any key
+// distribution; slow only when the keys crowd into few buckets.
+__global__ __launch_bounds__(kBkLocalBlock) void bk_local_big_kernel(uint32_t* keys_in, uint32_t* vals_in,
+                                                                uint32_t* keys_out, uint32_t* vals_out,
+                                                                const uint32_t* __restrict__ mm, int groups,
+                                                                const uint32_t* __restrict__ totals,
+                                                                const uint32_t* __restrict__ big) {
+  constexpr int kKpt = kBkLocalKpt;
+  __shared__ BkLocalShared sh;
+  if (blockIdx.x >= big[0]) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint32_t wbase = (uint32_t)w * 64u * kKpt;
+  for (int i = t; i < kBkLocalWaves * (kRadix + 1); i += kBkLocalBlock) (&sh.peer[0][0])[i] = 0;
+  bk_clear_counts(sh);
+  const BkRange r = bk_bucket(big[1 + blockIdx.x], totals, mm, groups, sh);
+  const uint32_t bs = sh.bucket[0], m = sh.bucket[1] - sh.bucket[0];
+  const uint32_t rmask = r.shift ? (0xFFFFFFFFu >> (32u - r.shift)) : 0u;
+  uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+  for (uint32_t p = t; p < m; p += kBkLocalBlock) {
+    const uint32_t rk = (keys_in[bs + p] - r.kmin) & rmask;
+    lo = min(lo, rk);
+    hi = max(hi, rk);
+  }
+  bk_block_minmax(lo, hi, sh.red);
+  uint32_t passes, width;
+  BkDigit dg = bk_digit_plan(r, rmask, lo, hi, &passes, &width);
+  uint32_t* src_k = keys_in;
+  uint32_t* src_v = vals_in;
+  uint32_t* dst_k = keys_out;
+  uint32_t* dst_v = vals_out;
+  if ((passes & 1u) == 0) {
+    for (uint32_t p = t; p < m; p += kBkLocalBlock) {
+      keys_out[bs + p] = keys_in[bs + p];
+      vals_out[bs + p] = vals_in[bs + p];
+    }
+    if (passes == 0) return;
+    __threadfence_block();
+    __syncthreads();
+    src_k = keys_out;
+    src_v = vals_out;
+    dst_k = keys_in;
+    dst_v = vals_in;
+  }
+  uint32_t key[kKpt], rank[kKpt];
+  for (uint32_t q = 0; q < passes; ++q) {
+    dg.sh = q * width;
+    if (t < kRadix) sh.run[t] = 0;
+    __syncthreads();
+    for (uint32_t p0 = 0; p0 < m; p0 += kBkLocalBlock) {
+      const uint32_t p = p0 + t;
+      const bool ok = p < m;
+      hist_add(sh.run, dg(ok ? src_k[bs + p] : 0u), ok);
+    }
+    __syncthreads();
+    {
+      const uint32_t c = t < kRadix ? sh.run[t] : 0u;
+      const uint32_t pre = block_exclusive_scan<kBkLocalBlock>(c, sh.red);
+      if (t < kRadix) sh.run[t] = pre;
+    }
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < m; c0 += kBkCap) {
+      const uint32_t valid = min((uint32_t)kBkCap, m - c0);
+#pragma unroll
+      for (int i = 0; i < kKpt; ++i) {
+        const uint32_t p = wbase + i * 64 + lane;
+        key[i] = p < valid ? src_k[bs + c0 + p] : 0u;
+      }
+      uint32_t st;
+      const uint32_t total = bk_rank8<kKpt>(key, dg, rank, kKpt, wbase, valid, sh, &st);
+      if (t < kRadix) sh.start[t] = st;
+#pragma unroll
+      for (int i = 0; i < kKpt; ++i) {
+        const uint32_t p = wbase + i * 64 + lane;
+        if (p < valid) sh.buf[rank[i] + sh.wave[w][dg(key[i])]] = make_uint2(key[i], src_v[bs + c0 + p]);
+      }
+      __syncthreads();
+      bk_clear_counts(sh);
+#pragma unroll
+      for (int j = 0; j < kKpt; ++j) {
+        const uint32_t p = j * kBkLocalBlock + t;
+        if (p < valid) {
+          const uint2 kv = sh.buf[p];
+          const uint32_t d = dg(kv.x);
+          const uint32_t dest = bs + sh.run[d] + p - sh.start[d];
+          dst_k[dest] = kv.x;
+          dst_v[dest] = kv.y;
+        }
+      }
+      __syncthreads();
+      if (t < kRadix) sh.run[t] += total;
+      __syncthreads();
+    }
+    // this pass's output is the next pass's input, read by other lanes of the workgroup
+    __threadfence_block();
+    __syncthreads();
+    uint32_t* tk = src_k;
+    uint32_t* tv = src_v;
+    src_k = dst_k;
+    src_v = dst_v;
+    dst_k = tk;
+    dst_v = tv;
+  }
+}
+
+// Every bucket of the partition sorted on the bits below its digit, stably, from (keys_in,
+// vals_in) into (keys_out, vals_out) at the same positions. Workgroup = bucket; its pairs are
+// spread evenly over the waves (ceil(m / 1024) per lane, in blocks per wave). Inside the bucket
+// only the relative key r = (key - kmin) mod 2^shift - min r matters; its ceil(bits / 8) LSD passes
+// of equal width run in LDS. Buckets of more than kBkCap pairs are appended to big (count in
+// big[0], cleared by the range kernel) for bk_local_big_kernel.
+__global__ __launch_bounds__(kBkLocalBlock) void bk_local_kernel(const uint32_t* __restrict__ keys_in,
+                                                            const uint32_t* __restrict__ vals_in,
+                                                            uint32_t* __restrict__ keys_out,
+                                                            uint32_t* __restrict__ vals_out,
+                                                            const uint32_t* __restrict__ mm, int groups,
+                                                            const uint32_t* __restrict__ totals,
+                                                            uint32_t* __restrict__ big) {
+  constexpr int kKpt = kBkLocalKpt;
+  __shared__ BkLocalShared sh;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (int i = t; i < kBkLocalWaves * (kRadix + 1); i += kBkLocalBlock) (&sh.peer[0][0])[i] = 0;
+  bk_clear_counts(sh);
+  const BkRange r = bk_bucket(blockIdx.x, totals, mm, groups, sh);
+  const uint32_t bs = sh.bucket[0], m = sh.bucket[1] - sh.bucket[0];
+  if (m == 0) return;
+  if (m > (uint32_t)kBkCap) {
+    if (t == 0) big[1 + atomicAdd(big, 1u)] = blockIdx.x;
+    return;
+  }
+  const uint32_t ke = (m + kBkLocalBlock - 1) / kBkLocalBlock;  // keys per lane
+  const uint32_t wbase = (uint32_t)w * 64u * ke;
+  const uint32_t rmask = r.shift ? (0xFFFFFFFFu >> (32u - r.shift)) : 0u;
+  uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+  uint32_t key[kKpt], val[kKpt];
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    const bool ok = (uint32_t)i < ke && p < m;
+    key[i] = ok ? keys_in[bs + p] : 0u;
+    val[i] = ok ? vals_in[bs + p] : 0u;
+    if (ok) {
+      const uint32_t rk = (key[i] - r.kmin) & rmask;
+      lo = min(lo, rk);
+      hi = max(hi, rk);
+    }
+  }
+  bk_block_minmax(lo, hi, sh.red);
+  uint32_t passes, width;
+  BkDigit dg = bk_digit_plan(r, rmask, lo, hi, &passes, &width);
+
+  uint32_t rank[kKpt];
+  for (uint32_t q = 0; q < passes; ++q) {
+    dg.sh = q * width;
+    uint32_t st;
+    bk_rank8<kKpt>(key, dg, rank, ke, wbase, m, sh, &st);
+#pragma unroll
+    for (int i = 0; i < kKpt; ++i) {
+      if ((uint32_t)i < ke && wbase + i * 64 + lane < m)
+        sh.buf[rank[i] + sh.wave[w][dg(key[i])]] = make_uint2(key[i], val[i]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kKpt; ++i) {
+      const uint32_t p = wbase + i * 64 + lane;
+      if ((uint32_t)i < ke && p < m) {
+        const uint2 kv = sh.buf[p];
+        key[i] = kv.x;
+        val[i] = kv.y;
+      }
+    }
+    bk_clear_counts(sh);  // every wave is past reading them
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < kKpt; ++i) {
+    const uint32_t p = wbase + i * 64 + lane;
+    if ((uint32_t)i < ke && p < m) {
+      keys_out[bs + p] = key[i];
+      vals_out[bs + p] = val[i];
+    }
+  }
+}
+
 }  // namespace
 
 size_t sort_status_bytes(size_t n) {
@@ -412,13 +1055,14 @@ void sort_reserve(swr_context* ctx, size_t n) {
     SWR_HIP(hipStreamSynchronize(ctx->stream));
   }
   // Per-tile digit counts of the scanned passes: 256 rows of the tile count at the smallest tile,
-  // rounded up to whole upsweep groups.
+  // rounded up to whole upsweep groups; the bucket sort's 2048 counts per 16 K-key tile share the
+  // buffer.
   size_t tiles = (n + (size_t)kSortBlock * kSortMinKeysPerThread - 1) / ((size_t)kSortBlock * kSortMinKeysPerThread);
   tiles = (tiles + kUpTiles - 1) / kUpTiles * kUpTiles;
-  if (tiles > ws.tile_stride) {
-    ensure(ctx, ws.tile_counts, (size_t)kRadix * tiles * sizeof(uint32_t));
-    ws.tile_stride = (uint32_t)tiles;
-  }
+  const size_t stride = std::max<size_t>(tiles, ws.tile_stride);
+  const size_t words = std::max((size_t)kRadix * stride, (size_t)kBkBins * ((n + kBkTile - 1) / kBkTile));
+  if (ws.tile_counts.bytes < words * sizeof(uint32_t)) ensure(ctx, ws.tile_counts, words * sizeof(uint32_t));
+  ws.tile_stride = (uint32_t)stride;
 }
 
 void sort_reset_epochs(swr_context* ctx, hipStream_t stream) {
@@ -442,6 +1086,7 @@ struct PassBuffers {
   uint64_t* status;       // onesweep look-back words
   uint32_t* tile_counts;  // reduce-then-scan per-tile digit counts, 256 rows of tile_stride
   uint32_t tile_stride;
+  size_t tile_counts_bytes;
 };
 
 // Launches the passes of one sort (histogram first for onesweep); shapes checked on the host.
@@ -506,6 +1151,39 @@ void launch_passes(const PassBuffers& b, bool scanned, int kpt, int num_cus, uin
   SWR_HIP(hipGetLastError());
 }
 
+// hist holds the bucket totals, the range kernel's (min, max) pairs and the oversize list.
+static_assert(kBkBins + 2 * kBkRangeGroups + 1 + kBkBins <= kHistCopies * 4 * kRadix, "bucket sort words fit hist");
+
+// The bucket sort of n pairs from (keys, vals) back into them, through (alt_keys, alt_vals).
+void launch_bucket_sort(const PassBuffers& b, uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals,
+                        size_t n, bool iota, hipStream_t stream) {
+  uint32_t* totals = b.hist;                                // 2048 bucket totals
+  uint32_t* mm = b.hist + kBkBins;                          // per-group (min, max) of the keys
+  uint32_t* big = b.hist + kBkBins + 2 * kBkRangeGroups;    // count, then the buckets beyond the LDS
+  const uint32_t groups = checked_grid(
+      std::min<size_t>(kBkRangeGroups, (n + 16 * kBkRangeBlock - 1) / (16 * kBkRangeBlock)), kBkRangeBlock,
+      "bucket sort range");
+  const uint32_t tiles = checked_grid((n + kBkTile - 1) / kBkTile, kBkBlock, "bucket sort partition");
+  if ((size_t)tiles * kBkBins * sizeof(uint32_t) > b.tile_counts_bytes)
+    fail(SWR_E_INVALID, "bucket sort: per-tile count buffer smaller than the tile count");
+  bk_range_kernel<<<groups, kBkRangeBlock, 0, stream>>>(keys, (uint32_t)n, mm, big);
+  bk_upsweep_kernel<kBkKpt><<<tiles, kBkBlock, 0, stream>>>(keys, (uint32_t)n, mm, (int)groups, b.tile_counts);
+  bk_scan_kernel<<<kBkBins / kBkScanCols, kBkScanBlock, 0, stream>>>(b.tile_counts, tiles, totals);
+  if (iota)
+    bk_partition_kernel<true, kBkKpt><<<tiles, kBkBlock, 0, stream>>>(keys, vals, alt_keys, alt_vals, (uint32_t)n, mm,
+                                                                      (int)groups, b.tile_counts, totals);
+  else
+    bk_partition_kernel<false, kBkKpt><<<tiles, kBkBlock, 0, stream>>>(keys, vals, alt_keys, alt_vals, (uint32_t)n, mm,
+                                                                       (int)groups, b.tile_counts, totals);
+  bk_local_kernel<<<kBkBins, kBkLocalBlock, 0, stream>>>(alt_keys, alt_vals, keys, vals, mm, (int)groups, totals,
+                                                         big);
+  // at most n / (kBkCap + 1) buckets exceed the LDS
+  const uint32_t big_groups = (uint32_t)std::min<size_t>(kBkBins, n / (kBkCap + 1) + 1);
+  bk_local_big_kernel<<<big_groups, kBkLocalBlock, 0, stream>>>(alt_keys, alt_vals, keys, vals, mm, (int)groups,
+                                                                totals, big);
+  SWR_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals, size_t n,
@@ -527,12 +1205,20 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
     sort_reserve(ctx, n);
     sort_epoch_guard(ctx, (uint64_t)passes, stream);
   }
-  ws.epoch += (uint64_t)passes;  // host mirror of the device epoch counter
   const PassBuffers b{static_cast<uint32_t*>(ws.hist.ptr), static_cast<uint32_t*>(ws.counters.ptr),
                       static_cast<uint64_t*>(ws.status.ptr), static_cast<uint32_t*>(ws.tile_counts.ptr),
-                      ws.tile_stride};
-  launch_passes(b, ctx->tuning.sort_algorithm == SWR_SORT_REDUCE_SCAN, ctx->tuning.sort_kpt, ctx->num_cus, keys,
-                vals, alt_keys, alt_vals, n, passes, iota, stream, n_dev);
+                      ws.tile_stride, ws.tile_counts.bytes};
+  // The bucket sort needs the count at launch and pays off from three 8-bit passes up; otherwise
+  // its tuning runs the reduce-then-scan passes.
+  if (ctx->tuning.sort_algorithm == SWR_SORT_BUCKET && !n_dev && passes >= 3) {
+    launch_bucket_sort(b, keys, vals, alt_keys, alt_vals, n, iota, stream);
+    *out_keys = keys;
+    *out_vals = vals;
+    return;
+  }
+  ws.epoch += (uint64_t)passes;  // host mirror of the device epoch counter
+  launch_passes(b, ctx->tuning.sort_algorithm != SWR_SORT_ONESWEEP, ctx->tuning.sort_kpt, ctx->num_cus, keys, vals,
+                alt_keys, alt_vals, n, passes, iota, stream, n_dev);
 }
 
 // ---- the sort without allocation ----
@@ -573,7 +1259,8 @@ void sort_pairs_noalloc(swr_context* ctx, uint32_t* keys, uint32_t* vals, size_t
   uint32_t* hist = tile_counts + kRadix * kScratchTiles;
   const int passes = (key_bits + 7) / 8;
   // Reduce-then-scan only: its passes use no look-back words, tickets or epochs.
-  const PassBuffers b{hist, nullptr, nullptr, tile_counts, (uint32_t)kScratchTiles};
+  const PassBuffers b{hist, nullptr, nullptr, tile_counts, (uint32_t)kScratchTiles,
+                      sizeof(uint32_t) * kRadix * kScratchTiles};
   launch_passes(b, true, ctx->tuning.sort_kpt, ctx->num_cus, keys, vals, alt_keys, alt_vals, n, passes, iota, stream,
                 nullptr);
   if (passes & 1) {
