@@ -763,10 +763,14 @@ __device__ inline uint32_t bk_rank8(const uint32_t (&key)[kKpt], const BkDigit& 
         peers = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         __hip_atomic_store(word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       }
-      const int leader = __ffsll((unsigned long long)peers) - 1;  // peers hold the lane itself
-      uint32_t old = 0;
-      if (lane == leader) old = atomicAdd(&sh.wave[w][d], (uint32_t)__popcll(peers));
-      rk = (uint32_t)__shfl((int)old, leader, 64) + (uint32_t)__popcll(peers & lt_mask);
+      // Every lane reads its digit's running count (before this key's group adds to it); the
+      // group's lowest lane alone writes the new count. A wave's LDS operations execute in order,
+      // so the next key reads what this one wrote.
+      uint32_t* cnt = &sh.wave[w][d];
+      const uint32_t before = __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if (lane == __ffsll((unsigned long long)peers) - 1)
+        __hip_atomic_store(cnt, before + (uint32_t)__popcll(peers), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      rk = before + (uint32_t)__popcll(peers & lt_mask);
     }
     rank[i] = rk;
   }
@@ -955,11 +959,15 @@ __global__ __launch_bounds__(kBkLocalBlock) void bk_local_big_kernel(uint32_t* k
 }
 
 // Every bucket of the partition sorted on the bits below its digit, stably, from (keys_in,
-// vals_in) into (keys_out, vals_out) at the same positions. Workgroup = bucket; its pairs are
-// spread evenly over the waves (ceil(m / 1024) per lane, in blocks per wave). Inside the bucket
-// only the relative key r = (key - kmin) mod 2^shift - min r matters; its ceil(bits / 8) LSD passes
-// of equal width run in LDS. Buckets of more than kBkCap pairs are appended to big (count in
-// big[0], cleared by the range kernel) for bk_local_big_kernel.
+// vals_in) into (keys_out, vals_out) at the same positions. A persistent workgroup per CU takes
+// buckets g, g + G, g + 2G, ... (G workgroups) and loads its next bucket's pairs into registers
+// while it sorts the current one: with one 150 KB workgroup per CU, a bucket's loads would
+// otherwise wait behind the previous bucket's sort (the load and store of every bucket alone took
+// 80 us for 2^24 pairs, twice the copy time). A bucket's pairs are spread evenly over the waves
+// (ceil(m / 1024) per lane, in blocks per wave). Inside the bucket only the relative key
+// r = (key - kmin) mod 2^shift - min r matters; its ceil(bits / 8) LSD passes of equal width run
+// in LDS. Buckets of more than kBkCap pairs are appended to big (count in big[0], cleared by the
+// range kernel) for bk_local_big_kernel.
 __global__ __launch_bounds__(kBkLocalBlock) void bk_local_kernel(const uint32_t* __restrict__ keys_in,
                                                             const uint32_t* __restrict__ vals_in,
                                                             uint32_t* __restrict__ keys_out,
@@ -969,66 +977,97 @@ __global__ __launch_bounds__(kBkLocalBlock) void bk_local_kernel(const uint32_t*
                                                             uint32_t* __restrict__ big) {
   constexpr int kKpt = kBkLocalKpt;
   __shared__ BkLocalShared sh;
+  __shared__ uint32_t s_first[kBkBins + 1];  // bucket starts in the partition output
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int i = t; i < kBkLocalWaves * (kRadix + 1); i += kBkLocalBlock) (&sh.peer[0][0])[i] = 0;
   bk_clear_counts(sh);
-  const BkRange r = bk_bucket(blockIdx.x, totals, mm, groups, sh);
-  const uint32_t bs = sh.bucket[0], m = sh.bucket[1] - sh.bucket[0];
-  if (m == 0) return;
-  if (m > (uint32_t)kBkCap) {
-    if (t == 0) big[1 + atomicAdd(big, 1u)] = blockIdx.x;
-    return;
+  {
+    const uint4 gt = t < kBkBins / 4 ? reinterpret_cast<const uint4*>(totals)[t] : make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t pre = block_exclusive_scan<kBkLocalBlock>(gt.x + gt.y + gt.z + gt.w, sh.red);
+    if (t < kBkBins / 4) {
+      s_first[4 * t] = pre;
+      s_first[4 * t + 1] = pre + gt.x;
+      s_first[4 * t + 2] = pre + gt.x + gt.y;
+      s_first[4 * t + 3] = pre + gt.x + gt.y + gt.z;
+      if (t == kBkBins / 4 - 1) s_first[kBkBins] = pre + gt.x + gt.y + gt.z + gt.w;
+    }
   }
-  const uint32_t ke = (m + kBkLocalBlock - 1) / kBkLocalBlock;  // keys per lane
-  const uint32_t wbase = (uint32_t)w * 64u * ke;
+  const BkRange r = bk_key_range<kBkLocalBlock>(mm, groups, sh.red);  // synchronises the workgroup
   const uint32_t rmask = r.shift ? (0xFFFFFFFFu >> (32u - r.shift)) : 0u;
-  uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-  uint32_t key[kKpt], val[kKpt];
-#pragma unroll
-  for (int i = 0; i < kKpt; ++i) {
-    const uint32_t p = wbase + i * 64 + lane;
-    const bool ok = (uint32_t)i < ke && p < m;
-    key[i] = ok ? keys_in[bs + p] : 0u;
-    val[i] = ok ? vals_in[bs + p] : 0u;
-    if (ok) {
-      const uint32_t rk = (key[i] - r.kmin) & rmask;
-      lo = min(lo, rk);
-      hi = max(hi, rk);
-    }
-  }
-  bk_block_minmax(lo, hi, sh.red);
-  uint32_t passes, width;
-  BkDigit dg = bk_digit_plan(r, rmask, lo, hi, &passes, &width);
 
-  uint32_t rank[kKpt];
-  for (uint32_t q = 0; q < passes; ++q) {
-    dg.sh = q * width;
-    uint32_t st;
-    bk_rank8<kKpt>(key, dg, rank, ke, wbase, m, sh, &st);
-#pragma unroll
-    for (int i = 0; i < kKpt; ++i) {
-      if ((uint32_t)i < ke && wbase + i * 64 + lane < m)
-        sh.buf[rank[i] + sh.wave[w][dg(key[i])]] = make_uint2(key[i], val[i]);
-    }
-    __syncthreads();
+  uint32_t key[kKpt], val[kKpt], nkey[kKpt], nval[kKpt];
+  // Loads bucket bb's pairs (if it fits the LDS) into (k, v): ceil(m / 1024) per lane.
+  auto load = [&](uint32_t bb, uint32_t (&k)[kKpt], uint32_t (&v)[kKpt]) {
+    const uint32_t bs = s_first[bb], m = s_first[bb + 1] - bs;
+    const uint32_t ke = m <= (uint32_t)kBkCap ? (m + kBkLocalBlock - 1) / kBkLocalBlock : 0u;
+    const uint32_t wbase = (uint32_t)w * 64u * ke;
 #pragma unroll
     for (int i = 0; i < kKpt; ++i) {
       const uint32_t p = wbase + i * 64 + lane;
-      if ((uint32_t)i < ke && p < m) {
-        const uint2 kv = sh.buf[p];
-        key[i] = kv.x;
-        val[i] = kv.y;
+      const bool ok = (uint32_t)i < ke && p < m;
+      k[i] = ok ? keys_in[bs + p] : 0u;
+      v[i] = ok ? vals_in[bs + p] : 0u;
+    }
+  };
+  uint32_t b = blockIdx.x;
+  if (b < (uint32_t)kBkBins) load(b, key, val);
+  for (; b < (uint32_t)kBkBins; b += gridDim.x) {
+    const uint32_t bs = s_first[b], m = s_first[b + 1] - bs;
+    // the next bucket's pairs, in flight while this one sorts
+    if (b + gridDim.x < (uint32_t)kBkBins) load(b + gridDim.x, nkey, nval);
+    if (m > (uint32_t)kBkCap) {
+      if (t == 0) big[1 + atomicAdd(big, 1u)] = b;
+    } else if (m > 0) {
+      const uint32_t ke = (m + kBkLocalBlock - 1) / kBkLocalBlock;  // keys per lane
+      const uint32_t wbase = (uint32_t)w * 64u * ke;
+      uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+#pragma unroll
+      for (int i = 0; i < kKpt; ++i) {
+        if ((uint32_t)i < ke && wbase + i * 64 + lane < m) {
+          const uint32_t rk = (key[i] - r.kmin) & rmask;
+          lo = min(lo, rk);
+          hi = max(hi, rk);
+        }
+      }
+      bk_block_minmax(lo, hi, sh.red);
+      uint32_t passes, width;
+      BkDigit dg = bk_digit_plan(r, rmask, lo, hi, &passes, &width);
+      uint32_t rank[kKpt];
+      for (uint32_t q = 0; q < passes; ++q) {
+        dg.sh = q * width;
+        uint32_t st;
+        bk_rank8<kKpt>(key, dg, rank, ke, wbase, m, sh, &st);
+#pragma unroll
+        for (int i = 0; i < kKpt; ++i) {
+          if ((uint32_t)i < ke && wbase + i * 64 + lane < m)
+            sh.buf[rank[i] + sh.wave[w][dg(key[i])]] = make_uint2(key[i], val[i]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kKpt; ++i) {
+          const uint32_t p = wbase + i * 64 + lane;
+          if ((uint32_t)i < ke && p < m) {
+            const uint2 kv = sh.buf[p];
+            key[i] = kv.x;
+            val[i] = kv.y;
+          }
+        }
+        bk_clear_counts(sh);  // every wave is past reading them
+        __syncthreads();
+      }
+#pragma unroll
+      for (int i = 0; i < kKpt; ++i) {
+        const uint32_t p = wbase + i * 64 + lane;
+        if ((uint32_t)i < ke && p < m) {
+          keys_out[bs + p] = key[i];
+          vals_out[bs + p] = val[i];
+        }
       }
     }
-    bk_clear_counts(sh);  // every wave is past reading them
-    __syncthreads();
-  }
 #pragma unroll
-  for (int i = 0; i < kKpt; ++i) {
-    const uint32_t p = wbase + i * 64 + lane;
-    if ((uint32_t)i < ke && p < m) {
-      keys_out[bs + p] = key[i];
-      vals_out[bs + p] = val[i];
+    for (int i = 0; i < kKpt; ++i) {
+      key[i] = nkey[i];
+      val[i] = nval[i];
     }
   }
 }
@@ -1156,7 +1195,7 @@ static_assert(kBkBins + 2 * kBkRangeGroups + 1 + kBkBins <= kHistCopies * 4 * kR
 
 // The bucket sort of n pairs from (keys, vals) back into them, through (alt_keys, alt_vals).
 void launch_bucket_sort(const PassBuffers& b, uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals,
-                        size_t n, bool iota, hipStream_t stream) {
+                        size_t n, bool iota, int num_cus, hipStream_t stream) {
   uint32_t* totals = b.hist;                                // 2048 bucket totals
   uint32_t* mm = b.hist + kBkBins;                          // per-group (min, max) of the keys
   uint32_t* big = b.hist + kBkBins + 2 * kBkRangeGroups;    // count, then the buckets beyond the LDS
@@ -1175,8 +1214,10 @@ void launch_bucket_sort(const PassBuffers& b, uint32_t* keys, uint32_t* vals, ui
   else
     bk_partition_kernel<false, kBkKpt><<<tiles, kBkBlock, 0, stream>>>(keys, vals, alt_keys, alt_vals, (uint32_t)n, mm,
                                                                        (int)groups, b.tile_counts, totals);
-  bk_local_kernel<<<kBkBins, kBkLocalBlock, 0, stream>>>(alt_keys, alt_vals, keys, vals, mm, (int)groups, totals,
-                                                         big);
+  // one persistent local-sort workgroup per CU (its LDS allows no second)
+  const uint32_t local_groups = (uint32_t)std::max(1, std::min(kBkBins, num_cus));
+  bk_local_kernel<<<local_groups, kBkLocalBlock, 0, stream>>>(alt_keys, alt_vals, keys, vals, mm, (int)groups, totals,
+                                                              big);
   // at most n / (kBkCap + 1) buckets exceed the LDS
   const uint32_t big_groups = (uint32_t)std::min<size_t>(kBkBins, n / (kBkCap + 1) + 1);
   bk_local_big_kernel<<<big_groups, kBkLocalBlock, 0, stream>>>(alt_keys, alt_vals, keys, vals, mm, (int)groups,
@@ -1211,7 +1252,7 @@ void sort_pairs(swr_context* ctx, uint32_t* keys, uint32_t* vals, uint32_t* alt_
   // The bucket sort needs the count at launch and pays off from three 8-bit passes up; otherwise
   // its tuning runs the reduce-then-scan passes.
   if (ctx->tuning.sort_algorithm == SWR_SORT_BUCKET && !n_dev && passes >= 3) {
-    launch_bucket_sort(b, keys, vals, alt_keys, alt_vals, n, iota, stream);
+    launch_bucket_sort(b, keys, vals, alt_keys, alt_vals, n, iota, ctx->num_cus, stream);
     *out_keys = keys;
     *out_vals = vals;
     return;
