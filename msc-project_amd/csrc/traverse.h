@@ -193,10 +193,13 @@ __device__ inline void node_entries(const RayState& r, const PlaneRows& pr, cons
 // the child boxes, and the exit distance gets a slack of 2^-21, so no box that the ray meets is
 // culled. Near and far planes are picked per axis by the sign of the direction, as plane_rows
 // does for full nodes.
+// kOct >= 0: every ray of the wave has the direction octant kOct (bit 0: x < 0, bit 1: y < 0,
+// bit 2: z < 0), so the near and far planes are fixed at compile time; -1: chosen per lane.
 constexpr float kBoxSlackQ = 1.00000048f;
 __device__ inline float qplane(uint32_t w, int c, float a, float b) {
   return fmaf((float)((w >> (8 * c)) & 0xFFu), a, b);
 }
+template <int kOct = -1>
 __device__ inline void node_entries_q(const RayState& r, const SWR_GLOBAL char* base, uint32_t node_off, float tfar,
                                       uint4* ch, float* t0, float* t1, float* t2, float* t3) {
   const gf4 h = *(const SWR_GLOBAL gf4*)(base + node_off);
@@ -206,7 +209,9 @@ __device__ inline void node_entries_q(const RayState& r, const SWR_GLOBAL char* 
   *ch = make_uint4(c.x, c.y, c.z, c.w);
   const float ax = __uint_as_float(qb.z) * r.ix, ay = __uint_as_float(qb.w) * r.iy, az = h.w * r.iz;
   const float bx = (h.x - r.ox) * r.ix, by = (h.y - r.oy) * r.iy, bz = (h.z - r.oz) * r.iz;
-  const bool sx = r.ix < 0.f, sy = r.iy < 0.f, sz = r.iz < 0.f;
+  const bool sx = kOct >= 0 ? (kOct & 1) != 0 : r.ix < 0.f;
+  const bool sy = kOct >= 0 ? (kOct & 2) != 0 : r.iy < 0.f;
+  const bool sz = kOct >= 0 ? (kOct & 4) != 0 : r.iz < 0.f;
   const uint32_t nx = sx ? qa.y : qa.x, fx = sx ? qa.x : qa.y;
   const uint32_t ny = sy ? qa.w : qa.z, fy = sy ? qa.z : qa.w;
   const uint32_t nz = sz ? qb.y : qb.x, fz = sz ? qb.x : qb.y;
@@ -339,7 +344,7 @@ __device__ inline bool leaf_tests(const RayState& r, const SWR_GLOBAL gf4* gtris
 // them back by slot: an LDS read per push instead of a chain of selects over the four registers
 // (the traversal is bound by vector-instruction issue; LDS has room).
 template <int kBlock, int kShort, bool kAny, bool kPairTris = false, bool kNearFar = false, bool kQuant = false,
-          bool kChildLds = false>
+          bool kChildLds = false, int kOct = -1>
 __device__ inline HitResult traverse(const void* __restrict__ nodes, const float4* __restrict__ leaf_tris,
                                      const RayState& r, uint32_t mask, uint32_t* lds_col, uint32_t* spill_col,
                                      uint32_t* child_col = nullptr) {
@@ -360,7 +365,7 @@ __device__ inline HitResult traverse(const void* __restrict__ nodes, const float
       uint4 ch;
       float t0, t1, t2, t3;
       if (kQuant)
-        node_entries_q(r, gnodes, node * 64u, best.t, &ch, &t0, &t1, &t2, &t3);
+        node_entries_q<kOct>(r, gnodes, node * 64u, best.t, &ch, &t0, &t1, &t2, &t3);
       else
         node_entries<kNearFar>(r, pr, gnodes, node * 128u, best.t, &ch, &t0, &t1, &t2, &t3);
       if (kAny) {
