@@ -395,7 +395,7 @@ __global__ __launch_bounds__(kSortBlock) void onesweep_kernel(const uint32_t* __
 //   1. the range [kmin, kmax] of the keys (one read of the keys);
 //   2. a stable partition into 2048 buckets by the top 11 bits of key - kmin over that range
 //      (upsweep of per-tile bucket counts, a scan over tiles, then the scatter: the scanned pass
-//      structure above with an 11-bit digit and 16 K-key tiles);
+//      structure above with an 11-bit digit and 8 K-key tiles);
 //   3. each bucket (about n / 2048 pairs on keys spread over their range) loaded into LDS, sorted
 //      stably on its remaining bits by LSD passes of at most 8 bits inside the workgroup, and
 //      written once into the caller's buffers.
@@ -406,10 +406,9 @@ constexpr int kBkBits = 11;
 constexpr int kBkBins = 1 << kBkBits;
 constexpr int kBkBlock = 512;
 constexpr int kBkWaves = kBkBlock / 64;
-#ifndef SWR_BK_KPT
-#define SWR_BK_KPT 16
-#endif
-constexpr int kBkKpt = SWR_BK_KPT;  // keys per lane of a partition tile
+// Keys per lane of a partition tile: 16 (8 K-key tiles, two workgroups per CU) measured faster
+// than 32 (16 K-key tiles, one workgroup per CU): 0.373 against 0.386 ms for 2^24 ray keys.
+constexpr int kBkKpt = 16;
 constexpr int kBkTile = kBkBlock * kBkKpt;
 constexpr int kBkRangeBlock = 1024;
 constexpr int kBkRangeGroups = 256;  // at most: the consumers reduce one (min, max) pair per group
@@ -1094,7 +1093,7 @@ void sort_reserve(swr_context* ctx, size_t n) {
     SWR_HIP(hipStreamSynchronize(ctx->stream));
   }
   // Per-tile digit counts of the scanned passes: 256 rows of the tile count at the smallest tile,
-  // rounded up to whole upsweep groups; the bucket sort's 2048 counts per 16 K-key tile share the
+  // rounded up to whole upsweep groups; the bucket sort's 2048 counts per 8 K-key tile share the
   // buffer.
   size_t tiles = (n + (size_t)kSortBlock * kSortMinKeysPerThread - 1) / ((size_t)kSortBlock * kSortMinKeysPerThread);
   tiles = (tiles + kUpTiles - 1) / kUpTiles * kUpTiles;
