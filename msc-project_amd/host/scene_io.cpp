@@ -1,6 +1,7 @@
-// scene_io.cpp -- YAML subset, OBJ (tinyobj semantics) and EXR writer.
+// scene_io.cpp -- YAML subset, OBJ (tinyobj semantics), EXR writer and the texture image readers (PFM, PPM, PNG, EXR).
 #include "scene_io.h"
 
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -12,6 +13,8 @@
 #include <sstream>
 #include <stdexcept>
 #include <tuple>
+
+#include <zlib.h>
 
 namespace swr {
 namespace io {
@@ -760,6 +763,143 @@ Image read_exr(const std::vector<uint8_t>& b, const std::string& path) {
   return im;
 }
 
+// PNG (ISO/IEC 15948): chunks checked against their CRC, the IDAT stream inflated with zlib,
+// each row unfiltered (None, Sub, Up, Average, Paeth), Adam7 passes placed on the grid. Samples
+// become floats as sample / (2^depth - 1) and palette entries as entry / 255 (a division, as the
+// PPM reader's sample / maxval). Channels are presented in file
+// order (grey: Y; grey + alpha: Y, A; RGB; RGBA; palette: RGB) with missing ones 0, as the other
+// readers present them; rows top first.
+uint32_t get_be32(const std::vector<uint8_t>& b, size_t pos, const std::string& path) {
+  if (pos > b.size() || b.size() - pos < 4) image_error(path, "truncated file");
+  return (uint32_t)b[pos] << 24 | (uint32_t)b[pos + 1] << 16 | (uint32_t)b[pos + 2] << 8 | b[pos + 3];
+}
+
+Image read_png(const std::vector<uint8_t>& b, const std::string& path) {
+  static const uint8_t kSig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
+  if (b.size() < 8 || std::memcmp(b.data(), kSig, 8) != 0) image_error(path, "bad PNG signature");
+  int64_t w = 0, h = 0;
+  int depth = 0, ctype = -1, interlace = 0;
+  std::vector<uint8_t> palette, z;
+  bool ended = false;
+  for (size_t pos = 8; !ended;) {
+    const uint32_t len = get_be32(b, pos, path);
+    if (len > 0x7FFFFFFFu || b.size() - pos < 12 + (size_t)len) image_error(path, "truncated chunk");
+    const std::string type(b.begin() + pos + 4, b.begin() + pos + 8);
+    const uint8_t* data = b.data() + pos + 8;
+    if ((uint32_t)crc32(crc32(0L, Z_NULL, 0), b.data() + pos + 4, 4 + len) != get_be32(b, pos + 8 + len, path))
+      image_error(path, "CRC mismatch in chunk " + type);
+    if (type == "IHDR") {
+      if (len != 13) image_error(path, "bad IHDR");
+      w = get_be32(b, pos + 8, path);
+      h = get_be32(b, pos + 12, path);
+      depth = data[8];
+      ctype = data[9];
+      interlace = data[12];
+      if (data[10] != 0 || data[11] != 0 || interlace > 1) image_error(path, "unknown compression, filter or interlace method");
+    } else if (type == "PLTE") {
+      palette.assign(data, data + len);
+    } else if (type == "IDAT") {
+      z.insert(z.end(), data, data + len);
+    } else if (type == "IEND") {
+      ended = true;
+    } else if (!(type[0] & 0x20)) {
+      image_error(path, "unknown critical chunk " + type);
+    }
+    pos += 12 + (size_t)len;
+  }
+  check_size(path, w, h);
+  int chans = 0;
+  switch (ctype) {
+    case 0: chans = 1; break;
+    case 2: chans = 3; break;
+    case 3: chans = 1; break;
+    case 4: chans = 2; break;
+    case 6: chans = 4; break;
+    default: image_error(path, "unknown colour type");
+  }
+  const bool depth_ok = ctype == 0 ? (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)
+                        : ctype == 3 ? (depth == 1 || depth == 2 || depth == 4 || depth == 8)
+                                     : (depth == 8 || depth == 16);
+  if (!depth_ok) image_error(path, "bit depth not allowed for the colour type");
+  if (ctype == 3 && (palette.empty() || palette.size() % 3 != 0 || palette.size() > 768)) image_error(path, "bad palette");
+  // Adam7 passes (x0, y0, dx, dy), or the whole image as one pass.
+  static const int kAdam7[7][4] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4},
+                                   {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
+  static const int kWhole[1][4] = {{0, 0, 1, 1}};
+  const int npass = interlace ? 7 : 1;
+  const int (*passes)[4] = interlace ? kAdam7 : kWhole;
+  const size_t bits = (size_t)chans * depth;
+  const size_t bpp = std::max<size_t>(1, bits / 8);
+  size_t raw_size = 0;
+  for (int p = 0; p < npass; ++p) {
+    const int64_t pw = w > passes[p][0] ? (w - passes[p][0] + passes[p][2] - 1) / passes[p][2] : 0;
+    const int64_t ph = h > passes[p][1] ? (h - passes[p][1] + passes[p][3] - 1) / passes[p][3] : 0;
+    if (pw && ph) raw_size += (size_t)ph * (1 + ((size_t)pw * bits + 7) / 8);
+  }
+  std::vector<uint8_t> raw(raw_size);
+  uLongf got = (uLongf)raw_size;
+  if (uncompress(raw.data(), &got, z.data(), (uLong)z.size()) != Z_OK || got != raw_size)
+    image_error(path, "corrupt or short image data");
+  Image im;
+  im.width = (int)w;
+  im.height = (int)h;
+  im.rgb.assign((size_t)w * h * 3, 0.f);
+  const float top = (float)((1u << depth) - 1u);
+  size_t off = 0;
+  for (int p = 0; p < npass; ++p) {
+    const int64_t pw = w > passes[p][0] ? (w - passes[p][0] + passes[p][2] - 1) / passes[p][2] : 0;
+    const int64_t ph = h > passes[p][1] ? (h - passes[p][1] + passes[p][3] - 1) / passes[p][3] : 0;
+    if (!pw || !ph) continue;
+    const size_t rowb = ((size_t)pw * bits + 7) / 8;
+    std::vector<uint8_t> prev(rowb, 0), cur(rowb);
+    for (int64_t y = 0; y < ph; ++y) {
+      const uint8_t filter = raw[off];
+      const uint8_t* in = raw.data() + off + 1;
+      off += 1 + rowb;
+      for (size_t i = 0; i < rowb; ++i) {
+        const int a = i >= bpp ? cur[i - bpp] : 0, up = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+        int pred = 0;
+        switch (filter) {
+          case 0: pred = 0; break;
+          case 1: pred = a; break;
+          case 2: pred = up; break;
+          case 3: pred = (a + up) / 2; break;
+          case 4: {
+            const int pa = std::abs(up - c), pb = std::abs(a - c), pc = std::abs(a + up - 2 * c);
+            pred = (pa <= pb && pa <= pc) ? a : (pb <= pc ? up : c);
+            break;
+          }
+          default: image_error(path, "unknown row filter");
+        }
+        cur[i] = (uint8_t)(in[i] + pred);
+      }
+      const int64_t yy = passes[p][1] + y * passes[p][3];
+      for (int64_t x = 0; x < pw; ++x) {
+        const int64_t xx = passes[p][0] + x * passes[p][2];
+        float* dst = &im.rgb[3 * (size_t)(yy * w + xx)];
+        for (int c = 0; c < chans; ++c) {
+          const size_t bit = ((size_t)x * chans + c) * depth;
+          uint32_t v;
+          if (depth == 16)
+            v = (uint32_t)cur[bit / 8] << 8 | cur[bit / 8 + 1];
+          else if (depth == 8)
+            v = cur[bit / 8];
+          else
+            v = (cur[bit / 8] >> (8 - depth - bit % 8)) & ((1u << depth) - 1u);
+          if (ctype == 3) {
+            if (3 * (size_t)v + 2 >= palette.size()) image_error(path, "palette index out of range");
+            for (int k = 0; k < 3; ++k) dst[k] = (float)palette[3 * v + k] / 255.f;
+          } else if (c < 3) {
+            dst[c] = (float)v / top;
+          }
+        }
+      }
+      std::swap(prev, cur);
+    }
+  }
+  return im;
+}
+
 }  // namespace
 
 Image read_image(const std::string& path) {
@@ -769,7 +909,8 @@ Image read_image(const std::string& path) {
   if (b.size() >= 4 && get_le<int32_t>(b, 0, path) == 20000630) return read_exr(b, path);
   if (b.size() >= 2 && b[0] == 'P' && (b[1] == 'F' || b[1] == 'f')) return read_pfm(b, path);
   if (b.size() >= 2 && b[0] == 'P' && b[1] == '6') return read_ppm(b, path);
-  image_error(path, "unknown format (PFM, binary PPM and uncompressed scanline EXR are read)");
+  if (b.size() >= 8 && b[0] == 137 && b[1] == 'P' && b[2] == 'N' && b[3] == 'G') return read_png(b, path);
+  image_error(path, "unknown format (PFM, binary PPM, PNG and uncompressed scanline EXR are read)");
 }
 
 }  // namespace io

@@ -1,7 +1,7 @@
 """Host I/O of the product (CPU only): the YAML subset against PyYAML, OBJ loading with
 tinyobj's semantics (tiny_obj_loader.cpp:276-407, 643-890) on hand-written fixtures, IEEE
 half rounding against numpy, the EXR file layout read back by a reader written from the
-OpenEXR file-layout description, and the texture image readers (PFM, PPM, uncompressed EXR)
+OpenEXR file-layout description, and the texture image readers (PFM, PPM, PNG, uncompressed EXR)
 against files written here from the same format descriptions."""
 import json
 import os
@@ -20,7 +20,7 @@ HOST = os.path.join(ROOT, "msc-project_amd", "host")
 def harness(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("harness") / "io_harness")
     subprocess.run(["g++", "-O1", "-std=c++17", "-I", HOST, os.path.join(ROOT, "tests", "harness", "io_harness.cpp"),
-                    os.path.join(HOST, "scene_io.cpp"), "-o", out], check=True)
+                    os.path.join(HOST, "scene_io.cpp"), "-o", out, "-lz"], check=True)
     return out
 
 
@@ -286,6 +286,154 @@ def test_product_exr_reads_back(harness, tmp_path):
     path = str(tmp_path / "o.exr")
     assert swr.load_library().swr_write_exr(path.encode(), rgb.ctypes.data, 5, 3) == 0
     np.testing.assert_array_equal(_read_image(harness, path), rgb.astype(np.float16).astype(np.float32))
+
+
+def _png_bytes(samples, ctype, depth, palette=None, filters=(0,), interlace=False):
+    """A PNG built from ISO/IEC 15948: samples is an (h, w, channels) integer array (palette
+    indices for colour type 3); row r of each pass is filtered with filters[r % len(filters)]."""
+    import zlib
+    h, w, chans = samples.shape
+    bits = chans * depth
+    bpp = max(1, bits // 8)
+
+    def pack_row(row):
+        if depth == 16:
+            return b"".join(struct.pack(">H", int(v)) for v in row.reshape(-1))
+        if depth == 8:
+            return bytes(int(v) for v in row.reshape(-1))
+        out, acc, n = bytearray(), 0, 0
+        for v in row.reshape(-1):
+            acc = (acc << depth) | int(v)
+            n += depth
+            if n == 8:
+                out.append(acc)
+                acc, n = 0, 0
+        if n:
+            out.append(acc << (8 - n))
+        return bytes(out)
+
+    def paeth(a, b, c):
+        p = a + b - c
+        pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+        return a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+
+    def filt(kind, cur, prev):
+        out = bytearray()
+        for i, x in enumerate(cur):
+            a = cur[i - bpp] if i >= bpp else 0
+            b = prev[i]
+            c = prev[i - bpp] if i >= bpp else 0
+            pred = [0, a, b, (a + b) // 2, paeth(a, b, c)][kind]
+            out.append((x - pred) & 0xFF)
+        return bytes([kind]) + bytes(out)
+
+    passes = ([(0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2), (0, 1, 1, 2)]
+              if interlace else [(0, 0, 1, 1)])
+    raw = bytearray()
+    for x0, y0, dx, dy in passes:
+        sub = samples[y0::dy, x0::dx]
+        if sub.shape[0] == 0 or sub.shape[1] == 0:
+            continue
+        prev = bytes(len(pack_row(sub[0])))
+        for r in range(sub.shape[0]):
+            cur = pack_row(sub[r])
+            raw += filt(filters[r % len(filters)], cur, prev)
+            prev = cur
+
+    def chunk(kind, data):
+        return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+    out = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, int(interlace)))
+    if palette is not None:
+        out += chunk(b"PLTE", bytes(palette.reshape(-1).astype(np.uint8)))
+    out += chunk(b"tEXt", b"Comment\0ancillary chunks are skipped")
+    z = zlib.compress(bytes(raw), 9)
+    out += chunk(b"IDAT", z[: len(z) // 2]) + chunk(b"IDAT", z[len(z) // 2:])  # the stream split over two chunks
+    return out + chunk(b"IEND", b"")
+
+
+@pytest.mark.parametrize("ctype,depth", [(0, 1), (0, 2), (0, 4), (0, 8), (0, 16), (2, 8), (2, 16), (3, 1), (3, 2),
+                                         (3, 4), (3, 8), (4, 8), (4, 16), (6, 8), (6, 16)])
+@pytest.mark.parametrize("interlace", [False, True])
+def test_png_colour_types_depths_filters_and_interlace(harness, tmp_path, ctype, depth, interlace):
+    """Every colour type and bit depth PNG allows, every row filter (rows cycle through None, Sub,
+    Up, Average, Paeth), plain and Adam7: samples read as sample / (2^depth - 1), palette entries as
+    / 255, channels in file order (grey: Y; grey + alpha: Y, A; RGBA: R, G, B) with missing ones 0."""
+    rng = np.random.default_rng(ctype * 100 + depth + 7 * interlace)
+    h, w = 13, 11  # odd sizes: partial Adam7 blocks and partial bytes at depths below 8
+    chans = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    top = (1 << depth) - 1
+    palette = None
+    if ctype == 3:
+        palette = rng.integers(0, 256, (min(1 << depth, 200), 3))
+        samples = rng.integers(0, len(palette), (h, w, 1))
+        want = palette[samples[:, :, 0]].astype(np.float32) / np.float32(255)
+    else:
+        samples = rng.integers(0, top + 1, (h, w, chans))
+        want = np.zeros((h, w, 3), np.float32)
+        k = min(chans, 3)
+        want[:, :, :k] = samples[:, :, :k].astype(np.float32) / np.float32(top)
+    p = tmp_path / "t.png"
+    p.write_bytes(_png_bytes(samples, ctype, depth, palette, filters=(0, 1, 2, 3, 4), interlace=interlace))
+    np.testing.assert_array_equal(_read_image(harness, p), want)
+
+
+def test_png_written_by_pillow(harness, tmp_path):
+    """PNGs from an independent encoder (Pillow, which chooses its own row filters): grey 1, 8 and
+    16 bits, grey + alpha, RGB, RGBA and a palette image read back to the same values."""
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(77)
+    h, w = 19, 23
+    g8 = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    g16 = rng.integers(0, 65536, (h, w)).astype(np.uint16)
+    bits = rng.integers(0, 2, (h, w)).astype(bool)
+    rgba = rng.integers(0, 256, (h, w, 4)).astype(np.uint8)
+    idx = rng.integers(0, 40, (h, w)).astype(np.uint8)
+    pal = rng.integers(0, 256, (40, 3)).astype(np.uint8)
+    z = np.zeros((h, w), np.float32)
+    f = lambda a, top: a.astype(np.float32) / np.float32(top)
+    cases = {
+        "L": (Image.fromarray(g8, "L"), np.stack([f(g8, 255), z, z], -1)),
+        "I16": (Image.fromarray(g16, "I;16"), np.stack([f(g16, 65535), z, z], -1)),
+        "1": (Image.fromarray(bits), np.stack([f(bits, 1), z, z], -1)),
+        "LA": (Image.fromarray(rgba[:, :, :2], "LA"), np.stack([f(rgba[:, :, 0], 255), f(rgba[:, :, 1], 255), z], -1)),
+        "RGB": (Image.fromarray(rgba[:, :, :3], "RGB"), f(rgba[:, :, :3], 255)),
+        "RGBA": (Image.fromarray(rgba, "RGBA"), f(rgba[:, :, :3], 255)),
+    }
+    pim = Image.fromarray(idx, "P")
+    pim.putpalette(pal.reshape(-1).tolist())
+    cases["P"] = (pim, f(pal[idx], 255))
+    for name, (im, want) in cases.items():
+        p = tmp_path / (name + ".png")
+        im.save(p)
+        np.testing.assert_array_equal(_read_image(harness, p), want, err_msg=name)
+
+
+def test_png_errors(harness, tmp_path):
+    """Corrupt PNGs are refused: a bad chunk CRC, truncated image data, a bit depth the colour type
+    does not allow, an unknown critical chunk, and a palette index past the palette."""
+    good = _png_bytes(np.zeros((4, 4, 3), np.int64), 2, 8)
+    bad_crc = bytearray(good)
+    bad_crc[29] ^= 1  # inside the IHDR CRC
+    short = _png_bytes(np.zeros((4, 4, 3), np.int64), 2, 8)
+    ihdr_end = 8 + 12 + 13
+    cases = {
+        "crc.png": bytes(bad_crc),
+        "depth.png": _png_bytes(np.zeros((4, 4, 3), np.int64), 2, 4),
+        "critical.png": good[:ihdr_end] + b"\0\0\0\0ZZZZ" + struct.pack(">I", 0x4F6B6F0A) + good[ihdr_end:],
+        "index.png": _png_bytes(np.full((4, 4, 1), 3, np.int64), 3, 8, palette=np.zeros((2, 3), np.int64)),
+        "truncated.png": short[: len(short) // 2],
+    }
+    import zlib
+    crit = bytearray(cases["critical.png"])
+    # give the unknown critical chunk a correct CRC, so that only its kind is wrong
+    crc = zlib.crc32(b"ZZZZ") & 0xFFFFFFFF
+    crit[ihdr_end + 8:ihdr_end + 12] = struct.pack(">I", crc)
+    cases["critical.png"] = bytes(crit)
+    for name, data in cases.items():
+        (tmp_path / name).write_bytes(data)
+        r = subprocess.run([harness, "image", str(tmp_path / name)], capture_output=True, text=True)
+        assert r.returncode == 1 and r.stdout.startswith("ERROR"), (name, r.stdout, r.stderr)
 
 
 def test_image_errors(harness, tmp_path):

@@ -47,6 +47,48 @@ def test_textured_yaml_scene_renders_like_the_arrays(ctx, tmp_path):
     sc_arr.close()
 
 
+def _write_png_rgb8(path, rgb8):
+    """An 8-bit RGB PNG (ISO/IEC 15948: IHDR, one IDAT of unfiltered rows, IEND)."""
+    import struct
+    import zlib
+    h, w, _ = rgb8.shape
+    raw = b"".join(b"\0" + rgb8[y].astype(np.uint8).tobytes() for y in range(h))
+
+    def chunk(kind, data):
+        return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0))
+                + chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))
+
+
+def test_png_textures_render_like_pfm(ctx, tmp_path):
+    """The textured room with its images quantised to 8 bits, once with the images as PFM and once
+    as PNG (a StandardTexture string naming a .png, as reference scene files do): the same image,
+    bit for bit (the PNG reader's sample / 255 is the value the PFM holds)."""
+    desc = scenes.textured_room(width=40, height=32, base=2, sphere_level=2)
+    q = [np.clip(np.rint(np.asarray(im, np.float32) * 255), 0, 255).astype(np.uint8) for im in desc["images"]]
+    desc["images"] = [x.astype(np.float32) / np.float32(255) for x in q]
+    path = scenes.write_scene(desc, str(tmp_path), "tex")
+    text = open(path).read()
+    for k, x in enumerate(q):
+        _write_png_rgb8(str(tmp_path / ("tex_image%d.png" % k)), x)
+        assert ("tex_image%d.pfm" % k) in text
+        text = text.replace("tex_image%d.pfm" % k, "tex_image%d.png" % k)
+    png_path = str(tmp_path / "tex_png.yaml")
+    with open(png_path, "w") as f:
+        f.write(text)
+    sc_pfm, rd_pfm = swr.load_yaml(ctx, path)
+    sc_png, rd_png = swr.load_yaml(ctx, png_path)
+    rd_png.seed = rd_pfm.seed
+    p_pfm, _, _ = sc_pfm.render(rd_pfm)
+    p_png, _, _ = sc_png.render(rd_png)
+    assert np.isfinite(p_png).all() and p_png.max() > 0
+    np.testing.assert_array_equal(p_png, p_pfm)
+    sc_pfm.close()
+    sc_png.close()
+
+
 def test_yaml_texture_errors(ctx, tmp_path):
     (tmp_path / "t.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\nf 1/1 2/2 3/3\n")
     obj = ("object: {type: Polygon, filename: t.obj, translation: [0, 0, 0], rotation: [0, 0, 0], "
