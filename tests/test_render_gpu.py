@@ -208,17 +208,26 @@ def test_budget_refuses_large_render(ctx):
 
 
 def test_tuning_changes_speed_not_pixels(ctx):
-    """Sort tile sizes, key widths and traversal stack rings only reorder rays and hits or move
-    stack entries between LDS and device memory: every tuning gives the same image, bit for bit."""
+    """Sort pass structures, tile sizes, key widths and traversal stack rings only reorder rays and
+    hits or move stack entries between LDS and device memory: every tuning gives the same image,
+    bit for bit."""
     desc = scenes.box_room(width=40, height=40, base=2, sphere_level=3)
     sc = swr.Scene(ctx, desc)
     rd, _ = _pair(desc)
     before = ctx.tuning()
     ref = sc.render(rd)[0]
     try:
-        for kpt, rb, hb, ring, alg in [(8, 32, 0, 8, 0), (12, 24, 0, 16, 1), (20, 16, 8, 64, 1), (24, 3, 4, 32, 0)]:
+        for kpt, rb, hb, ring, alg in [(8, 32, 0, 8, 0), (12, 24, 0, 16, 1), (20, 16, 8, 64, 1), (24, 3, 4, 32, 0),
+                                       (16, 32, 0, 16, 2)]:
             ctx.set_tuning(kpt, rb, hb, ring, alg)
             np.testing.assert_array_equal(sc.render(rd)[0], ref)
+        # With the bucket sort selected, a scene committed now sorts its triangles' 30-bit Morton
+        # codes through it (the count is known at launch): the same BVH, the same image.
+        sc2 = swr.Scene(ctx, desc)
+        np.testing.assert_array_equal(sc2.render(rd)[0], ref)
+        sc2.close()
+        with pytest.raises(swr.SwrError):
+            ctx.set_tuning(16, 32, 0, 16, 3)
         with pytest.raises(swr.SwrError):
             ctx.set_tuning(10, 32, 0)
         with pytest.raises(swr.SwrError):
