@@ -3,15 +3,15 @@
 // Replaces Embree's acceleration build, rtcCommit (src/core/Pathtracer.cpp:274), over the
 // triangle meshes the reference registers per object and light (:178-201, :242-265).
 // Steps (Karras, "Maximizing parallelism in the construction of BVHs, octrees and k-d trees",
-// HPG 2012, plus a greedy top-down BVH2 -> BVH4 collapse):
+// HPG 2012, plus a surface-area-heuristic BVH2 -> BVH4 collapse):
 //   1. 30-bit Morton code of every triangle centroid over the centroid bounds;
 //   2. stable radix sort of the codes (sort.hip), ties broken by triangle index;
 //   3. one thread per internal node finds its key range and split -> binary radix tree;
-//   4. leaves walk up with per-node arrival counters, the second arrival unions the boxes;
-//   5. level by level, each BVH4 node adopts up to four BVH2 descendants (the one with the
-//      largest box area x triangle count, its SAH cost, expanded first); subtrees of <= leaf_max
-//      triangles become leaves, which are contiguous
-//      runs of the sorted triangle order;
+//   4. leaves walk up with per-node arrival counters, the second arrival unions the boxes and
+//      picks the node's least-cost BVH4 layout from its children's (dynamic programming);
+//   5. level by level, each BVH4 node adopts the up to four BVH2 descendants those choices name;
+//      subtrees the choices make leaves (<= leaf_max triangles) are contiguous runs of the sorted
+//      triangle order;
 //   6. the leaf triangles are written in sorted order as (v0|id, e1|mask, e2).
 #include <algorithm>
 #include <cfloat>
@@ -116,6 +116,89 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
 // workgroup boundary (the top of the tree) need device scope, whose release and acquire write
 // back and invalidate the XCD's L2 (12.4 ms for 2.1 M triangles with device scope throughout,
 // 4.4 ms with 256-leaf workgroups, 1.9 ms with 1024).
+__device__ inline float half_area(float4 lo, float4 hi) {
+  const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// Surface-area-heuristic collapse of the BVH2 into a BVH4 (Ylitie, Karras and Laine, "Efficient
+// incoherent ray traversal on GPUs through compressed wide BVHs", HPG 2017, for four slots):
+// C(x, i) is the least cost of the subtree of BVH2 node x placed in at most i child slots of a
+// BVH4 node, a BVH4 node of box area A costing A * kSahNode and a leaf of t triangles A * t *
+// kSahTri. With L, R the children of x and S(i) = min over j of C(L, j) + C(R, i - j):
+//   C(x, 1) = min(leaf cost if x has at most leaf_max triangles, A * kSahNode + S(4)),
+//   C(x, i) = min(C(x, 1), S(i))  for i = 2, 3, 4.
+// A single triangle (a BVH2 leaf) costs A * kSahTri in any number of slots. The choices are
+// packed into one word per node (BvhChoice) for the top-down collapse. Only the ratio of the two
+// costs matters: on the 2.1 M-triangle terrain (leaves of <= 2 triangles) closest-hit traversal
+// takes 25.8 ms per frame with a triangle costing 0.25 or 0.5 of a node and 26.3 ms at 1.0, against
+// 26.45 ms for a greedy collapse that expands the child of largest area x triangle count first.
+constexpr float kSahNode = 1.0f;
+constexpr float kSahTri = 0.5f;
+// bits 0-1: the slots of L when x is a BVH4 node (1..3; R takes 4 minus that); bit 2: x is a leaf
+// in one slot; bits 4-5, 6-7, 8-9: for i = 2, 3, 4 slots, 0 = x takes one slot, else the slots
+// of L (R takes the rest).
+struct BvhChoice {
+  static __host__ __device__ uint32_t node_split(uint32_t c) { return c & 3u; }
+  static __host__ __device__ bool leaf(uint32_t c) { return (c >> 2) & 1u; }
+  static __host__ __device__ uint32_t split(uint32_t c, uint32_t slots) { return (c >> (2u * slots)) & 3u; }
+};
+
+// The costs C(x, 1..4) and the choice word of internal node `node` from its children's costs.
+__device__ inline void sah_choose(uint32_t node, uint2 ch, float4 lo, float4 hi, uint32_t size, uint32_t leaf_max,
+                                  const float4* __restrict__ leaf_lo, const float4* __restrict__ leaf_hi,
+                                  float4* __restrict__ cost, uint32_t* __restrict__ choice) {
+  float cl[5], cr[5];
+  auto child_costs = [&](uint32_t c, float* out) {
+    if (c & kBvh2Leaf) {
+      const float t = half_area(leaf_lo[c & ~kBvh2Leaf], leaf_hi[c & ~kBvh2Leaf]) * kSahTri;
+      out[1] = out[2] = out[3] = out[4] = t;
+    } else {
+      const float4 k = cost[c];
+      out[1] = k.x;
+      out[2] = k.y;
+      out[3] = k.z;
+      out[4] = k.w;
+    }
+  };
+  child_costs(ch.x, cl);
+  child_costs(ch.y, cr);
+  float split_cost[5];
+  uint32_t split_left[5];
+  for (uint32_t i = 2; i <= 4; ++i) {
+    split_cost[i] = cl[1] + cr[i - 1];
+    split_left[i] = 1;
+    for (uint32_t j = 2; j < i; ++j) {
+      const float c = cl[j] + cr[i - j];
+      if (c < split_cost[i]) {
+        split_cost[i] = c;
+        split_left[i] = j;
+      }
+    }
+  }
+  const float area = half_area(lo, hi);
+  const float as_node = area * kSahNode + split_cost[4];
+  const float as_leaf = size <= leaf_max ? area * kSahTri * (float)size : __int_as_float(0x7F800000);
+  uint32_t c = split_left[4];
+  float one = as_node;
+  if (as_leaf <= as_node) {
+    one = as_leaf;
+    c |= 4u;
+  }
+  float out[5];
+  out[1] = one;
+  for (uint32_t i = 2; i <= 4; ++i) {
+    if (split_cost[i] < one) {
+      out[i] = split_cost[i];
+      c |= split_left[i] << (2u * i);
+    } else {
+      out[i] = one;
+    }
+  }
+  cost[node] = make_float4(out[1], out[2], out[3], out[4]);
+  choice[node] = c;
+}
+
 __global__ __launch_bounds__(1024) void refit_kernel(const float4* __restrict__ pos, const uint4* __restrict__ tri,
                                                      const uint32_t* __restrict__ perm, int n,
                                                      const uint2* __restrict__ children,
@@ -124,7 +207,8 @@ __global__ __launch_bounds__(1024) void refit_kernel(const float4* __restrict__ 
                                                      const uint32_t* __restrict__ parent_leaf,
                                                      uint32_t* __restrict__ visits, float4* __restrict__ leaf_lo,
                                                      float4* __restrict__ leaf_hi, float4* __restrict__ node_lo,
-                                                     float4* __restrict__ node_hi) {
+                                                     float4* __restrict__ node_hi, uint32_t leaf_max,
+                                                     float4* __restrict__ cost, uint32_t* __restrict__ choice) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float3 a, b, c;
@@ -158,63 +242,61 @@ __global__ __launch_bounds__(1024) void refit_kernel(const float4* __restrict__ 
       l1 = node_lo[ch.y];
       h1 = node_hi[ch.y];
     }
-    node_lo[node] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
-    node_hi[node] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+    const float4 lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+    const float4 hi = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
+    node_lo[node] = lo;
+    node_hi[node] = hi;
+    // the collapse's costs and choices, from the children's (published like their boxes)
+    sah_choose(node, ch, lo, hi, rg.y - rg.x + 1u, leaf_max, leaf_lo, leaf_hi, cost, choice);
     if (node == 0) return;
     node = parent_internal[node];
   }
-}
-
-__device__ inline float half_area(float4 lo, float4 hi) {
-  const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
-  return dx * dy + dy * dz + dz * dx;
 }
 
 struct Bvh2View {
   const uint2* children;
   const uint2* range;
   const float4 *leaf_lo, *leaf_hi, *node_lo, *node_hi;
+  const uint32_t* choice;  // BvhChoice words of the internal nodes
 };
-
-// Size of the sorted-primitive run under a BVH2 reference (bit 31 = leaf).
-__device__ inline uint32_t ref_size(const Bvh2View& v, uint32_t r) {
-  if (r & kBvh2Leaf) return 1u;
-  const uint2 g = v.range[r];
-  return g.y - g.x + 1u;
-}
 
 // One frontier entry (BVH4 node index, BVH2 internal node) becomes a filled BVH4 node.
 __global__ void collapse_kernel(Bvh2View v, const uint2* __restrict__ frontier, const uint32_t* __restrict__ count_in,
                                 uint2* __restrict__ next, uint32_t* __restrict__ count_next,
                                 uint32_t* __restrict__ node_alloc, Node4* __restrict__ nodes, uint32_t depth,
-                                uint32_t leaf_max, uint32_t capacity, uint32_t* __restrict__ overflow) {
+                                uint32_t capacity, uint32_t* __restrict__ overflow) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= *count_in) return;
   const uint2 item = frontier[i];
-  uint32_t ref[4];
-  int cnt = 2;
-  const uint2 ch = v.children[item.y];
-  ref[0] = ch.x;
-  ref[1] = ch.y;
-  // Expand the internal child with the largest SAH cost that is too big to be a leaf, up to four
-  // children.
-  while (cnt < 4) {
-    int best = -1;
-    float best_area = -1.f;
-    for (int c = 0; c < cnt; ++c) {
-      if ((ref[c] & kBvh2Leaf) || ref_size(v, ref[c]) <= leaf_max) continue;
-      // surface-area-heuristic cost of the child: box area times the triangles under it
-      const float a = half_area(v.node_lo[ref[c]], v.node_hi[ref[c]]) * (float)ref_size(v, ref[c]);
-      if (a > best_area) {
-        best_area = a;
-        best = c;
-      }
-    }
-    if (best < 0) break;
-    const uint2 g = v.children[ref[best]];
-    ref[best] = g.x;
-    ref[cnt++] = g.y;
+  // The node's four slots, filled as the surface-area-heuristic choices of the refit say: the
+  // BVH2 node's children share the slots, and a descendant given more than one slot passes them
+  // on to its own children (left subtree first). At most four entries are pending at a time.
+  uint32_t ref[4], pend_x[4], pend_slots[4];
+  int cnt = 0, np = 0;
+  {
+    const uint2 ch = v.children[item.y];
+    const uint32_t j = BvhChoice::node_split(v.choice[item.y]);
+    pend_x[np] = ch.y;
+    pend_slots[np++] = 4u - j;
+    pend_x[np] = ch.x;
+    pend_slots[np++] = j;
   }
+  while (np > 0) {
+    --np;
+    const uint32_t x = pend_x[np], slots = pend_slots[np];
+    const uint32_t d = (x & kBvh2Leaf) || slots == 1u ? 0u : BvhChoice::split(v.choice[x], slots);
+    if (d == 0u) {
+      ref[cnt++] = x;
+      continue;
+    }
+    const uint2 g = v.children[x];
+    pend_x[np] = g.y;
+    pend_slots[np++] = slots - d;
+    pend_x[np] = g.x;
+    pend_slots[np++] = d;
+  }
+  // A child in one slot is a leaf when it is one triangle or the choices made it one.
+  auto is_leaf = [&](uint32_t r) { return (r & kBvh2Leaf) || BvhChoice::leaf(v.choice[r]); };
   Node4 nd;
   float lo[3][4], hi[3][4];
   uint32_t cc[4];
@@ -222,7 +304,7 @@ __global__ void collapse_kernel(Bvh2View v, const uint2* __restrict__ frontier, 
   // pairs; the allocator starts at 2), so that quantised siblings (64 B) share a 128-byte line.
   uint32_t internal = 0;
   for (int c = 0; c < cnt; ++c)
-    if (!(ref[c] & kBvh2Leaf) && ref_size(v, ref[c]) > leaf_max) ++internal;
+    if (!is_leaf(ref[c])) ++internal;
   uint32_t next_idx = internal ? atomicAdd(node_alloc, (internal + 1u) & ~1u) : 0u;
   if (internal && next_idx + internal > capacity) {  // no room: the host fails the build
     atomicOr(overflow, 1u);
@@ -248,7 +330,7 @@ __global__ void collapse_kernel(Bvh2View v, const uint2* __restrict__ frontier, 
     hi[0][c] = h.x; hi[1][c] = h.y; hi[2][c] = h.z;
     if (r & kBvh2Leaf) {
       cc[c] = leaf_code(r & ~kBvh2Leaf, 1u);
-    } else if (ref_size(v, r) <= leaf_max) {
+    } else if (is_leaf(r)) {
       const uint2 g = v.range[r];
       cc[c] = leaf_code(g.x, g.y - g.x + 1u);
     } else {
@@ -358,11 +440,11 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
 
   // Temporary buffers for the build, released at the end.
   DeviceBuffer codes, perm, alt_k, alt_v, children, range, par_i, par_l, visits, leaf_lo, leaf_hi, node_lo, node_hi,
-      fr_a, fr_b, counters;
+      fr_a, fr_b, counters, sah_cost, sah_choice;
   auto cleanup = [&] {
     (void)hipStreamSynchronize(s);
     for (DeviceBuffer* b : {&codes, &perm, &alt_k, &alt_v, &children, &range, &par_i, &par_l, &visits, &leaf_lo,
-                            &leaf_hi, &node_lo, &node_hi, &fr_a, &fr_b, &counters})
+                            &leaf_hi, &node_lo, &node_hi, &fr_a, &fr_b, &counters, &sah_cost, &sah_choice})
       release(ctx, *b);
   };
   try {
@@ -392,6 +474,8 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
     ensure(ctx, visits, ni * 4);
     ensure(ctx, node_lo, ni * 16);
     ensure(ctx, node_hi, ni * 16);
+    ensure(ctx, sah_cost, ni * 16);
+    ensure(ctx, sah_choice, ni * 4);
     if (n > 1)
       karras_kernel<<<blocks(n - 1), 256, 0, s>>>(sk, (int)n, static_cast<uint2*>(children.ptr),
                                                   static_cast<uint2*>(range.ptr), static_cast<uint32_t*>(par_i.ptr),
@@ -402,7 +486,9 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
                                            static_cast<uint32_t*>(par_l.ptr),
                                            static_cast<uint32_t*>(visits.ptr), static_cast<float4*>(leaf_lo.ptr),
                                            static_cast<float4*>(leaf_hi.ptr), static_cast<float4*>(node_lo.ptr),
-                                           static_cast<float4*>(node_hi.ptr));
+                                           static_cast<float4*>(node_hi.ptr), (uint32_t)leaf_max,
+                                           static_cast<float4*>(sah_cost.ptr),
+                                           static_cast<uint32_t*>(sah_choice.ptr));
     leaf_tris_kernel<<<blocks(n), 256, 0, s>>>(pos, tri, geoms, sv, n, static_cast<float4*>(out->leaf_tris.ptr));
     SWR_HIP(hipGetLastError());
 
@@ -443,7 +529,8 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
     SWR_HIP(hipMemcpyAsync(fr_a.ptr, &root_item, 8, hipMemcpyHostToDevice, s));
     Bvh2View view{static_cast<uint2*>(children.ptr),  static_cast<uint2*>(range.ptr),
                   static_cast<float4*>(leaf_lo.ptr),  static_cast<float4*>(leaf_hi.ptr),
-                  static_cast<float4*>(node_lo.ptr),  static_cast<float4*>(node_hi.ptr)};
+                  static_cast<float4*>(node_lo.ptr),  static_cast<float4*>(node_hi.ptr),
+                  static_cast<uint32_t*>(sah_choice.ptr)};
     uint32_t* cnt = static_cast<uint32_t*>(counters.ptr);
     uint2* cur = static_cast<uint2*>(fr_a.ptr);
     uint2* nxt = static_cast<uint2*>(fr_b.ptr);
@@ -452,7 +539,7 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
     while (frontier) {
       SWR_HIP(hipMemsetAsync(cnt + nxt_slot, 0, 4, s));
       collapse_kernel<<<blocks(frontier), 256, 0, s>>>(view, cur, cnt + cur_slot, nxt, cnt + nxt_slot, cnt, nodes, depth,
-                                                       (uint32_t)leaf_max, (uint32_t)nn, cnt + 3);
+                                                       (uint32_t)nn, cnt + 3);
       SWR_HIP(hipGetLastError());
       SWR_HIP(hipMemcpyAsync(&frontier, cnt + nxt_slot, 4, hipMemcpyDeviceToHost, s));
       SWR_HIP(hipStreamSynchronize(s));
