@@ -3,16 +3,20 @@
 // Replaces Embree's acceleration build, rtcCommit (src/core/Pathtracer.cpp:274), over the
 // triangle meshes the reference registers per object and light (:178-201, :242-265).
 // Steps (Karras, "Maximizing parallelism in the construction of BVHs, octrees and k-d trees",
-// HPG 2012, plus a surface-area-heuristic BVH2 -> BVH4 collapse):
+// HPG 2012; Karras and Aila's treelet restructuring, HPG 2013; a surface-area-heuristic
+// BVH2 -> BVH4 collapse):
 //   1. 30-bit Morton code of every triangle centroid over the centroid bounds;
 //   2. stable radix sort of the codes (sort.hip), ties broken by triangle index;
 //   3. one thread per internal node finds its key range and split -> binary radix tree;
-//   4. leaves walk up with per-node arrival counters, the second arrival unions the boxes and
-//      picks the node's least-cost BVH4 layout from its children's (dynamic programming);
-//   5. level by level, each BVH4 node adopts the up to four BVH2 descendants those choices name;
-//      subtrees the choices make leaves (<= leaf_max triangles) are contiguous runs of the sorted
+//   4. leaves walk up with per-node arrival counters, the second arrival unions the boxes;
+//   5. rounds of treelet restructuring (bottom-up the same way) lower the tree's SAH cost; the
+//      triangles are then renumbered in the new tree's depth-first order;
+//   6. step 4 again over that order, the second arrival also picking the node's least-cost BVH4
+//      layout from its children's (dynamic programming);
+//   7. level by level, each BVH4 node adopts the up to four BVH2 descendants those choices name;
+//      subtrees the choices make leaves (<= leaf_max triangles) are contiguous runs of the
 //      triangle order;
-//   6. the leaf triangles are written in sorted order as (v0|id, e1|mask, e2).
+//   8. the leaf triangles are written in that order as (v0|id, e1|mask, e2).
 #include <algorithm>
 #include <cfloat>
 
@@ -242,14 +246,299 @@ __global__ __launch_bounds__(1024) void refit_kernel(const float4* __restrict__ 
       l1 = node_lo[ch.y];
       h1 = node_hi[ch.y];
     }
-    const float4 lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
+    // w: the triangle count under the node (bits) and the node's binary-tree SAH cost, the sum of
+    // the box areas of the internal nodes of its subtree (a triangle's box area is paid once in
+    // any tree, so it is left out; leaf boxes carry 0 in w).
+    const float4 lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z),
+                                  __uint_as_float(rg.y - rg.x + 1u));
     const float4 hi = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
     node_lo[node] = lo;
-    node_hi[node] = hi;
+    node_hi[node] = make_float4(hi.x, hi.y, hi.z, half_area(lo, hi) + h0.w + h1.w);
     // the collapse's costs and choices, from the children's (published like their boxes)
     sah_choose(node, ch, lo, hi, rg.y - rg.x + 1u, leaf_max, leaf_lo, leaf_hi, cost, choice);
     if (node == 0) return;
     node = parent_internal[node];
+  }
+}
+
+// Treelet restructuring of the binary tree (Karras and Aila, "Fast parallel construction of
+// high-quality bounding volume hierarchies", HPG 2013). A bottom-up pass like the refit's: the
+// second arrival at a node of at least `min_size` triangles grows a treelet under it (starting
+// from the node's two children, the treelet leaf of largest box area is replaced by its children
+// until there are kTreeletLeaves leaves) and rebuilds the treelet's internal nodes in the topology
+// that minimises the sum of internal box areas, by dynamic programming over the 2^7 subsets of
+// treelet leaves. The treelet's subtrees are left as they are; its internal nodes reuse their
+// indices. The wave that finds such a node works the subsets lane-parallel; the node's own lane
+// does every global load and store of the treelet, so that its acquiring arrival orders the
+// loads and its next (releasing) arrival publishes the stores.
+//
+// Node records while restructuring: node_lo.w = triangle count (bits), node_hi.w = subtree cost,
+// range = (lowest, highest) sorted leaf index under the node, which is no longer a contiguous run;
+// treelet_layout_kernel renumbers the leaves afterwards.
+constexpr int kTreeletLeaves = 7;
+constexpr uint32_t kTreeletSets = 1u << kTreeletLeaves;
+constexpr int kTreeletBlock = 1024;
+// Rounds, each with its least treelet-root size in triangles. On the 2.1 M-triangle terrain
+// (closest-hit traversal per 1080p frame; scene commit): none 25.8 ms, 61 ms; one round at 7
+// triangles 23.4 ms, 70 ms; three 22.0 ms, 82 ms; five at 3 triangles 21.6 ms, 101 ms; eight or
+// twelve rounds no faster.
+constexpr uint32_t kTreeletMinSize[] = {3u, 3u, 3u, 3u, 3u};
+// Deepest binary tree the leaf renumbering walks (30-bit Morton codes and a 2^30-triangle index
+// bound a radix tree's depth by 61; each treelet rebuild can only add levels inside its 7 leaves).
+constexpr uint32_t kMaxBvh2Depth = 4096;
+
+struct TreeletWave {
+  float cost[kTreeletSets];   // least cost of a subset of treelet leaves joined under one node
+  uint8_t part[kTreeletSets];  // the subset's best split: the part holding its lowest leaf
+  float4 lo[8], hi[8];        // treelet leaves: box, lo.w = triangles (bits), hi.w = cost
+  uint2 rng[8];
+  uint32_t ref[8];    // treelet leaves (bit 31 = triangle)
+  uint32_t inner[8];  // the treelet's internal node indices, reused by the new topology
+  uint32_t q_sub[8], q_idx[8];
+  uint32_t k;
+};
+
+// Orders this wave's shared-memory writes before its later reads in other lanes.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline void treelet_load(TreeletWave& t, uint32_t j, uint32_t r, const uint2* range,
+                                    const float4* leaf_lo, const float4* leaf_hi, const float4* node_lo,
+                                    const float4* node_hi) {
+  t.ref[j] = r;
+  if (r & kBvh2Leaf) {
+    const uint32_t l = r & ~kBvh2Leaf;
+    const float4 lo = leaf_lo[l], hi = leaf_hi[l];
+    t.lo[j] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(1u));
+    t.hi[j] = make_float4(hi.x, hi.y, hi.z, 0.f);
+    t.rng[j] = make_uint2(l, l);
+  } else {
+    t.lo[j] = node_lo[r];
+    t.hi[j] = node_hi[r];
+    t.rng[j] = range[r];
+  }
+}
+
+// Restructures the treelet under `root`; called by the whole wave, `own` on the root's lane only.
+__device__ inline void treelet_restructure(TreeletWave& t, uint32_t root, bool own, uint32_t lane,
+                                           uint2* children, uint2* range, uint32_t* parent_internal,
+                                           uint32_t* parent_leaf, const float4* leaf_lo, const float4* leaf_hi,
+                                           float4* node_lo, float4* node_hi) {
+  if (own) {
+    const uint2 ch = children[root];
+    t.inner[0] = root;
+    treelet_load(t, 0, ch.x, range, leaf_lo, leaf_hi, node_lo, node_hi);
+    treelet_load(t, 1, ch.y, range, leaf_lo, leaf_hi, node_lo, node_hi);
+    uint32_t k = 2;
+    while (k < (uint32_t)kTreeletLeaves) {
+      int best = -1;
+      float best_area = -1.f;
+      for (uint32_t j = 0; j < k; ++j) {
+        if (t.ref[j] & kBvh2Leaf) continue;
+        const float a = half_area(t.lo[j], t.hi[j]);
+        if (a > best_area) {
+          best_area = a;
+          best = (int)j;
+        }
+      }
+      if (best < 0) break;
+      const uint32_t x = t.ref[best];
+      t.inner[k - 1] = x;
+      const uint2 g = children[x];
+      treelet_load(t, (uint32_t)best, g.x, range, leaf_lo, leaf_hi, node_lo, node_hi);
+      treelet_load(t, k, g.y, range, leaf_lo, leaf_hi, node_lo, node_hi);
+      ++k;
+    }
+    t.k = k;
+  }
+  wave_sync();
+  const uint32_t k = t.k;
+  if (k <= 2u) {  // two triangles: nothing to rearrange, the cost is unchanged
+    wave_sync();
+    return;
+  }
+  const uint32_t full = (1u << k) - 1u;
+  for (uint32_t x = lane + 1u; x <= full; x += 64u)
+    if ((x & (x - 1u)) == 0u) t.cost[x] = t.hi[__builtin_ctz(x)].w;
+  for (uint32_t size = 2; size <= k; ++size) {
+    wave_sync();
+    for (uint32_t x = lane + 1u; x <= full; x += 64u) {
+      if ((uint32_t)__builtin_popcount(x) != size) continue;
+      float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+      for (uint32_t m = x; m; m &= m - 1u) {
+        const uint32_t j = __builtin_ctz(m);
+        const float4 a = t.lo[j], b = t.hi[j];
+        lx = fminf(lx, a.x); ly = fminf(ly, a.y); lz = fminf(lz, a.z);
+        hx = fmaxf(hx, b.x); hy = fmaxf(hy, b.y); hz = fmaxf(hz, b.z);
+      }
+      // splits (p, x \ p) with the lowest leaf of x in p
+      const uint32_t low = x & (0u - x), rest = x ^ low;
+      float best = FLT_MAX;
+      uint32_t best_p = low;
+      uint32_t q = 0;
+      do {
+        const uint32_t p = low | q;
+        const float c = t.cost[p] + t.cost[x ^ p];
+        if (c < best) {
+          best = c;
+          best_p = p;
+        }
+        q = (q - rest) & rest;
+      } while (q != rest);
+      const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+      t.cost[x] = dx * dy + dy * dz + dz * dx + best;
+      t.part[x] = (uint8_t)best_p;
+    }
+  }
+  wave_sync();
+  if (own) {
+    // Rebuild breadth first from the full set; internal nodes take the treelet's old indices.
+    uint32_t head = 0, tail = 1, next_inner = 1;
+    t.q_sub[0] = full;
+    t.q_idx[0] = root;
+    auto lowest = [&](uint32_t x) {
+      uint32_t r = 0xFFFFFFFFu;
+      for (uint32_t m = x; m; m &= m - 1u) r = min(r, t.rng[__builtin_ctz(m)].x);
+      return r;
+    };
+    while (head < tail) {
+      const uint32_t x = t.q_sub[head], idx = t.q_idx[head];
+      ++head;
+      uint32_t a = t.part[x], b = x ^ a;
+      if (lowest(a) > lowest(b)) {  // keep the lower leaves on the left (Morton order)
+        const uint32_t w = a;
+        a = b;
+        b = w;
+      }
+      auto child = [&](uint32_t y) {
+        uint32_t r;
+        if ((y & (y - 1u)) == 0u) {
+          r = t.ref[__builtin_ctz(y)];
+        } else {
+          r = t.inner[next_inner++];
+          t.q_sub[tail] = y;
+          t.q_idx[tail] = r;
+          ++tail;
+        }
+        if (r & kBvh2Leaf) parent_leaf[r & ~kBvh2Leaf] = idx;
+        else parent_internal[r] = idx;
+        return r;
+      };
+      const uint32_t ra = child(a);
+      const uint32_t rb = child(b);
+      children[idx] = make_uint2(ra, rb);
+      float4 lo = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.f), hi = make_float4(-FLT_MAX, -FLT_MAX, -FLT_MAX, 0.f);
+      uint32_t tris = 0, rmin = 0xFFFFFFFFu, rmax = 0;
+      for (uint32_t m = x; m; m &= m - 1u) {
+        const uint32_t j = __builtin_ctz(m);
+        const float4 l = t.lo[j], h = t.hi[j];
+        lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), 0.f);
+        hi = make_float4(fmaxf(hi.x, h.x), fmaxf(hi.y, h.y), fmaxf(hi.z, h.z), 0.f);
+        tris += __float_as_uint(l.w);
+        rmin = min(rmin, t.rng[j].x);
+        rmax = max(rmax, t.rng[j].y);
+      }
+      node_lo[idx] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(tris));
+      node_hi[idx] = make_float4(hi.x, hi.y, hi.z, t.cost[x]);
+      range[idx] = make_uint2(rmin, rmax);
+    }
+  }
+  wave_sync();
+}
+
+__global__ __launch_bounds__(kTreeletBlock) void treelet_kernel(uint2* children, uint2* range,
+                                                                uint32_t* parent_internal, uint32_t* parent_leaf,
+                                                                uint32_t* visits, const float4* leaf_lo,
+                                                                const float4* leaf_hi, float4* node_lo,
+                                                                float4* node_hi, int n, uint32_t min_size) {
+  __shared__ TreeletWave s_wave[kTreeletBlock / 64];
+  TreeletWave& t = s_wave[threadIdx.x / 64u];
+  const uint32_t lane = threadIdx.x % 64u;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t wg_first = blockIdx.x * blockDim.x, wg_last = wg_first + blockDim.x - 1u;
+  bool active = i < n;
+  uint32_t node = active ? parent_leaf[i] : 0u;
+  while (__ballot(active) != 0ull) {
+    bool second = false;
+    if (active) {
+      // The arrival counters work as in the refit: a node whose leaves all lie in this workgroup's
+      // range sees both arrivals here (the range bounds the leaves, contiguous or not).
+      const uint2 rg = range[node];
+      const bool local = rg.x >= wg_first && rg.y <= wg_last;
+      const uint32_t prior = local ? __hip_atomic_fetch_add(&visits[node], 1u, __ATOMIC_ACQ_REL,
+                                                            __HIP_MEMORY_SCOPE_WORKGROUP)
+                                   : __hip_atomic_fetch_add(&visits[node], 1u, __ATOMIC_ACQ_REL,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+      if (prior == 0u) active = false;
+      else second = true;
+    }
+    const bool big = second && __float_as_uint(node_lo[node].w) >= min_size;
+    for (uint64_t m = __ballot(big); m; m &= m - 1ull) {
+      const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+      const uint32_t root = (uint32_t)__shfl((int)node, (int)leader);
+      treelet_restructure(t, root, lane == leader, lane, children, range, parent_internal, parent_leaf, leaf_lo,
+                          leaf_hi, node_lo, node_hi);
+    }
+    if (second) {
+      if (node == 0u) active = false;
+      else node = parent_internal[node];
+    }
+  }
+}
+
+// Leaf order after restructuring: a node's leaves become the contiguous run of a depth-first
+// (left child first) order. Element e < n is triangle leaf e, e >= n internal node e - n; each
+// walks to the root adding the triangle counts of the left siblings on its path.
+__global__ void treelet_layout_kernel(const uint2* __restrict__ children, const uint32_t* __restrict__ parent_internal,
+                                      const uint32_t* __restrict__ parent_leaf, const float4* __restrict__ node_lo,
+                                      uint32_t n, uint32_t* __restrict__ leaf_pos, uint2* __restrict__ range,
+                                      uint32_t* __restrict__ error) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2u * n - 1u) return;
+  const bool leaf = e < n;
+  const uint32_t x = leaf ? e : e - n;
+  if (!leaf && x == 0u) {
+    range[0] = make_uint2(0u, n - 1u);
+    return;
+  }
+  uint32_t cur = leaf ? (kBvh2Leaf | x) : x;
+  uint32_t p = leaf ? parent_leaf[x] : parent_internal[x];
+  uint32_t off = 0;
+  for (uint32_t step = 0;; ++step) {
+    if (step == kMaxBvh2Depth) {  // not a tree: the host fails the build
+      atomicOr(error, 2u);
+      return;
+    }
+    const uint2 ch = children[p];
+    if (ch.y == cur) off += (ch.x & kBvh2Leaf) ? 1u : __float_as_uint(node_lo[ch.x].w);
+    if (p == 0u) break;
+    cur = p;
+    p = parent_internal[p];
+  }
+  if (leaf) leaf_pos[x] = off;
+  else range[x] = make_uint2(off, off + __float_as_uint(node_lo[x].w) - 1u);
+}
+
+// Renumbers the triangle leaves by their new positions: the sorted-order permutation, the leaves'
+// parents and the internal nodes' leaf children.
+__global__ void treelet_renumber_kernel(uint2* __restrict__ children, const uint32_t* __restrict__ parent_leaf,
+                                        const uint32_t* __restrict__ perm, const uint32_t* __restrict__ leaf_pos,
+                                        uint32_t n, uint32_t* __restrict__ perm_out,
+                                        uint32_t* __restrict__ parent_leaf_out) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) {
+    const uint32_t p = leaf_pos[e];
+    perm_out[p] = perm[e];
+    parent_leaf_out[p] = parent_leaf[e];
+  } else if (e < 2u * n - 1u) {
+    const uint32_t x = e - n;
+    uint2 ch = children[x];
+    if (ch.x & kBvh2Leaf) ch.x = kBvh2Leaf | leaf_pos[ch.x & ~kBvh2Leaf];
+    if (ch.y & kBvh2Leaf) ch.y = kBvh2Leaf | leaf_pos[ch.y & ~kBvh2Leaf];
+    children[x] = ch;
   }
 }
 
@@ -440,11 +729,12 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
 
   // Temporary buffers for the build, released at the end.
   DeviceBuffer codes, perm, alt_k, alt_v, children, range, par_i, par_l, visits, leaf_lo, leaf_hi, node_lo, node_hi,
-      fr_a, fr_b, counters, sah_cost, sah_choice;
+      fr_a, fr_b, counters, sah_cost, sah_choice, leaf_pos, perm_r, par_l_r;
   auto cleanup = [&] {
     (void)hipStreamSynchronize(s);
     for (DeviceBuffer* b : {&codes, &perm, &alt_k, &alt_v, &children, &range, &par_i, &par_l, &visits, &leaf_lo,
-                            &leaf_hi, &node_lo, &node_hi, &fr_a, &fr_b, &counters, &sah_cost, &sah_choice})
+                            &leaf_hi, &node_lo, &node_hi, &fr_a, &fr_b, &counters, &sah_cost, &sah_choice,
+                            &leaf_pos, &perm_r, &par_l_r})
       release(ctx, *b);
   };
   try {
@@ -489,10 +779,11 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
                                            static_cast<float4*>(node_hi.ptr), (uint32_t)leaf_max,
                                            static_cast<float4*>(sah_cost.ptr),
                                            static_cast<uint32_t*>(sah_choice.ptr));
-    leaf_tris_kernel<<<blocks(n), 256, 0, s>>>(pos, tri, geoms, sv, n, static_cast<float4*>(out->leaf_tris.ptr));
     SWR_HIP(hipGetLastError());
 
     if ((int)n <= leaf_max || n == 1) {
+      leaf_tris_kernel<<<blocks(n), 256, 0, s>>>(pos, tri, geoms, sv, n, static_cast<float4*>(out->leaf_tris.ptr));
+      SWR_HIP(hipGetLastError());
       // Everything fits one leaf: root with a single child spanning all triangles.
       float4 l, h;
       SWR_HIP(hipMemcpyAsync(&l, leaf_lo.ptr, 16, hipMemcpyDeviceToHost, s));
@@ -519,19 +810,54 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
       return;
     }
 
+    // Treelet restructuring rounds, then the leaves renumbered in the new depth-first order and
+    // the boxes, counts and collapse choices recomputed over it.
+    ensure(ctx, leaf_pos, nn * 4);
+    ensure(ctx, perm_r, nn * 4);
+    ensure(ctx, par_l_r, nn * 4);
+    ensure(ctx, counters, 16);
+    uint32_t* cnt = static_cast<uint32_t*>(counters.ptr);
+    SWR_HIP(hipMemsetAsync(cnt + 3, 0, 4, s));
+    for (uint32_t min_size : kTreeletMinSize) {
+      fill_u32_kernel<<<blocks(ni), 256, 0, s>>>(static_cast<uint32_t*>(visits.ptr), (uint32_t)ni, 0u);
+      treelet_kernel<<<blocks(n, kTreeletBlock), kTreeletBlock, 0, s>>>(
+          static_cast<uint2*>(children.ptr), static_cast<uint2*>(range.ptr), static_cast<uint32_t*>(par_i.ptr),
+          static_cast<uint32_t*>(par_l.ptr), static_cast<uint32_t*>(visits.ptr), static_cast<float4*>(leaf_lo.ptr),
+          static_cast<float4*>(leaf_hi.ptr), static_cast<float4*>(node_lo.ptr), static_cast<float4*>(node_hi.ptr),
+          (int)n, min_size);
+    }
+    treelet_layout_kernel<<<blocks(2 * (size_t)n - 1), 256, 0, s>>>(
+        static_cast<uint2*>(children.ptr), static_cast<uint32_t*>(par_i.ptr), static_cast<uint32_t*>(par_l.ptr),
+        static_cast<float4*>(node_lo.ptr), n, static_cast<uint32_t*>(leaf_pos.ptr), static_cast<uint2*>(range.ptr),
+        cnt + 3);
+    treelet_renumber_kernel<<<blocks(2 * (size_t)n - 1), 256, 0, s>>>(
+        static_cast<uint2*>(children.ptr), static_cast<uint32_t*>(par_l.ptr), sv,
+        static_cast<uint32_t*>(leaf_pos.ptr), n, static_cast<uint32_t*>(perm_r.ptr),
+        static_cast<uint32_t*>(par_l_r.ptr));
+    sv = static_cast<uint32_t*>(perm_r.ptr);
+    fill_u32_kernel<<<blocks(ni), 256, 0, s>>>(static_cast<uint32_t*>(visits.ptr), (uint32_t)ni, 0u);
+    refit_kernel<<<blocks(n, 1024), 1024, 0, s>>>(pos, tri, sv, (int)n, static_cast<uint2*>(children.ptr),
+                                           static_cast<uint2*>(range.ptr), static_cast<uint32_t*>(par_i.ptr),
+                                           static_cast<uint32_t*>(par_l_r.ptr),
+                                           static_cast<uint32_t*>(visits.ptr), static_cast<float4*>(leaf_lo.ptr),
+                                           static_cast<float4*>(leaf_hi.ptr), static_cast<float4*>(node_lo.ptr),
+                                           static_cast<float4*>(node_hi.ptr), (uint32_t)leaf_max,
+                                           static_cast<float4*>(sah_cost.ptr),
+                                           static_cast<uint32_t*>(sah_choice.ptr));
+    leaf_tris_kernel<<<blocks(n), 256, 0, s>>>(pos, tri, geoms, sv, n, static_cast<float4*>(out->leaf_tris.ptr));
+    SWR_HIP(hipGetLastError());
+
     // Level-by-level collapse. counters: [0] node allocator, [1] frontier A count, [2] B count.
     ensure(ctx, fr_a, nn * 8);
     ensure(ctx, fr_b, nn * 8);
-    ensure(ctx, counters, 16);
-    uint32_t init[4] = {2u, 1u, 0u, 0u};  // node 0 is the root; node 1 is left unused
+    uint32_t init[3] = {2u, 1u, 0u};  // node 0 is the root; node 1 is left unused
     uint2 root_item = make_uint2(0u, 0u);
-    SWR_HIP(hipMemcpyAsync(counters.ptr, init, 16, hipMemcpyHostToDevice, s));
+    SWR_HIP(hipMemcpyAsync(counters.ptr, init, 12, hipMemcpyHostToDevice, s));
     SWR_HIP(hipMemcpyAsync(fr_a.ptr, &root_item, 8, hipMemcpyHostToDevice, s));
     Bvh2View view{static_cast<uint2*>(children.ptr),  static_cast<uint2*>(range.ptr),
                   static_cast<float4*>(leaf_lo.ptr),  static_cast<float4*>(leaf_hi.ptr),
                   static_cast<float4*>(node_lo.ptr),  static_cast<float4*>(node_hi.ptr),
                   static_cast<uint32_t*>(sah_choice.ptr)};
-    uint32_t* cnt = static_cast<uint32_t*>(counters.ptr);
     uint2* cur = static_cast<uint2*>(fr_a.ptr);
     uint2* nxt = static_cast<uint2*>(fr_b.ptr);
     int cur_slot = 1, nxt_slot = 2;
@@ -552,6 +878,7 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
     SWR_HIP(hipMemcpyAsync(&num_nodes, cnt, 4, hipMemcpyDeviceToHost, s));
     SWR_HIP(hipMemcpyAsync(&overflow, cnt + 3, 4, hipMemcpyDeviceToHost, s));
     SWR_HIP(hipStreamSynchronize(s));
+    if (overflow & 2u) fail(SWR_E_INVALID, "bvh: restructured binary tree is not a tree");
     if (overflow) fail(SWR_E_INVALID, "bvh: node array too small for the collapsed hierarchy");
     out->num_nodes = num_nodes;
     out->max_depth = depth;
