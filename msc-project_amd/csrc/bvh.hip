@@ -266,7 +266,7 @@ __global__ __launch_bounds__(1024) void refit_kernel(const float4* __restrict__ 
 // second arrival at a node of at least `min_size` triangles grows a treelet under it (starting
 // from the node's two children, the treelet leaf of largest box area is replaced by its children
 // until there are kTreeletLeaves leaves) and rebuilds the treelet's internal nodes in the topology
-// that minimises the sum of internal box areas, by dynamic programming over the 2^7 subsets of
+// that minimises the sum of internal box areas, by dynamic programming over the 2^8 subsets of
 // treelet leaves. The treelet's subtrees are left as they are; its internal nodes reuse their
 // indices. The wave that finds such a node works the subsets lane-parallel; the node's own lane
 // does every global load and store of the treelet, so that its acquiring arrival orders the
@@ -275,26 +275,59 @@ __global__ __launch_bounds__(1024) void refit_kernel(const float4* __restrict__ 
 // Node records while restructuring: node_lo.w = triangle count (bits), node_hi.w = subtree cost,
 // range = (lowest, highest) sorted leaf index under the node, which is no longer a contiguous run;
 // treelet_layout_kernel renumbers the leaves afterwards.
-constexpr int kTreeletLeaves = 7;
+constexpr int kTreeletLeaves = 8;
 constexpr uint32_t kTreeletSets = 1u << kTreeletLeaves;
 constexpr int kTreeletBlock = 1024;
 // Rounds, each with its least treelet-root size in triangles. On the 2.1 M-triangle terrain
-// (closest-hit traversal per 1080p frame; scene commit): none 25.8 ms, 61 ms; one round at 7
-// triangles 23.4 ms, 70 ms; three 22.0 ms, 82 ms; five at 3 triangles 21.6 ms, 101 ms; eight or
-// twelve rounds no faster.
+// (closest-hit traversal per 1080p frame; scene commit), with treelets of 7 leaves: none 25.8 ms,
+// 61 ms; one round at 7 triangles 23.4 ms, 70 ms; three 22.0 ms, 82 ms; five at 3 triangles
+// 21.6 ms, 92 ms; eight or twelve rounds no faster. Treelets of 8 leaves, five rounds at 3:
+// 21.4 ms, 97 ms.
 constexpr uint32_t kTreeletMinSize[] = {3u, 3u, 3u, 3u, 3u};
-// Deepest binary tree the leaf renumbering walks (30-bit Morton codes and a 2^30-triangle index
-// bound a radix tree's depth by 61; each treelet rebuild can only add levels inside its 7 leaves).
+// Deepest binary tree the leaf renumbering walks (one walk per node and triangle, so a chain-like
+// tree would cost depth x triangles). The radix tree is at most 61 levels deep (30-bit codes, ties
+// by a 31-bit index) and restructuring changes the terrain's little; a tree deeper than this
+// fails the build.
 constexpr uint32_t kMaxBvh2Depth = 4096;
+
+// The bit masks of 0..255 ordered by bit count, then by value: those of one count that lie below
+// 2^k (the subsets of k treelet leaves) come first, count[k][s] of them.
+struct SubsetTable {
+  uint8_t mask[256];
+  uint16_t start[10];
+  uint8_t count[9][9];
+};
+constexpr SubsetTable make_subset_table() {
+  SubsetTable t{};
+  uint32_t n = 0;
+  for (uint32_t s = 0; s <= 8; ++s) {
+    t.start[s] = (uint16_t)n;
+    for (uint32_t m = 0; m < 256; ++m) {
+      uint32_t bits = 0;
+      for (uint32_t v = m; v; v &= v - 1u) ++bits;
+      if (bits == s) t.mask[n++] = (uint8_t)m;
+    }
+  }
+  t.start[9] = (uint16_t)n;
+  for (uint32_t k = 0; k <= 8; ++k)
+    for (uint32_t m = 0; m < (1u << k); ++m) {
+      uint32_t bits = 0;
+      for (uint32_t v = m; v; v &= v - 1u) ++bits;
+      ++t.count[k][bits];
+    }
+  return t;
+}
+__constant__ SubsetTable kSubsets = make_subset_table();
+static_assert(kTreeletLeaves <= 8, "the subset table covers treelets of up to 8 leaves");
 
 struct TreeletWave {
   float cost[kTreeletSets];   // least cost of a subset of treelet leaves joined under one node
   uint8_t part[kTreeletSets];  // the subset's best split: the part holding its lowest leaf
-  float4 lo[8], hi[8];        // treelet leaves: box, lo.w = triangles (bits), hi.w = cost
-  uint2 rng[8];
-  uint32_t ref[8];    // treelet leaves (bit 31 = triangle)
-  uint32_t inner[8];  // the treelet's internal node indices, reused by the new topology
-  uint32_t q_sub[8], q_idx[8];
+  float4 lo[kTreeletLeaves], hi[kTreeletLeaves];  // treelet leaves: box, lo.w = triangles, hi.w = cost
+  uint2 rng[kTreeletLeaves];
+  uint32_t ref[kTreeletLeaves];    // treelet leaves (bit 31 = triangle)
+  uint32_t inner[kTreeletLeaves];  // the treelet's internal node indices, reused by the new topology
+  uint32_t q_sub[kTreeletLeaves], q_idx[kTreeletLeaves];
   uint32_t k;
 };
 
@@ -363,34 +396,65 @@ __device__ inline void treelet_restructure(TreeletWave& t, uint32_t root, bool o
   const uint32_t full = (1u << k) - 1u;
   for (uint32_t x = lane + 1u; x <= full; x += 64u)
     if ((x & (x - 1u)) == 0u) t.cost[x] = t.hi[__builtin_ctz(x)].w;
+  // Subsets of one size at a time, each subset's splits shared by a group of g lanes (a power of
+  // two, as many as the subsets of this size leave free), then the group's least cost by xor
+  // shuffles. Equal costs (leaves with one box, as coincident triangles have) keep the most even
+  // split, then the one of smaller bit mask: a total order, so the tree does not depend on the
+  // grouping, and ties do not chain the leaves one below the other.
   for (uint32_t size = 2; size <= k; ++size) {
     wave_sync();
-    for (uint32_t x = lane + 1u; x <= full; x += 64u) {
-      if ((uint32_t)__builtin_popcount(x) != size) continue;
-      float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
-      for (uint32_t m = x; m; m &= m - 1u) {
-        const uint32_t j = __builtin_ctz(m);
-        const float4 a = t.lo[j], b = t.hi[j];
-        lx = fminf(lx, a.x); ly = fminf(ly, a.y); lz = fminf(lz, a.z);
-        hx = fmaxf(hx, b.x); hy = fmaxf(hy, b.y); hz = fmaxf(hz, b.z);
-      }
-      // splits (p, x \ p) with the lowest leaf of x in p
-      const uint32_t low = x & (0u - x), rest = x ^ low;
+    const uint32_t ns = kSubsets.count[k][size], splits = (1u << (size - 1u)) - 1u;
+    uint32_t g = 1;
+    while (2u * g * ns <= 64u && 2u * g <= splits) g *= 2u;
+    const uint32_t c = lane % g;
+    const uint32_t r0 = c * splits / g, r1 = (c + 1u) * splits / g;
+    for (uint32_t first = 0; first < ns; first += 64u / g) {
+      const uint32_t i = first + lane / g;
       float best = FLT_MAX;
-      uint32_t best_p = low;
-      uint32_t q = 0;
-      do {
-        const uint32_t p = low | q;
-        const float c = t.cost[p] + t.cost[x ^ p];
-        if (c < best) {
-          best = c;
-          best_p = p;
+      uint32_t best_p = 0xFFu, best_u = 0xFFu;  // split, its unevenness |2 |p| - size|
+      uint32_t x = 0;
+      if (i < ns) {
+        x = kSubsets.mask[kSubsets.start[size] + i];
+        // splits (p, x \ p) with the lowest leaf of x in p; split r puts bit b of r on the b-th
+        // remaining leaf of x, and the scan steps through the subsets of `rest` in that order
+        const uint32_t low = x & (0u - x), rest = x ^ low;
+        uint32_t q = 0;
+        for (uint32_t m = rest, b = r0; m && b; m &= m - 1u, b >>= 1)
+          if (b & 1u) q |= m & (0u - m);
+        for (uint32_t r = r0; r < r1; ++r) {
+          const uint32_t p = low | q;
+          const float cp = t.cost[p] + t.cost[x ^ p];
+          const uint32_t u = (uint32_t)abs(2 * __builtin_popcount(p) - (int)size);
+          if (cp < best || (cp == best && u < best_u)) {  // r ascends with p: the smaller mask wins
+            best = cp;
+            best_p = p;
+            best_u = u;
+          }
+          q = (q - rest) & rest;
         }
-        q = (q - rest) & rest;
-      } while (q != rest);
-      const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
-      t.cost[x] = dx * dy + dy * dz + dz * dx + best;
-      t.part[x] = (uint8_t)best_p;
+      }
+      for (uint32_t o = g / 2u; o; o >>= 1) {
+        const float ob = __shfl_xor(best, (int)o);
+        const uint32_t op = (uint32_t)__shfl_xor((int)best_p, (int)o);
+        const uint32_t ou = (uint32_t)__shfl_xor((int)best_u, (int)o);
+        if (ob < best || (ob == best && (ou < best_u || (ou == best_u && op < best_p)))) {
+          best = ob;
+          best_p = op;
+          best_u = ou;
+        }
+      }
+      if (i < ns && c == 0u) {
+        float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX, hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+        for (uint32_t m = x; m; m &= m - 1u) {
+          const uint32_t j = __builtin_ctz(m);
+          const float4 a = t.lo[j], b = t.hi[j];
+          lx = fminf(lx, a.x); ly = fminf(ly, a.y); lz = fminf(lz, a.z);
+          hx = fmaxf(hx, b.x); hy = fmaxf(hy, b.y); hz = fmaxf(hz, b.z);
+        }
+        const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+        t.cost[x] = dx * dy + dy * dz + dz * dx + best;
+        t.part[x] = (uint8_t)best_p;
+      }
     }
   }
   wave_sync();
@@ -508,7 +572,7 @@ __global__ void treelet_layout_kernel(const uint2* __restrict__ children, const 
   uint32_t p = leaf ? parent_leaf[x] : parent_internal[x];
   uint32_t off = 0;
   for (uint32_t step = 0;; ++step) {
-    if (step == kMaxBvh2Depth) {  // not a tree: the host fails the build
+    if (step == kMaxBvh2Depth) {  // too deep (or not a tree): the host fails the build
       atomicOr(error, 2u);
       return;
     }
@@ -878,7 +942,8 @@ void build_bvh(swr_context* ctx, const float4* pos, const uint4* tri, const Geom
     SWR_HIP(hipMemcpyAsync(&num_nodes, cnt, 4, hipMemcpyDeviceToHost, s));
     SWR_HIP(hipMemcpyAsync(&overflow, cnt + 3, 4, hipMemcpyDeviceToHost, s));
     SWR_HIP(hipStreamSynchronize(s));
-    if (overflow & 2u) fail(SWR_E_INVALID, "bvh: restructured binary tree is not a tree");
+    if (overflow & 2u)
+      fail(SWR_E_INVALID, "bvh: restructured binary tree deeper than " + std::to_string(kMaxBvh2Depth) + " levels");
     if (overflow) fail(SWR_E_INVALID, "bvh: node array too small for the collapsed hierarchy");
     out->num_nodes = num_nodes;
     out->max_depth = depth;

@@ -95,6 +95,9 @@ def test_restructured_tree_edge_cases(ctx, case):
         sc = swr.Scene(ctx, desc)
         info = sc.bvh_info()
         assert info["valid"], (case, info)
+        if case == "coincident":  # every split costs the same: even splits keep the tree shallow
+            n = desc["meshes"][0]["idx"].shape[0]
+            assert info["max_depth"] <= 2 * int(np.ceil(np.log2(n))), (n, info)
         oc = orc.Scene(desc)
         rays = random_rays(rng, 8000)
         g = sc.intersect(rays)
