@@ -1499,10 +1499,10 @@ int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc
       d.spp = std::max(1, d.spp / 16);
     else
       d.base = std::max(1, d.base / 4), d.spp = 0;
-    auto time_it = [&](const swr::Tuning& t) {
+    auto time_it = [&](const swr::Tuning& t, int reps = 3) {
       ctx->tuning = t;
       double best = 1e30;
-      for (int rep = 0; rep < 3; ++rep) {
+      for (int rep = 0; rep < reps; ++rep) {
         swr_stats st{};
         swr::render_frame(ctx, scene, d, nullptr, nullptr, &st);
         best = std::min(best, st.ms_total);
@@ -1538,10 +1538,20 @@ int swr_autotune(swr_context* ctx, const swr_scene* scene, const swr_render_desc
       t.hit_key_bits = hb;
       consider(t);
     }
-    for (int ring : {8, 16, 32, 64}) {
-      swr::Tuning t = best;
-      t.trace_stack = ring;
-      consider(t);
+    // No stack ring is cheaper than another by construction, so the fastest one is kept (best of
+    // five runs each), without the noise margin: with the margin the first ring that beat the
+    // default stayed even where a later one was faster (terrain: ring 8 kept, 16 faster by 1.5%
+    // of closest-hit traversal on the full frame).
+    {
+      swr::Tuning pick = best;
+      double pick_ms = time_it(best, 5);
+      for (int ring : {8, 16, 32, 64}) {
+        swr::Tuning t = best;
+        t.trace_stack = ring;
+        const double ms = time_it(t, 5);
+        if (ms < pick_ms) pick_ms = ms, pick = t;
+      }
+      best = pick;
     }
     ctx->tuning = best;
     if (out) swr_get_tuning(ctx, out);
