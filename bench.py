@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--serve-state-gib", type=float, default=32.0,
                     help="device memory budget for request state (per-sample arenas + framebuffers)")
     ap.add_argument("--batch-tuning", default=None,
-                    help="batched workload: fixed launch shapes (as --tuning, ring 0 = per scene by depth) "
+                    help="batched workload: fixed launch shapes (as --tuning, ring 0 = per scene from its BVH depth, at most 16) "
                          "instead of the batch autotuner")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="accepted for older profiling commands: the copy of known size (gather_kernel<4>, "

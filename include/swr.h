@@ -146,7 +146,8 @@ typedef struct {
   int32_t hit_key_bits;          /* top bits of the hit key the hit sort orders by (0 = all bits) */
   int32_t trace_stack_entries;   /* LDS ring of the traversal stack per ray: 8, 16, 32 or 64 entries, deeper
                                     entries spill to device memory (0 = per scene: the ring
-                                    swr_batch_autotune chose for it, else sized from its BVH depth) */
+                                    swr_batch_autotune chose for it, else sized from its BVH depth,
+                                    at most 16) */
   int32_t sort_algorithm;        /* SWR_SORT_ONESWEEP, SWR_SORT_REDUCE_SCAN or SWR_SORT_BUCKET */
 } swr_tuning;
 
@@ -264,7 +265,8 @@ int swr_intersect(swr_context* ctx, const swr_scene* scene, const float* rays_de
 /* rtcOccluded (Integrator.cpp:153-174): flags_dev[i] = 1 if any hit in (tmin, tmax). */
 int swr_occluded(swr_context* ctx, const swr_scene* scene, const float* rays_dev, size_t n, uint32_t mask,
                  uint32_t* flags_dev);
-/* Traversal stack ring of this scene (0 = sized from its BVH depth); swr_batch_autotune sets it.
+/* Traversal stack ring of this scene (0 = sized from its BVH depth, at most 16); swr_batch_autotune
+ * sets it.
  * The context's trace_stack_entries, when nonzero, overrides it. Rings: 0, 8, 16, 32 or 64. */
 int swr_scene_set_stack_ring(swr_scene* scene, int32_t ring);
 int swr_scene_stack_ring(const swr_scene* scene, int32_t* ring);
