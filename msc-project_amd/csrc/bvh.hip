@@ -16,7 +16,7 @@
 //   7. level by level, each BVH4 node adopts the up to four BVH2 descendants those choices name;
 //      subtrees the choices make leaves (<= leaf_max triangles) are contiguous runs of the
 //      triangle order;
-//   8. the leaf triangles are written in that order as (v0|id, e1|mask, e2).
+//   8. the leaf triangles are written in that order as (v0|id, e1|mask, e2|material).
 #include <algorithm>
 #include <cfloat>
 
@@ -703,7 +703,8 @@ __global__ void collapse_kernel(Bvh2View v, const uint2* __restrict__ frontier, 
   nodes[item.x] = nd;
 }
 
-// Leaf triangles in sorted order: v0 (w = global id), e1 = v0 - v1 (w = geometry mask), e2 = v2 - v0.
+// Leaf triangles in sorted order: v0 (w = global id), e1 = v0 - v1 (w = geometry mask), e2 = v2 - v0
+// (w = the geometry's material word: kLeafLight | light index for a light, else its shader index).
 __global__ void leaf_tris_kernel(const float4* __restrict__ pos, const uint4* __restrict__ tri,
                                  const GeomRec* __restrict__ geoms, const uint32_t* __restrict__ perm, uint32_t n,
                                  float4* __restrict__ out) {
@@ -713,10 +714,11 @@ __global__ void leaf_tris_kernel(const float4* __restrict__ pos, const uint4* __
   const uint4 ix = tri[t];
   float3 a, b, c;
   tri_verts(pos, ix, &a, &b, &c);
-  const uint32_t mask = geoms[ix.w].mask;
+  const GeomRec g = geoms[ix.w];
+  const uint32_t material = g.light >= 0 ? kLeafLight | (uint32_t)g.light : (uint32_t)g.shader;
   out[3 * (size_t)j + 0] = make_float4(a.x, a.y, a.z, __uint_as_float(t));
-  out[3 * (size_t)j + 1] = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, __uint_as_float(mask));
-  out[3 * (size_t)j + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, __uint_as_float(ix.w));
+  out[3 * (size_t)j + 1] = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, __uint_as_float(g.mask));
+  out[3 * (size_t)j + 2] = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, __uint_as_float(material));
 }
 
 inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }

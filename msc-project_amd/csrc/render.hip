@@ -333,7 +333,9 @@ struct ShadeOut {
   uint32_t* block_min;   // per workgroup: smallest admission number among its continuations
 };
 
-// Workgroup sizes of the shading kernel: one job (62 VGPRs) fits two 1024-lane workgroups per CU.
+// Workgroup sizes of the shading kernel: one untextured job is held to 64 VGPRs, so that two
+// 1024-lane workgroups (8 waves per SIMD) share a CU (at 66 VGPRs and 7 waves shading took 10.1
+// instead of 9.1 ms per terrain frame).
 // The multi-job variant holds its lanes' job constants in vector registers; it is held to 80 VGPRs
 // (6 waves per SIMD), so that three 512-lane workgroups share a CU.
 constexpr int kShadeBlock = 1024;
@@ -352,7 +354,7 @@ constexpr int kShadeBlockMulti = 512;
 // kTex: some scene of the loop has textured shaders (a separate instance, so that untextured
 // loops do not carry the texture code's registers and stack).
 template <bool kOne, int kBlock, bool kTex>
-__global__ __launch_bounds__(kBlock, kOne ? 1 : 6) void shade_kernel(const DeviceScene* __restrict__ scenes,
+__global__ __launch_bounds__(kBlock, kOne ? (kTex ? 1 : 8) : 6) void shade_kernel(const DeviceScene* __restrict__ scenes,
                                                             const FrameParams* __restrict__ jobs,
                                                             const uint32_t* __restrict__ perm, WaveState* w,
                                                             const RayRec* __restrict__ q, ShadeOut out) {
@@ -392,10 +394,10 @@ __global__ __launch_bounds__(kBlock, kOne ? 1 : 6) void shade_kernel(const Devic
       const float t = __uint_as_float(rec.id.z);
       const SWR_GLOBAL gf4* lt = global_f4(s.leaf_tris) + 3 * (size_t)leaf;
       const float4 e1 = gload(lt + 1), e2 = gload(lt + 2);
-      const GeomRec g = gload_rec(s.geoms + __float_as_uint(e2.w));
+      const uint32_t material = __float_as_uint(e2.w);  // the geometry's light or shader
       const V3 dn = normalize(v3(rec.d.x, rec.d.y, rec.d.z));
-      if (g.light >= 0) {  // light hit (Integrator.cpp:37-75)
-        const LightRec L = gload_rec(s.lights + g.light);
+      if (material & kLeafLight) {  // light hit (Integrator.cpp:37-75)
+        const LightRec L = gload_rec(s.lights + (material & ~kLeafLight));
         const float c = dot(v3(L.n[0], L.n[1], L.n[2]), dn * -1.f);  // QuadLight::radiance
         const V3 le = c < 0.f ? v3(0.f, 0.f, 0.f) : v3(L.intensity, L.intensity, L.intensity);
         const float pdfa = 1.f / (L.sx * L.sy);
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(kBlock, kOne ? 1 : 6) void shade_kernel(const Devic
           contrib = v3(le.x * mis * w.x, le.y * mis * w.y, le.z * mis * w.z);
         }
       } else {
-        ShaderRec sh = gload_rec(s.shaders + g.shader);
+        ShaderRec sh = gload_rec(s.shaders + material);
         if (kTex && sh.tex_len > 0) {
           // The texture coordinate of the hit (PolygonObject::texture, PolygonObject.cpp:45-63)
           // from the triangle's barycentrics, recomputed as traversal computed them.

@@ -47,6 +47,9 @@ static_assert(sizeof(NodeQ) == 64, "quantised node must be 64 bytes");
 
 constexpr uint32_t kNoChild = 0xFFFFFFFFu;
 constexpr uint32_t kLeafBit = 0x80000000u;
+// Material word of a leaf triangle (w of its third row): this bit and the light index for a light,
+// else the shader index, so that shading needs no geometry-table read.
+constexpr uint32_t kLeafLight = 0x80000000u;
 constexpr float kEmptyBox = 1e30f;
 __host__ __device__ inline uint32_t leaf_code(uint32_t first, uint32_t count) {
   return kLeafBit | ((count - 1u) << 27) | first;
