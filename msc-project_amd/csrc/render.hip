@@ -362,6 +362,7 @@ __global__ __launch_bounds__(kBlock, kOne ? (kTex ? 1 : 8) : 6) void shade_kerne
   __shared__ uint32_t s_min[kBlock / 64];
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = m < w->n;
+  if (m == 0u) w->shadow_n = w->n;  // for this wave's shadow pass (one record per position)
   bool has_c = false, has_s = false, has_n = false;
   V3 contrib = v3(0.f, 0.f, 0.f), srgb = v3(0.f, 0.f, 0.f);
   V3 P = v3(0.f, 0.f, 0.f), sdir = v3(0.f, 0.f, 0.f), ndir = v3(0.f, 0.f, 0.f), nw = v3(0.f, 0.f, 0.f);
@@ -533,7 +534,7 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const DeviceScene* 
   __shared__ uint32_t s_ring[kShort * kTraceBlock];
   __shared__ uint32_t s_scenes[2];
   const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
-  const uint32_t n = w->n;
+  const uint32_t n = w->shadow_n;
   if (blockIdx.x * kTraceBlock >= n) return;
   const float4* sr = shadow + 3 * (size_t)i;
   float4 d = make_float4(0.f, 0.f, 0.f, -1.f);
@@ -895,10 +896,15 @@ int bits_for(uint64_t max_value) {
 
 // ---- WaveLoop --------------------------------------------------------------------------------
 void LoopBuffers::release(swr_context* ctx) {
+  if (side) (void)hipStreamSynchronize(side);
   for (DeviceBuffer* b : {&queue[0], &queue[1], &keys[0], &keys[1], &hit_keys, &perm, &alt_k, &alt_v, &shadow, &spill,
-                          &jobs, &ranges, &scene_tab, &scene_ring, &wave, &block_min, &radiance, &sample_xy, &pixels,
-                          &staging})
+                          &spill_shadow, &jobs, &ranges, &scene_tab, &scene_ring, &wave, &block_min, &radiance,
+                          &sample_xy, &pixels, &staging})
     swr::release(ctx, *b);
+  for (hipEvent_t* e : {&shade_ev[0], &shade_ev[1], &shadow_ev[0], &shadow_ev[1]})
+    if (*e) (void)hipEventDestroy(*e), *e = nullptr;
+  if (side) (void)hipStreamDestroy(side);
+  side = nullptr;
   if (host_state) (void)hipHostFree(host_state);
   host_state = nullptr;
   for (auto& e : snap_ev)
@@ -998,6 +1004,7 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
 
 WaveLoop::~WaveLoop() {
   (void)hipStreamSynchronize(stream_);  // replays may still be in flight
+  if (b_.side) (void)hipStreamSynchronize(b_.side);  // and a shadow pass of the run-ahead loop
   for (int c = 0; c < 2; ++c) {
     if (exec_[c]) (void)hipGraphExecDestroy(exec_[c]);
     if (graph_[c]) (void)hipGraphDestroy(graph_[c]);
@@ -1052,7 +1059,8 @@ void WaveLoop::retire_ranges(uint64_t cursor) {
   }
 }
 
-void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times, int ev_set) {
+void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times, int ev_set,
+                            bool overlap) {
   hipStream_t st = stream_;
   const uint32_t nb = exact ? n : cap_, kb = exact ? k : cap_;
   WaveState* w = static_cast<WaveState*>(b_.wave.ptr);
@@ -1093,6 +1101,19 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
     uint32_t *rk = nullptr, *rperm = nullptr;
     sort_pairs(ctx_, keys[cur], perm, alt_k, alt_v, nb, ray_bits_, true, &rk, &rperm, st, n_dev);
     mark(2);
+    if (overlap && shadow_deferred_) {
+      // The previous wave's shadow pass touches only the shadow records, per-sample radiance and
+      // a stack spill: on the second stream it uses a spill of its own, and this wave's kernels up
+      // to its shading write none of the rest (its camera rays start samples of their own). It
+      // starts with this wave's traversal, so its workgroups fill the traversal's tail.
+      SWR_HIP(hipEventRecord(b_.shade_ev[deferred_cur_], st));
+      SWR_HIP(hipStreamWaitEvent(b_.side, b_.shade_ev[deferred_cur_], 0));
+      launch_shadow(b_.side, static_cast<uint32_t*>(b_.spill_shadow.ptr), cap_, times, deferred_set_);
+      SWR_HIP(hipEventRecord(b_.shadow_ev[deferred_cur_], b_.side));
+      shadow_deferred_ = false;
+      shadow_pending_ = true;
+      shadow_last_ = deferred_cur_;
+    }
     const uint32_t scene_shift = (uint32_t)(ray_bits_ - scene_bits_);  // used only with scene_bits_ > 0
     for (int ring : ring_set_) {
       with_ring(ring, [&](auto kShort) {
@@ -1141,6 +1162,8 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
     so.shadow = static_cast<float4*>(b_.shadow.ptr);
     so.radiance = radiance_;
     so.block_min = static_cast<uint32_t*>(b_.block_min.ptr);
+    // The previous wave's shadow pass adds to the same samples: it goes first.
+    if (overlap && shadow_pending_) SWR_HIP(hipStreamWaitEvent(st, b_.shadow_ev[shadow_last_], 0));
     if (one_job && textured_)
       shade_kernel<true, kShadeBlock, true><<<shade_blocks, kShadeBlock, 0, st>>>(stab, jobs, hperm, w, queue[cur], so);
     else if (one_job)
@@ -1152,20 +1175,15 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
       shade_kernel<false, kShadeBlockMulti, false><<<shade_blocks, kShadeBlockMulti, 0, st>>>(stab, jobs, hperm, w,
                                                                                               queue[cur], so);
     mark(5);
-    for (int ring : ring_set_) {
-      with_ring(ring, [&](auto kShort) {
-        if (filter)
-          shadow_kernel<kShort.value, true, false><<<trace_grid, kTraceBlock, 0, st>>>(
-              stab, sring, ring, w, so.shadow, radiance_, spill, spill_stride_);
-        else if (one_scene)
-          shadow_kernel<kShort.value, false, true><<<trace_grid, kTraceBlock, 0, st>>>(
-              stab, sring, ring, w, so.shadow, radiance_, spill, spill_stride_);
-        else
-          shadow_kernel<kShort.value, false, false><<<trace_grid, kTraceBlock, 0, st>>>(
-              stab, sring, ring, w, so.shadow, radiance_, spill, spill_stride_);
-      });
+    if (overlap) {
+      // Launched by the next wave once its rays are sorted, beside its traversal, or by
+      // flush_shadow after the last wave.
+      shadow_deferred_ = true;
+      deferred_cur_ = cur;
+      deferred_set_ = ev_set;
+    } else {
+      launch_shadow(st, spill, nb, times, ev_set);
     }
-    mark(6);
   }
   wave_end_kernel<<<1, 1024, 0, st>>>(static_cast<const uint32_t*>(b_.block_min.ptr), nb ? shade_blocks : 0u, ranges,
                                       mask, w);
@@ -1193,8 +1211,35 @@ void WaveLoop::run_paced(StageTimes* times) {
   }
 }
 
+void WaveLoop::launch_shadow(hipStream_t sst, uint32_t* sspill, uint32_t nb, StageTimes* times, int ev_set) {
+  WaveState* w = static_cast<WaveState*>(b_.wave.ptr);
+  const DeviceScene* stab = static_cast<const DeviceScene*>(b_.scene_tab.ptr);
+  const uint8_t* sring = static_cast<const uint8_t*>(b_.scene_ring.ptr);
+  const float4* shadow = static_cast<const float4*>(b_.shadow.ptr);
+  const bool filter = ring_set_.size() > 1, one_scene = scenes_.size() == 1;
+  // The pass reads its wave's size (shadow_n) on the device; the grid covers nb positions.
+  const uint32_t grid = checked_grid(blocks(nb, kTraceBlock), kTraceBlock, "shadow");
+  if (times) SWR_HIP(hipEventRecord(b_.stage_ev[ev_set][7], sst));
+  for (int ring : ring_set_) {
+    with_ring(ring, [&](auto kShort) {
+      if (filter)
+        shadow_kernel<kShort.value, true, false><<<grid, kTraceBlock, 0, sst>>>(stab, sring, ring, w, shadow,
+                                                                                radiance_, sspill, spill_stride_);
+      else if (one_scene)
+        shadow_kernel<kShort.value, false, true><<<grid, kTraceBlock, 0, sst>>>(stab, sring, ring, w, shadow,
+                                                                                radiance_, sspill, spill_stride_);
+      else
+        shadow_kernel<kShort.value, false, false><<<grid, kTraceBlock, 0, sst>>>(stab, sring, ring, w, shadow,
+                                                                                 radiance_, sspill, spill_stride_);
+    });
+  }
+  if (times) SWR_HIP(hipEventRecord(b_.stage_ev[ev_set][6], sst));
+  SWR_HIP(hipGetLastError());
+}
+
 void WaveLoop::add_stage_times(StageTimes* times, int ev_set) {
   if (!times) return;
+  SWR_HIP(hipEventSynchronize(b_.stage_ev[ev_set][6]));  // the shadow pass may run on the second stream
   auto ms = [&](int a, int b) {
     float v = 0.f;
     SWR_HIP(hipEventElapsedTime(&v, b_.stage_ev[ev_set][a], b_.stage_ev[ev_set][b]));
@@ -1205,20 +1250,41 @@ void WaveLoop::add_stage_times(StageTimes* times, int ev_set) {
   times->trace += ms(2, 3);
   times->sort_hits += ms(3, 4);
   times->shade += ms(4, 5);
-  times->shadow += ms(5, 6);
+  times->shadow += ms(7, 6);  // the shadow pass's own launches (it may run on the second stream)
+}
+
+void WaveLoop::drain_side() {
+  if (shadow_pending_) SWR_HIP(hipStreamWaitEvent(stream_, b_.shadow_ev[shadow_last_], 0));
+  shadow_pending_ = false;
+}
+
+void WaveLoop::flush_shadow(StageTimes* times) {
+  if (shadow_deferred_)
+    launch_shadow(stream_, static_cast<uint32_t*>(b_.spill.ptr), cap_, times, deferred_set_);
+  shadow_deferred_ = false;
+  drain_side();
 }
 
 void WaveLoop::run_ahead(StageTimes* times) {
+  if (!b_.side) {
+    SWR_HIP(hipStreamCreateWithFlags(&b_.side, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&b_.shade_ev[0], &b_.shade_ev[1], &b_.shadow_ev[0], &b_.shadow_ev[1]})
+      SWR_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  }
+  if (spill_) ensure(ctx_, b_.spill_shadow, 4ull * cap_ * spill_stride_);
+  shadow_pending_ = shadow_deferred_ = false;
   int pending = -1, pending_set = 0;
   for (uint64_t wave = 0;; ++wave) {
     if (ctx_->cancel.load()) {
+      shadow_deferred_ = false;
+      drain_side();
       SWR_HIP(hipStreamSynchronize(stream_));
       fail(SWR_E_CANCELLED, "render cancelled");
     }
     // Sorts sized on the device skip their own epoch check.
     sort_epoch_guard(ctx_, 2ull * (uint64_t)passes_per_wave_, stream_);
     const int set = (int)(wave % 3);
-    enqueue_wave(cur_, false, cap_, cap_, times, set);
+    enqueue_wave(cur_, false, cap_, cap_, times, set, true);
     cur_ ^= 1;
     const int t = snapshot();
     if (pending >= 0) {
@@ -1233,6 +1299,7 @@ void WaveLoop::run_ahead(StageTimes* times) {
     pending = t;
     pending_set = set;
   }
+  flush_shadow(times);
   SWR_HIP(hipStreamSynchronize(stream_));
   add_stage_times(times, pending_set);  // the empty wave after the last
 }

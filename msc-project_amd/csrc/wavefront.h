@@ -26,6 +26,8 @@ struct alignas(128) WaveState {
   uint32_t done_seq;      // every job with a smaller admission number has finished
   uint32_t max_wave;
   uint32_t live_min;      // smallest admission number among the last wave's continuations
+  uint32_t shadow_n;      // shadow-record positions of the last shaded wave (its size), which its
+                          // shadow pass reads while the next wave may already have begun
   uint64_t raygen_base;   // global path index of the wave's first camera ray
   uint64_t cursor;        // next global path index to generate
   uint64_t total;         // global path indices admitted so far
@@ -54,11 +56,16 @@ struct StageTimes {
 // render keeps one set per context and reuses it frame after frame.
 struct LoopBuffers {
   DeviceBuffer queue[2], keys[2], hit_keys, perm, alt_k, alt_v, shadow, spill;
+  DeviceBuffer spill_shadow;  // the shadow pass's stack spill when it overlaps the next traversal
   DeviceBuffer jobs, ranges, scene_tab, scene_ring, wave, block_min;
   DeviceBuffer radiance, sample_xy, pixels, staging;  // arenas, when the driver keeps them here
   WaveState* host_state = nullptr;                     // pinned snapshot slots
   hipEvent_t snap_ev[8] = {};
   hipEvent_t stage_ev[3][8] = {};  // per-stage events of up to three waves in flight
+  // The run-ahead loop's second stream (default priority): a wave's shadow pass runs there while
+  // the next wave's ray generation, sorts and traversal run on the loop's stream.
+  hipStream_t side = nullptr;
+  hipEvent_t shade_ev[2] = {}, shadow_ev[2] = {};  // per queue parity
   void release(swr_context* ctx);
 };
 
@@ -95,7 +102,11 @@ class WaveLoop {
   // size is known on the host and launches are sized to it; otherwise they are sized for the
   // capacity and read the size on the device (the form captured into graphs).
   // With `times`, per-stage events are recorded into event set `ev_set` (0..2).
-  void enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times, int ev_set = 0);
+  // With `overlap` (run-ahead loop only) the wave's shadow pass goes to the second stream: the
+  // loop's stream goes on to the next wave and waits for it only before that wave's shading,
+  // the first kernel after it that adds to per-sample radiance.
+  void enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times, int ev_set = 0,
+                    bool overlap = false);
   // Adds the stage times of a finished wave recorded into event set `ev_set`.
   void add_stage_times(StageTimes* times, int ev_set);
 
@@ -160,6 +171,15 @@ class WaveLoop {
   hipGraph_t graph_[2] = {nullptr, nullptr};
   int cur_ = 0;
   int passes_per_wave_ = 0;
+  bool shadow_pending_ = false;  // a shadow pass on the second stream not yet waited for
+  int shadow_last_ = 0;          // its queue parity
+  bool shadow_deferred_ = false; // a shaded wave whose shadow pass is not launched yet
+  int deferred_cur_ = 0, deferred_set_ = 0;
+  // Shadow pass of the last shaded wave on stream sst with stack spill sspill, over nb positions
+  // (the wave's size when the host knows it, else the capacity).
+  void launch_shadow(hipStream_t sst, uint32_t* sspill, uint32_t nb, StageTimes* times, int ev_set);
+  void drain_side();             // the loop's stream waits for the shadow pass on the second stream
+  void flush_shadow(StageTimes* times);  // launches a deferred shadow pass, then drain_side
   int snap_next_ = 0;
   std::vector<uint64_t> range_begin_;  // host copy of the range table (path_begin per range)
 };
