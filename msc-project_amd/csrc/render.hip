@@ -598,9 +598,20 @@ __global__ __launch_bounds__(kTraceBlock) void shadow_kernel(const DeviceScene* 
 // After a wave: every job admitted before the oldest job that still has a live path (a
 // continuation in the next queue, or paths not yet generated) has finished -- its last shadow
 // rays were traced in this wave.
+// The smallest admission number of a job that may still receive radiance: one with paths not
+// generated yet, or with rays in the last wave (lagged: its shadow pass runs with the next wave),
+// or else with continuations after the last wave.
+__device__ inline uint32_t unfinished_min(const JobRange* __restrict__ ranges, uint32_t range_mask,
+                                          const WaveState* w, bool lagged) {
+  uint32_t m = lagged ? w->live_min_prev : w->live_min;
+  if (lagged && w->k > 0u) m = min(m, ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, w->raygen_base)].seq);
+  if (w->cursor < w->total) m = min(m, ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, w->cursor)].seq);
+  return m;
+}
+
 __global__ __launch_bounds__(1024) void wave_end_kernel(const uint32_t* __restrict__ block_min, uint32_t nblocks,
                                                         const JobRange* __restrict__ ranges, uint32_t range_mask,
-                                                        WaveState* w) {
+                                                        WaveState* w, bool lagged) {
   __shared__ uint32_t s_min[1024 / 64];
   uint32_t m = 0xFFFFFFFFu;
   for (uint32_t i = threadIdx.x; i < nblocks; i += 1024) m = min(m, block_min[i]);
@@ -610,10 +621,9 @@ __global__ __launch_bounds__(1024) void wave_end_kernel(const uint32_t* __restri
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 0; i < 1024 / 64; ++i) m = min(m, s_min[i]);
+    w->live_min_prev = w->live_min;
     w->live_min = m;
-    if (w->cursor < w->total)
-      m = min(m, ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, w->cursor)].seq);
-    w->done_seq = m;
+    w->done_seq = unfinished_min(ranges, range_mask, w, lagged);
   }
 }
 
@@ -629,11 +639,8 @@ __global__ void admit_kernel(FrameParams* jobs, FrameParams p, JobRange* ranges,
 __global__ void set_range_lo_kernel(WaveState* w, uint32_t range_lo) { w->range_lo = range_lo; }
 
 // Job completion as of now: admissions since the last wave moved the cursor's job.
-__global__ void done_kernel(const JobRange* __restrict__ ranges, uint32_t range_mask, WaveState* w) {
-  uint32_t m = w->live_min;
-  if (w->cursor < w->total)
-    m = min(m, ranges[find_range(ranges, range_mask, w->range_lo, w->num_ranges, w->cursor)].seq);
-  w->done_seq = m;
+__global__ void done_kernel(const JobRange* __restrict__ ranges, uint32_t range_mask, WaveState* w, bool lagged) {
+  w->done_seq = unfinished_min(ranges, range_mask, w, lagged);
 }
 
 __global__ void clear_kernel(float* p, size_t n) {
@@ -997,7 +1004,7 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
   WaveState w{};
   w.cap = cap_;
   w.done_seq = 0;
-  w.live_min = 0xFFFFFFFFu;
+  w.live_min = w.live_min_prev = 0xFFFFFFFFu;
   SWR_HIP(hipMemcpyAsync(b_.wave.ptr, &w, sizeof w, hipMemcpyHostToDevice, stream_));
   SWR_HIP(hipStreamSynchronize(stream_));
 }
@@ -1186,11 +1193,12 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
     }
   }
   wave_end_kernel<<<1, 1024, 0, st>>>(static_cast<const uint32_t*>(b_.block_min.ptr), nb ? shade_blocks : 0u, ranges,
-                                      mask, w);
+                                      mask, w, overlap);
   SWR_HIP(hipGetLastError());
 }
 
 void WaveLoop::run_paced(StageTimes* times) {
+  lagged_ = false;  // the shadow pass runs in line
   WaveState* hs = b_.host_state;
   SWR_HIP(hipMemcpyAsync(hs, b_.wave.ptr, sizeof(WaveState), hipMemcpyDeviceToHost, stream_));
   SWR_HIP(hipStreamSynchronize(stream_));
@@ -1265,13 +1273,18 @@ void WaveLoop::flush_shadow(StageTimes* times) {
   drain_side();
 }
 
-void WaveLoop::run_ahead(StageTimes* times) {
+void WaveLoop::ensure_side() {
   if (!b_.side) {
     SWR_HIP(hipStreamCreateWithFlags(&b_.side, hipStreamNonBlocking));
     for (hipEvent_t* e : {&b_.shade_ev[0], &b_.shade_ev[1], &b_.shadow_ev[0], &b_.shadow_ev[1]})
       SWR_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   if (spill_) ensure(ctx_, b_.spill_shadow, 4ull * cap_ * spill_stride_);
+}
+
+void WaveLoop::run_ahead(StageTimes* times) {
+  ensure_side();
+  lagged_ = true;
   shadow_pending_ = shadow_deferred_ = false;
   int pending = -1, pending_set = 0;
   for (uint64_t wave = 0;; ++wave) {
@@ -1306,18 +1319,27 @@ void WaveLoop::run_ahead(StageTimes* times) {
 
 void WaveLoop::capture() {
   if (exec_[0]) return;
+  ensure_side();  // no allocation inside a capture
+  lagged_ = true;
   const uint64_t epoch_mirror = ctx_->sort.epoch;
   for (int c = 0; c < 2; ++c) {
+    // Each captured wave starts the previous wave's shadow pass on a branch of its own beside
+    // its traversal (the first replay's is empty: shadow_n starts at 0) and defers its own.
+    shadow_deferred_ = true;
+    deferred_cur_ = c ^ 1;
+    shadow_pending_ = false;
     SWR_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
-      enqueue_wave(c, false, cap_, cap_, nullptr);
+      enqueue_wave(c, false, cap_, cap_, nullptr, 0, true);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(stream_, &g);
       if (g) (void)hipGraphDestroy(g);
       ctx_->sort.epoch = epoch_mirror;
+      shadow_deferred_ = shadow_pending_ = false;
       throw;
     }
+    shadow_deferred_ = shadow_pending_ = false;
     SWR_HIP(hipStreamEndCapture(stream_, &graph_[c]));
     SWR_HIP(hipGraphInstantiate(&exec_[c], graph_[c], nullptr, nullptr, 0));
   }
@@ -1338,7 +1360,7 @@ int WaveLoop::snapshot() {
   const int t = snap_next_;
   snap_next_ = (snap_next_ + 1) & 7;
   done_kernel<<<1, 1, 0, stream_>>>(static_cast<const JobRange*>(b_.ranges.ptr), range_cap_ - 1,
-                                    static_cast<WaveState*>(b_.wave.ptr));
+                                    static_cast<WaveState*>(b_.wave.ptr), lagged_);
   SWR_HIP(hipMemcpyAsync(b_.host_state + t, b_.wave.ptr, sizeof(WaveState), hipMemcpyDeviceToHost, stream_));
   SWR_HIP(hipEventRecord(b_.snap_ev[t], stream_));
   return t;

@@ -260,7 +260,8 @@ void retire(swr_server* s, const WaveState& w, uint32_t seq_limit) {
     s->completing.push_back(r);
   }
   s->active.swap(still);
-  s->loop->retire_ranges(w.cursor);
+  // Job completion still looks up the job of the last wave's first camera ray.
+  s->loop->retire_ranges(w.raygen_base);
   std::lock_guard<std::mutex> lk(s->mu);
   s->last_cursor = w.cursor;
   s->stats.camera_rays = w.camera_rays;

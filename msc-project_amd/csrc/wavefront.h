@@ -28,6 +28,7 @@ struct alignas(128) WaveState {
   uint32_t live_min;      // smallest admission number among the last wave's continuations
   uint32_t shadow_n;      // shadow-record positions of the last shaded wave (its size), which its
                           // shadow pass reads while the next wave may already have begun
+  uint32_t live_min_prev; // live_min of the wave before the last
   uint64_t raygen_base;   // global path index of the wave's first camera ray
   uint64_t cursor;        // next global path index to generate
   uint64_t total;         // global path indices admitted so far
@@ -174,6 +175,10 @@ class WaveLoop {
   bool shadow_pending_ = false;  // a shadow pass on the second stream not yet waited for
   int shadow_last_ = 0;          // its queue parity
   bool shadow_deferred_ = false; // a shaded wave whose shadow pass is not launched yet
+  // Each wave's shadow pass runs with the next wave (run-ahead and graph loops): a job is then
+  // done only once no ray of the last wave belongs to it (its last shadow pass has run).
+  bool lagged_ = false;
+  void ensure_side();            // the second stream, its events and the shadow pass's spill
   int deferred_cur_ = 0, deferred_set_ = 0;
   // Shadow pass of the last shaded wave on stream sst with stack spill sspill, over nb positions
   // (the wave's size when the host knows it, else the capacity).
