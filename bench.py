@@ -219,7 +219,8 @@ def main():
             "rays_per_launch": agg["closest_rays"] // agg["waves"],
             "note": "algorithmic 48 B/ray (ray in + hit out), node traffic excluded; traversal is bound by "
                     "vector-instruction issue at 8 waves per SIMD over chains of node fetches that miss L2 "
-                    "half the time, not by HBM bandwidth",
+                    "half the time, not by HBM bandwidth; each launch shares the device with the previous "
+                    "wave's shadow pass (second stream), which lengthens it (0.72 ms per launch alone)",
             "mrays_s": round(agg["closest_rays"] / (agg["ms_trace"] * 1e-3) / 1e6, 1),
             "l2_hit": None,
         },
