@@ -611,7 +611,7 @@ __device__ inline uint32_t unfinished_min(const JobRange* __restrict__ ranges, u
 
 __global__ __launch_bounds__(1024) void wave_end_kernel(const uint32_t* __restrict__ block_min, uint32_t nblocks,
                                                         const JobRange* __restrict__ ranges, uint32_t range_mask,
-                                                        WaveState* w, bool lagged) {
+                                                        WaveState* w, bool lagged, WaveState* snap) {
   __shared__ uint32_t s_min[1024 / 64];
   uint32_t m = 0xFFFFFFFFu;
   for (uint32_t i = threadIdx.x; i < nblocks; i += 1024) m = min(m, block_min[i]);
@@ -624,6 +624,7 @@ __global__ __launch_bounds__(1024) void wave_end_kernel(const uint32_t* __restri
     w->live_min_prev = w->live_min;
     w->live_min = m;
     w->done_seq = unfinished_min(ranges, range_mask, w, lagged);
+    if (snap) *snap = *w;  // the run-ahead loop's snapshot, straight into host memory
   }
 }
 
@@ -914,6 +915,8 @@ void LoopBuffers::release(swr_context* ctx) {
   side = nullptr;
   if (host_state) (void)hipHostFree(host_state);
   host_state = nullptr;
+  if (host_wave) (void)hipHostFree(host_wave);
+  host_wave = nullptr;
   for (auto& e : snap_ev)
     if (e) (void)hipEventDestroy(e), e = nullptr;
   for (auto& row : stage_ev)
@@ -989,6 +992,7 @@ WaveLoop::WaveLoop(swr_context* ctx, LoopBuffers& bufs, const std::vector<const 
   sort_reserve(ctx, cap_);
   if (!b_.host_state) {
     SWR_HIP(hipHostMalloc((void**)&b_.host_state, 8 * sizeof(WaveState), hipHostMallocDefault));
+    SWR_HIP(hipHostMalloc((void**)&b_.host_wave, 8 * sizeof(WaveState), hipHostMallocCoherent));
     for (auto& e : b_.snap_ev) SWR_HIP(hipEventCreate(&e));
     for (auto& row : b_.stage_ev)
       for (auto& e : row) SWR_HIP(hipEventCreate(&e));
@@ -1067,7 +1071,7 @@ void WaveLoop::retire_ranges(uint64_t cursor) {
 }
 
 void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times, int ev_set,
-                            bool overlap) {
+                            bool overlap, WaveState* snap) {
   hipStream_t st = stream_;
   const uint32_t nb = exact ? n : cap_, kb = exact ? k : cap_;
   WaveState* w = static_cast<WaveState*>(b_.wave.ptr);
@@ -1193,7 +1197,7 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
     }
   }
   wave_end_kernel<<<1, 1024, 0, st>>>(static_cast<const uint32_t*>(b_.block_min.ptr), nb ? shade_blocks : 0u, ranges,
-                                      mask, w, overlap);
+                                      mask, w, overlap, snap);
   SWR_HIP(hipGetLastError());
 }
 
@@ -1297,11 +1301,15 @@ void WaveLoop::run_ahead(StageTimes* times) {
     // Sorts sized on the device skip their own epoch check.
     sort_epoch_guard(ctx_, 2ull * (uint64_t)passes_per_wave_, stream_);
     const int set = (int)(wave % 3);
-    enqueue_wave(cur_, false, cap_, cap_, times, set, true);
+    // The wave's last kernel writes its state into a host slot; an event marks it written.
+    const int t = snap_next_;
+    snap_next_ = (snap_next_ + 1) & 7;
+    enqueue_wave(cur_, false, cap_, cap_, times, set, true, b_.host_wave + t);
+    SWR_HIP(hipEventRecord(b_.snap_ev[t], stream_));
     cur_ ^= 1;
-    const int t = snapshot();
     if (pending >= 0) {
-      const WaveState& s = wait_snapshot(pending);  // the previous wave has finished
+      SWR_HIP(hipEventSynchronize(b_.snap_ev[pending]));  // the previous wave has finished
+      const WaveState& s = b_.host_wave[pending];
       add_stage_times(times, pending_set);
       if (s.cursor == s.total && s.total == total_ && (uint32_t)s.counters == 0) {
         pending = t;

@@ -61,6 +61,7 @@ struct LoopBuffers {
   DeviceBuffer jobs, ranges, scene_tab, scene_ring, wave, block_min;
   DeviceBuffer radiance, sample_xy, pixels, staging;  // arenas, when the driver keeps them here
   WaveState* host_state = nullptr;                     // pinned snapshot slots
+  WaveState* host_wave = nullptr;  // coherent pinned slots the run-ahead loop's wave end writes
   hipEvent_t snap_ev[8] = {};
   hipEvent_t stage_ev[3][8] = {};  // per-stage events of up to three waves in flight
   // The run-ahead loop's second stream (default priority): a wave's shadow pass runs there while
@@ -106,8 +107,9 @@ class WaveLoop {
   // With `overlap` (run-ahead loop only) the wave's shadow pass goes to the second stream: the
   // loop's stream goes on to the next wave and waits for it only before that wave's shading,
   // the first kernel after it that adds to per-sample radiance.
+  // With `snap`, the wave's last kernel also copies the device state there (host memory).
   void enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTimes* times, int ev_set = 0,
-                    bool overlap = false);
+                    bool overlap = false, WaveState* snap = nullptr);
   // Adds the stage times of a finished wave recorded into event set `ev_set`.
   void add_stage_times(StageTimes* times, int ev_set);
 
