@@ -286,8 +286,8 @@ constexpr int kTreeletBlock = 1024;
 constexpr uint32_t kTreeletMinSize[] = {3u, 3u, 3u, 3u, 3u};
 // Deepest binary tree the leaf renumbering walks (one walk per node and triangle, so a chain-like
 // tree would cost depth x triangles). The radix tree is at most 61 levels deep (30-bit codes, ties
-// by a 31-bit index) and restructuring changes the terrain's little; a tree deeper than this
-// fails the build.
+// by a 31-bit index); restructuring keeps it shallow (even splits win ties, and the terrain's
+// BVH4 stays 12 levels deep). A tree deeper than this fails the build.
 constexpr uint32_t kMaxBvh2Depth = 4096;
 
 // The bit masks of 0..255 ordered by bit count, then by value: those of one count that lie below
