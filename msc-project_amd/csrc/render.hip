@@ -909,7 +909,7 @@ void LoopBuffers::release(swr_context* ctx) {
                           &spill_shadow, &jobs, &ranges, &scene_tab, &scene_ring, &wave, &block_min, &radiance,
                           &sample_xy, &pixels, &staging})
     swr::release(ctx, *b);
-  for (hipEvent_t* e : {&shade_ev[0], &shade_ev[1], &shadow_ev[0], &shadow_ev[1]})
+  for (hipEvent_t* e : {&fork_ev[0], &fork_ev[1], &shadow_ev[0], &shadow_ev[1]})
     if (*e) (void)hipEventDestroy(*e), *e = nullptr;
   if (side) (void)hipStreamDestroy(side);
   side = nullptr;
@@ -1117,8 +1117,8 @@ void WaveLoop::enqueue_wave(int cur, bool exact, uint32_t n, uint32_t k, StageTi
       // a stack spill: on the second stream it uses a spill of its own, and this wave's kernels up
       // to its shading write none of the rest (its camera rays start samples of their own). It
       // starts with this wave's traversal, so its workgroups fill the traversal's tail.
-      SWR_HIP(hipEventRecord(b_.shade_ev[deferred_cur_], st));
-      SWR_HIP(hipStreamWaitEvent(b_.side, b_.shade_ev[deferred_cur_], 0));
+      SWR_HIP(hipEventRecord(b_.fork_ev[deferred_cur_], st));
+      SWR_HIP(hipStreamWaitEvent(b_.side, b_.fork_ev[deferred_cur_], 0));
       launch_shadow(b_.side, static_cast<uint32_t*>(b_.spill_shadow.ptr), cap_, times, deferred_set_);
       SWR_HIP(hipEventRecord(b_.shadow_ev[deferred_cur_], b_.side));
       shadow_deferred_ = false;
@@ -1280,7 +1280,7 @@ void WaveLoop::flush_shadow(StageTimes* times) {
 void WaveLoop::ensure_side() {
   if (!b_.side) {
     SWR_HIP(hipStreamCreateWithFlags(&b_.side, hipStreamNonBlocking));
-    for (hipEvent_t* e : {&b_.shade_ev[0], &b_.shade_ev[1], &b_.shadow_ev[0], &b_.shadow_ev[1]})
+    for (hipEvent_t* e : {&b_.fork_ev[0], &b_.fork_ev[1], &b_.shadow_ev[0], &b_.shadow_ev[1]})
       SWR_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   if (spill_) ensure(ctx_, b_.spill_shadow, 4ull * cap_ * spill_stride_);

@@ -67,7 +67,8 @@ struct LoopBuffers {
   // The run-ahead loop's second stream (default priority): a wave's shadow pass runs there while
   // the next wave's ray generation, sorts and traversal run on the loop's stream.
   hipStream_t side = nullptr;
-  hipEvent_t shade_ev[2] = {}, shadow_ev[2] = {};  // per queue parity
+  // per queue parity: where the shadow pass may start (after the next wave's ray sort), its end
+  hipEvent_t fork_ev[2] = {}, shadow_ev[2] = {};
   void release(swr_context* ctx);
 };
 
