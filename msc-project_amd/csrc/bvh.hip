@@ -266,7 +266,7 @@ __global__ __launch_bounds__(1024) void refit_kernel(const float4* __restrict__ 
 // second arrival at a node of at least `min_size` triangles grows a treelet under it (starting
 // from the node's two children, the treelet leaf of largest box area is replaced by its children
 // until there are kTreeletLeaves leaves) and rebuilds the treelet's internal nodes in the topology
-// that minimises the sum of internal box areas, by dynamic programming over the 2^8 subsets of
+// that minimises the sum of internal box areas, by dynamic programming over the 2^9 subsets of
 // treelet leaves. The treelet's subtrees are left as they are; its internal nodes reuse their
 // indices. The wave that finds such a node works the subsets lane-parallel; the node's own lane
 // does every global load and store of the treelet, so that its acquiring arrival orders the
@@ -275,14 +275,14 @@ __global__ __launch_bounds__(1024) void refit_kernel(const float4* __restrict__ 
 // Node records while restructuring: node_lo.w = triangle count (bits), node_hi.w = subtree cost,
 // range = (lowest, highest) sorted leaf index under the node, which is no longer a contiguous run;
 // treelet_layout_kernel renumbers the leaves afterwards.
-constexpr int kTreeletLeaves = 8;
+constexpr int kTreeletLeaves = 9;
 constexpr uint32_t kTreeletSets = 1u << kTreeletLeaves;
 constexpr int kTreeletBlock = 1024;
 // Rounds, each with its least treelet-root size in triangles. On the 2.1 M-triangle terrain
 // (closest-hit traversal per 1080p frame; scene commit), with treelets of 7 leaves: none 25.8 ms,
 // 61 ms; one round at 7 triangles 23.4 ms, 70 ms; three 22.0 ms, 82 ms; five at 3 triangles
 // 21.6 ms, 92 ms; eight or twelve rounds no faster. Treelets of 8 leaves, five rounds at 3:
-// 21.4 ms, 97 ms.
+// 21.4 ms, 97 ms; of 9 leaves: 0.25% less frame time, 123 ms.
 constexpr uint32_t kTreeletMinSize[] = {3u, 3u, 3u, 3u, 3u};
 // Deepest binary tree the leaf renumbering walks (one walk per node and triangle, so a chain-like
 // tree would cost depth x triangles). The radix tree is at most 61 levels deep (30-bit codes, ties
@@ -290,26 +290,26 @@ constexpr uint32_t kTreeletMinSize[] = {3u, 3u, 3u, 3u, 3u};
 // BVH4 stays 12 levels deep). A tree deeper than this fails the build.
 constexpr uint32_t kMaxBvh2Depth = 4096;
 
-// The bit masks of 0..255 ordered by bit count, then by value: those of one count that lie below
+// The bit masks of 0..511 ordered by bit count, then by value: those of one count that lie below
 // 2^k (the subsets of k treelet leaves) come first, count[k][s] of them.
 struct SubsetTable {
-  uint8_t mask[256];
-  uint16_t start[10];
-  uint8_t count[9][9];
+  uint16_t mask[512];
+  uint16_t start[11];
+  uint8_t count[10][10];
 };
 constexpr SubsetTable make_subset_table() {
   SubsetTable t{};
   uint32_t n = 0;
-  for (uint32_t s = 0; s <= 8; ++s) {
+  for (uint32_t s = 0; s <= 9; ++s) {
     t.start[s] = (uint16_t)n;
-    for (uint32_t m = 0; m < 256; ++m) {
+    for (uint32_t m = 0; m < 512; ++m) {
       uint32_t bits = 0;
       for (uint32_t v = m; v; v &= v - 1u) ++bits;
-      if (bits == s) t.mask[n++] = (uint8_t)m;
+      if (bits == s) t.mask[n++] = (uint16_t)m;
     }
   }
-  t.start[9] = (uint16_t)n;
-  for (uint32_t k = 0; k <= 8; ++k)
+  t.start[10] = (uint16_t)n;
+  for (uint32_t k = 0; k <= 9; ++k)
     for (uint32_t m = 0; m < (1u << k); ++m) {
       uint32_t bits = 0;
       for (uint32_t v = m; v; v &= v - 1u) ++bits;
@@ -318,11 +318,11 @@ constexpr SubsetTable make_subset_table() {
   return t;
 }
 __constant__ SubsetTable kSubsets = make_subset_table();
-static_assert(kTreeletLeaves <= 8, "the subset table covers treelets of up to 8 leaves");
+static_assert(kTreeletLeaves <= 9, "the subset table covers treelets of up to 9 leaves");
 
 struct TreeletWave {
   float cost[kTreeletSets];   // least cost of a subset of treelet leaves joined under one node
-  uint8_t part[kTreeletSets];  // the subset's best split: the part holding its lowest leaf
+  uint16_t part[kTreeletSets];  // the subset's best split: the part holding its lowest leaf
   float4 lo[kTreeletLeaves], hi[kTreeletLeaves];  // treelet leaves: box, lo.w = triangles, hi.w = cost
   uint2 rng[kTreeletLeaves];
   uint32_t ref[kTreeletLeaves];    // treelet leaves (bit 31 = triangle)
@@ -369,11 +369,16 @@ __device__ inline void treelet_restructure(TreeletWave& t, uint32_t root, bool o
     while (k < (uint32_t)kTreeletLeaves) {
       int best = -1;
       float best_area = -1.f;
+      uint32_t best_tris = 0;
       for (uint32_t j = 0; j < k; ++j) {
         if (t.ref[j] & kBvh2Leaf) continue;
+        // the largest box; on equal boxes (coincident triangles) the most triangles, so that the
+        // treelet's leaves, and the even splits that win ties over them, stay balanced
         const float a = half_area(t.lo[j], t.hi[j]);
-        if (a > best_area) {
+        const uint32_t tris = __float_as_uint(t.lo[j].w);
+        if (a > best_area || (a == best_area && tris > best_tris)) {
           best_area = a;
+          best_tris = tris;
           best = (int)j;
         }
       }
@@ -411,7 +416,7 @@ __device__ inline void treelet_restructure(TreeletWave& t, uint32_t root, bool o
     for (uint32_t first = 0; first < ns; first += 64u / g) {
       const uint32_t i = first + lane / g;
       float best = FLT_MAX;
-      uint32_t best_p = 0xFFu, best_u = 0xFFu;  // split, its unevenness |2 |p| - size|
+      uint32_t best_p = 0xFFFFu, best_u = 0xFFu;  // split, its unevenness |2 |p| - size|
       uint32_t x = 0;
       if (i < ns) {
         x = kSubsets.mask[kSubsets.start[size] + i];
@@ -453,7 +458,7 @@ __device__ inline void treelet_restructure(TreeletWave& t, uint32_t root, bool o
         }
         const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
         t.cost[x] = dx * dy + dy * dz + dz * dx + best;
-        t.part[x] = (uint8_t)best_p;
+        t.part[x] = (uint16_t)best_p;
       }
     }
   }
