@@ -244,7 +244,7 @@ int swr_scene_add_quad_light(swr_scene* scene, const swr_quad_light_desc* desc);
  * Integrator.cpp:26-27; default 0). */
 int swr_scene_set_environment(swr_scene* scene, const float rgb[3]);
 /* rtcCommit (Pathtracer.cpp:274): uploads the geometry and builds the BVH4 on device (Morton
- * radix tree, treelet restructuring, SAH-optimal collapse; about 100 ms for 2.1 M triangles). */
+ * radix tree, treelet restructuring, SAH-optimal collapse; about 125 ms for 2.1 M triangles). */
 int swr_scene_commit(swr_scene* scene);
 /* Decodes a reference two-document YAML scene file (Pathtracer.cpp:32-275) into a committed
  * scene plus its render description. OBJ paths resolve relative to the YAML file. */
