@@ -174,6 +174,16 @@ def main():
     ms_step = 1000.0 * total_s / args.steps
     host_times, host_stats, pix = steps(True)
     host_s = sum(host_times)
+    # Outside the timed steps: one frame with the shadow pass in line, so that each stage's own
+    # time is on record (in the timed steps the shadow pass overlaps the next wave's traversal).
+    alone = None
+    saved_tuning = ctx.tuning()
+    if saved_tuning.get("shadow_overlap", 0):
+        ctx.set_tuning(**dict(saved_tuning, shadow_overlap=0))
+        scene.render(rd, pixels=False)
+        _, _, st_alone = scene.render(rd, pixels=False)
+        ctx.set_tuning(**saved_tuning)
+        alone = {k[3:]: round(st_alone[k], 3) for k in st_alone if k.startswith("ms_")}
     host_rays = sum(s["closest_rays"] + s["shadow_rays"] for s in host_stats)
 
     agg = {k: sum(s[k] for s in stats) for k in stats[0]}
@@ -242,6 +252,10 @@ def main():
         },
         "rays_per_step": rays // args.steps,
         "stage_ms_per_step": stage_ms,
+        "stage_ms_alone": alone,
+        "stage_note": "each wave's shadow pass runs on a second stream beside the next wave's traversal, "
+                      "so trace and shadow overlap in stage_ms_per_step; stage_ms_alone is one more frame "
+                      "(outside the timed steps) with the pass in line (tuning shadow_overlap 0)",
         "dominant_stage": dominant,
         "waves_per_step": agg["waves"] // args.steps,
         "scene_gen_s": round(gen_s, 2),

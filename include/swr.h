@@ -149,6 +149,9 @@ typedef struct {
                                     swr_batch_autotune chose for it, else sized from its BVH depth,
                                     at most 16) */
   int32_t sort_algorithm;        /* SWR_SORT_ONESWEEP, SWR_SORT_REDUCE_SCAN or SWR_SORT_BUCKET */
+  int32_t shadow_overlap;        /* 1 (swr_create's default): each wave's shadow pass runs on a second
+                                    stream beside the next wave's traversal; 0: in line after its
+                                    shading. The same images either way. */
 } swr_tuning;
 
 /* ---- context (Pathtracer ctor/dtor, Pathtracer.cpp:369-392) ------------------------------ */

@@ -153,6 +153,7 @@ int swr_get_tuning(const swr_context* ctx, swr_tuning* out) {
   out->hit_key_bits = ctx->tuning.hit_key_bits;
   out->trace_stack_entries = ctx->tuning.trace_stack;
   out->sort_algorithm = ctx->tuning.sort_algorithm;
+  out->shadow_overlap = ctx->tuning.shadow_overlap;
   return SWR_OK;
 }
 
@@ -164,10 +165,11 @@ int swr_set_tuning(swr_context* ctx, const swr_tuning* t) {
       !(t->trace_stack_entries == 0 || t->trace_stack_entries == 8 || t->trace_stack_entries == 16 ||
         t->trace_stack_entries == 32 || t->trace_stack_entries == 64) ||
       !(t->sort_algorithm == SWR_SORT_ONESWEEP || t->sort_algorithm == SWR_SORT_REDUCE_SCAN ||
-        t->sort_algorithm == SWR_SORT_BUCKET)) {
+        t->sort_algorithm == SWR_SORT_BUCKET) ||
+      !(t->shadow_overlap == 0 || t->shadow_overlap == 1)) {
     ctx->err =
         "set_tuning: keys per thread must be 8/12/16/20/24, ray key bits 3..32, hit key bits 0..32, "
-        "trace stack entries 0/8/16/32/64, sort algorithm 0/1/2";
+        "trace stack entries 0/8/16/32/64, sort algorithm 0/1/2, shadow overlap 0/1";
     return SWR_E_INVALID;
   }
   ctx->tuning.sort_kpt = k;
@@ -175,6 +177,7 @@ int swr_set_tuning(swr_context* ctx, const swr_tuning* t) {
   ctx->tuning.hit_key_bits = t->hit_key_bits;
   ctx->tuning.trace_stack = t->trace_stack_entries;
   ctx->tuning.sort_algorithm = t->sort_algorithm;
+  ctx->tuning.shadow_overlap = t->shadow_overlap;
   return SWR_OK;
 }
 

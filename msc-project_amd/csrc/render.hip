@@ -1287,8 +1287,9 @@ void WaveLoop::ensure_side() {
 }
 
 void WaveLoop::run_ahead(StageTimes* times) {
-  ensure_side();
-  lagged_ = true;
+  const bool overlap = ctx_->tuning.shadow_overlap != 0;
+  if (overlap) ensure_side();
+  lagged_ = overlap;
   shadow_pending_ = shadow_deferred_ = false;
   int pending = -1, pending_set = 0;
   for (uint64_t wave = 0;; ++wave) {
@@ -1304,7 +1305,7 @@ void WaveLoop::run_ahead(StageTimes* times) {
     // The wave's last kernel writes its state into a host slot; an event marks it written.
     const int t = snap_next_;
     snap_next_ = (snap_next_ + 1) & 7;
-    enqueue_wave(cur_, false, cap_, cap_, times, set, true, b_.host_wave + t);
+    enqueue_wave(cur_, false, cap_, cap_, times, set, overlap, b_.host_wave + t);
     SWR_HIP(hipEventRecord(b_.snap_ev[t], stream_));
     cur_ ^= 1;
     if (pending >= 0) {
@@ -1327,18 +1328,19 @@ void WaveLoop::run_ahead(StageTimes* times) {
 
 void WaveLoop::capture() {
   if (exec_[0]) return;
-  ensure_side();  // no allocation inside a capture
-  lagged_ = true;
+  const bool overlap = ctx_->tuning.shadow_overlap != 0;
+  if (overlap) ensure_side();  // no allocation inside a capture
+  lagged_ = overlap;
   const uint64_t epoch_mirror = ctx_->sort.epoch;
   for (int c = 0; c < 2; ++c) {
     // Each captured wave starts the previous wave's shadow pass on a branch of its own beside
     // its traversal (the first replay's is empty: shadow_n starts at 0) and defers its own.
-    shadow_deferred_ = true;
+    shadow_deferred_ = overlap;
     deferred_cur_ = c ^ 1;
     shadow_pending_ = false;
     SWR_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
-      enqueue_wave(c, false, cap_, cap_, nullptr, 0, true);
+      enqueue_wave(c, false, cap_, cap_, nullptr, 0, overlap);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(stream_, &g);

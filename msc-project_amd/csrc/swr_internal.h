@@ -74,6 +74,7 @@ struct Tuning {
   int hit_key_bits = 0;    // top bits of the hit key sorted (0 = all: full (geomID, primID) order)
   int trace_stack = 0;     // LDS ring entries of the traversal stack (0 = from the BVH depth)
   int sort_algorithm = SWR_SORT_ONESWEEP;  // pass structure of the radix sort
+  int shadow_overlap = 1;  // the shadow pass beside the next wave's traversal (second stream)
 };
 }  // namespace swr
 

@@ -77,7 +77,8 @@ def texture_desc(t: dict, cls=TextureDesc):
 
 class Tuning(C.Structure):
     _fields_ = [("sort_keys_per_thread", C.c_int32), ("ray_key_bits", C.c_int32), ("hit_key_bits", C.c_int32),
-                ("trace_stack_entries", C.c_int32), ("sort_algorithm", C.c_int32)]
+                ("trace_stack_entries", C.c_int32), ("sort_algorithm", C.c_int32),
+                ("shadow_overlap", C.c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -276,8 +277,9 @@ class Context:
         return t.as_dict()
 
     def set_tuning(self, sort_keys_per_thread=16, ray_key_bits=32, hit_key_bits=0, trace_stack_entries=0,
-                   sort_algorithm=0):
-        t = Tuning(sort_keys_per_thread, ray_key_bits, hit_key_bits, trace_stack_entries, sort_algorithm)
+                   sort_algorithm=0, shadow_overlap=1):
+        t = Tuning(sort_keys_per_thread, ray_key_bits, hit_key_bits, trace_stack_entries, sort_algorithm,
+                   shadow_overlap)
         self.check(self.lib.swr_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, scene: "Scene", desc: "RenderDesc") -> dict:

@@ -217,9 +217,10 @@ def test_tuning_changes_speed_not_pixels(ctx):
     before = ctx.tuning()
     ref = sc.render(rd)[0]
     try:
-        for kpt, rb, hb, ring, alg in [(8, 32, 0, 8, 0), (12, 24, 0, 16, 1), (20, 16, 8, 64, 1), (24, 3, 4, 32, 0),
-                                       (16, 32, 0, 16, 2)]:
-            ctx.set_tuning(kpt, rb, hb, ring, alg)
+        for kpt, rb, hb, ring, alg, ov in [(8, 32, 0, 8, 0, 1), (12, 24, 0, 16, 1, 0), (20, 16, 8, 64, 1, 1),
+                                           (24, 3, 4, 32, 0, 0), (16, 32, 0, 16, 2, 1), (16, 32, 0, 16, 2, 0)]:
+            # ov: the shadow pass beside the next wave's traversal (second stream) or in line
+            ctx.set_tuning(kpt, rb, hb, ring, alg, ov)
             np.testing.assert_array_equal(sc.render(rd)[0], ref)
         # With the bucket sort selected, a scene committed now sorts its triangles' 30-bit Morton
         # codes through it (the count is known at launch): the same BVH, the same image.
@@ -228,6 +229,8 @@ def test_tuning_changes_speed_not_pixels(ctx):
         sc2.close()
         with pytest.raises(swr.SwrError):
             ctx.set_tuning(16, 32, 0, 16, 3)
+        with pytest.raises(swr.SwrError):
+            ctx.set_tuning(16, 32, 0, 16, 1, 2)
         with pytest.raises(swr.SwrError):
             ctx.set_tuning(10, 32, 0)
         with pytest.raises(swr.SwrError):
