@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-kernels", action="store_true",
                     help="skip the sort/traversal kernel bench (16M ray keys, 16M hit keys, 16M rays on the terrain BVH)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-stage-alone", action="store_true",
+                    help="skip the extra frame with the shadow pass in line (stage_ms_alone; for kernel profiles)")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shapes instead of the autotuner: keys per thread, ray key bits, hit key bits, "
                          "stack ring entries (e.g. 16,16,16,16)")
@@ -178,7 +180,7 @@ def main():
     # time is on record (in the timed steps the shadow pass overlaps the next wave's traversal).
     alone = None
     saved_tuning = ctx.tuning()
-    if saved_tuning.get("shadow_overlap", 0):
+    if saved_tuning.get("shadow_overlap", 0) and not args.no_stage_alone:
         ctx.set_tuning(**dict(saved_tuning, shadow_overlap=0))
         scene.render(rd, pixels=False)
         _, _, st_alone = scene.render(rd, pixels=False)
